@@ -1,0 +1,124 @@
+/*
+ * ql_dns.h -- C ABI of the MI355X (gfx950) quasilinear shearing-box time step.
+ *
+ * One context (ql_ctx) holds one run: its parameters, its device-resident
+ * state and its HIP stream.  Host buffers passed in are copied and never
+ * retained.  Every entry point returns QL_OK (0) or a negative QL_ERR_* code;
+ * ql_last_error() gives the message of the last failure on the calling
+ * thread.  Calls on one context are not thread-safe (as the reference's
+ * objects are not); separate contexts are independent.
+ *
+ * Host layouts (little-endian doubles, complex = {re, im}):
+ *   lin    [nxy][nLin][nz] complex  -- solution::pLin(i, Vnum), solution.h:31
+ *   mf     [nMF][nz] complex        -- solution::pMF(i), solution.h:32
+ *   linop  [nxy][nLin][nz] double   -- linOpFluct[i][Vnum], Model.h:46-50
+ *   kx0    [nxy] double             -- Im(Kdata::kx[i]) after remaps, Kdata.h:212
+ * with nxy = (NX-1)*NY/2, nz = 1 + 2*int(NZ/3) (Models/Model.h:32-36), mode
+ * order j + ny*i' and kz order (0..nzl, -nzl..-1) (Auxiliary/Kdata.cpp:28-80),
+ * fluctuations stored at NX*NY*NZ x and mean fields at NZ x the Fourier
+ * coefficient, exactly as the reference stores them.
+ */
+#ifndef QL_DNS_H
+#define QL_DNS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QL_OK 0
+#define QL_ERR_ARG (-1)         /* invalid argument or unsupported size      */
+#define QL_ERR_HIP (-2)         /* HIP runtime failure / no device           */
+#define QL_ERR_UNSUPPORTED (-3) /* feature outside this build (e.g. dt < 0)  */
+#define QL_ERR_STATE (-4)       /* non-finite or runaway state detected      */
+
+enum { QL_MODEL_MHD_FULLQLIN = 0, QL_MODEL_MHD_BQLIN = 1 };
+enum { QL_INT_RK3CN = 0, QL_INT_RK3CN_ELIN = 1 };
+enum { QL_PREC_FP64 = 0, QL_PREC_MIXED = 1 };
+
+/* Parameters of one run.  Field meanings follow the reference's Inputs
+ * (Auxiliary/Input_parameters.cpp:68-138); model/integrator select what
+ * main.cpp:58-74 and main.cpp:95 choose. */
+typedef struct ql_params {
+    int model;           /* QL_MODEL_*: equations_to_use_                    */
+    int integrator;      /* QL_INT_*: RK3CN or RK3CN_Elin (main.cpp:95)       */
+    int precision;       /* QL_PREC_*                                        */
+    int NX, NY, NZ;      /* nx_, ny_, nz_ (NZ in {32, 64, 128, 256})          */
+    double L[3];         /* lx_, ly_, lz_                                    */
+    double q, omega;     /* q_, omega_ (MHD_BQlin uses 2q/3, MHD_BQlin.cpp:19)*/
+    double nu, eta;      /* nu_, eta_                                        */
+    double B0z;          /* B0z_                                             */
+    int viscosity_order; /* viscosity_order_                                 */
+    int quasilinear;     /* QuasiLinear?                                     */
+    int remap;           /* Remap? (applied by ql_steps after each step)      */
+    double dt;           /* dt_ (> 0; variable-dt CFL stepping unsupported)   */
+    int device;          /* HIP device ordinal                               */
+} ql_params;
+
+typedef struct ql_ctx ql_ctx;
+
+/* Model constructor + device allocation (replaces MHD_FullQlin::MHD_FullQlin,
+ * Models/MHD_FullQlin.cpp:12-161, and MHD_BQlin::MHD_BQlin, MHD_BQlin.cpp:12-109;
+ * the integrator's linearOPs_Init call is done with t0 = 0). */
+int ql_create(const ql_params *p, ql_ctx **out);
+int ql_destroy(ql_ctx *ctx);
+/* dims[0..5] = nxy (Model::Dimxy, Model.h:56), nz (Model::NZ, :54),
+ * nMF (num_MFs, :57), nLin (num_Lin, :58), half ((nz-1)/2+1), NZ (NZfull, :55) */
+int ql_dims(const ql_ctx *ctx, int dims[6]);
+
+/* solution upload/download (solution::pLin/pMF, solution.h:31-32).  Any
+ * pointer may be NULL to leave that part unchanged / not read back. */
+int ql_set_state(ql_ctx *ctx, const double *lin, const double *mf, const double *kx0);
+int ql_get_state(ql_ctx *ctx, double *lin, double *mf, double *kx0);
+
+/* Model::linearOPs_Init (Models/Model.h:50; MHD_FullQlin.cpp:429-456).
+ * Also primes the integrator at t0 (RK3CN.cpp:57, RK3CN_Elin.cpp:257).
+ * Outputs may be NULL. */
+int ql_linear_ops_init(ql_ctx *ctx, double t0, double *linop, double *linop_mf);
+
+/* Model::rhs(t, dt_lin, SolIn = current state, SolOut, linOpFluct)
+ * (Models/Model.h:46-47; MHD_FullQlin.cpp:171-427; MHD_BQlin.cpp:117-284).
+ * The state is not modified.  Outputs may be NULL. */
+int ql_rhs(ql_ctx *ctx, double t, double dt_lin, double *rhs_lin, double *rhs_mf, double *linop);
+
+/* Integrator::Step(t, sol) (Integrators/Integrator.h:23; RK3CN.cpp:77-201,
+ * RK3CN_Elin.cpp:275-389).  Writes the step size to *dt_out (may be NULL). */
+int ql_step(ql_ctx *ctx, double t, double *dt_out);
+
+/* nsteps of the reference main loop (main.cpp:108-123, noise off, no output):
+ * Step, t += dt (accumulated in double, as main.cpp:116), then
+ * ShearingBox_Remap(q t) if remap is on.  *t_inout is read and updated. */
+int ql_steps(ql_ctx *ctx, int nsteps, double *t_inout);
+
+/* Model::ShearingBox_Remap(qt, sol) (MHD_FullQlin.cpp:464-481). */
+int ql_remap(ql_ctx *ctx, double qt);
+
+/* Model::Calc_Energy_AM_Diss with every save flag on (MHD_FullQlin.cpp:487-717):
+ * out[0..3] energy (MU, MB, u, b), out[4..7] angular momentum, out[8..11]
+ * dissipation, out[12..15] Reynolds projections (MHD_FullQlin only),
+ * out[16..17] etaK*kmax.  Projections use the drive of the last rhs. */
+int ql_diagnostics(ql_ctx *ctx, double t, double out[18]);
+
+/* EMF / Reynolds-stress drives of the last rhs (Bx_drive_, By_drive_,
+ * Ux_drive_, Uy_drive_ of MHD_FullQlin.cpp:389-399): [nMF][nz] complex. */
+int ql_get_drive(ql_ctx *ctx, double *drive);
+
+/* TimeVariables::Save_Mean_Fields values (TimeVariables.cpp:202-228):
+ * fac1D * Re(ifft(pad(MF))) -> [nMF][NZ] doubles. */
+int ql_mf_real(ql_ctx *ctx, double *out);
+
+/* Waits for all work queued on the context's stream. */
+int ql_synchronize(ql_ctx *ctx);
+
+/* Runs nsteps like ql_steps with HIP events around every kernel launch and
+ * returns, per kernel kind (0 mode kernel, 1 partial reduction, 2 mean-field
+ * kernel), the average launch duration in ms and the launch count.  For
+ * measurement only: the events serialise the launches. */
+int ql_profile_kernels(ql_ctx *ctx, int nsteps, double *t_inout, double avg_ms[3], int counts[3]);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ql_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

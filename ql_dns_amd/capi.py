@@ -1,0 +1,191 @@
+"""ctypes binding of the C ABI in include/ql_dns.h (libql_dns.so).
+
+`QLContext` is a thin object over one ``ql_ctx``: every method is one C-ABI
+call.  Loading fails loudly when the library is missing; there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import QLParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libql_dns.so")
+
+# every symbol include/ql_dns.h declares
+EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
+           "ql_linear_ops_init", "ql_rhs", "ql_step", "ql_steps", "ql_remap", "ql_diagnostics",
+           "ql_get_drive", "ql_mf_real", "ql_synchronize", "ql_profile_kernels", "ql_last_error")
+
+
+class QLError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ql error {code}: {msg}")
+        self.code = code
+
+
+class ql_params(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int), ("integrator", ctypes.c_int), ("precision", ctypes.c_int),
+                ("NX", ctypes.c_int), ("NY", ctypes.c_int), ("NZ", ctypes.c_int),
+                ("L", ctypes.c_double * 3), ("q", ctypes.c_double), ("omega", ctypes.c_double),
+                ("nu", ctypes.c_double), ("eta", ctypes.c_double), ("B0z", ctypes.c_double),
+                ("viscosity_order", ctypes.c_int), ("quasilinear", ctypes.c_int),
+                ("remap", ctypes.c_int), ("dt", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libql_dns.so and declare its signatures (raises if absent)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -m ql_dns_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(path)
+    P, D, I = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "ql_create": ([ctypes.POINTER(ql_params), ctypes.POINTER(P)], ctypes.c_int),
+        "ql_destroy": ([P], ctypes.c_int),
+        "ql_dims": ([P, I], ctypes.c_int),
+        "ql_set_state": ([P, D, D, D], ctypes.c_int),
+        "ql_get_state": ([P, D, D, D], ctypes.c_int),
+        "ql_linear_ops_init": ([P, ctypes.c_double, D, D], ctypes.c_int),
+        "ql_rhs": ([P, ctypes.c_double, ctypes.c_double, D, D, D], ctypes.c_int),
+        "ql_step": ([P, ctypes.c_double, D], ctypes.c_int),
+        "ql_steps": ([P, ctypes.c_int, D], ctypes.c_int),
+        "ql_remap": ([P, ctypes.c_double], ctypes.c_int),
+        "ql_diagnostics": ([P, ctypes.c_double, D], ctypes.c_int),
+        "ql_get_drive": ([P, D], ctypes.c_int),
+        "ql_mf_real": ([P, D], ctypes.c_int),
+        "ql_synchronize": ([P], ctypes.c_int),
+        "ql_profile_kernels": ([P, ctypes.c_int, D, D, I], ctypes.c_int),
+        "ql_last_error": ([], ctypes.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def to_c_params(p: QLParams, device: int = 0) -> ql_params:
+    return ql_params(p.model, p.integrator, p.precision, p.NX, p.NY, p.NZ,
+                     (ctypes.c_double * 3)(p.Lx, p.Ly, p.Lz), p.q, p.omega, p.nu, p.eta, p.B0z,
+                     p.viscosity_order, p.quasilinear, p.remap, p.dt, device)
+
+
+class QLContext:
+    """One run on one GPU, driven through the C ABI."""
+
+    def __init__(self, p: QLParams, device: int = 0):
+        self._L = load()
+        self.params = p
+        self._h = ctypes.c_void_p()
+        cp = to_c_params(p, device)
+        self._check(self._L.ql_create(ctypes.byref(cp), ctypes.byref(self._h)))
+        d = (ctypes.c_int * 6)()
+        self._check(self._L.ql_dims(self._h, d))
+        self.nxy, self.nz, self.nMF, self.nLin, self.half, self.NZ = list(d)
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise QLError(rc, self._L.ql_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.ql_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_state(self, lin=None, mf=None, kx0=None):
+        def c(a, dt, n):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            if a.size != n:
+                raise ValueError(f"expected {n} values, got {a.size}")
+            return a
+        lin = c(lin, np.complex128, self.nxy * 4 * self.nz)
+        mf = c(mf, np.complex128, self.nMF * self.nz)
+        kx0 = c(kx0, np.float64, self.nxy)
+        self._check(self._L.ql_set_state(self._h, _dp(lin), _dp(mf), _dp(kx0)))
+
+    def get_state(self):
+        lin = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        mf = np.zeros((self.nMF, self.nz), dtype=np.complex128)
+        kx0 = np.zeros(self.nxy)
+        self._check(self._L.ql_get_state(self._h, _dp(lin), _dp(mf), _dp(kx0)))
+        return lin, mf, kx0
+
+    def linear_ops_init(self, t0: float = 0.0):
+        L = np.zeros((self.nxy, 4, self.nz))
+        Lmf = np.zeros((self.nMF, self.nz))
+        self._check(self._L.ql_linear_ops_init(self._h, t0, _dp(L), _dp(Lmf)))
+        return L, Lmf
+
+    def rhs(self, t: float, dt_lin: float):
+        out = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        omf = np.zeros((self.nMF, self.nz), dtype=np.complex128)
+        L = np.zeros((self.nxy, 4, self.nz))
+        self._check(self._L.ql_rhs(self._h, t, dt_lin, _dp(out), _dp(omf), _dp(L)))
+        return out, omf, L
+
+    def step(self, t: float) -> float:
+        dt = ctypes.c_double(0.0)
+        self._check(self._L.ql_step(self._h, t, ctypes.byref(dt)))
+        return dt.value
+
+    def steps(self, n: int, t: float = 0.0) -> float:
+        tt = ctypes.c_double(t)
+        self._check(self._L.ql_steps(self._h, n, ctypes.byref(tt)))
+        return tt.value
+
+    def remap(self, qt: float):
+        self._check(self._L.ql_remap(self._h, qt))
+
+    def diagnostics(self, t: float):
+        out = np.zeros(18)
+        self._check(self._L.ql_diagnostics(self._h, t, _dp(out)))
+        return out
+
+    def drive(self):
+        d = np.zeros((self.nMF, self.nz), dtype=np.complex128)
+        self._check(self._L.ql_get_drive(self._h, _dp(d)))
+        return d
+
+    def mf_real(self):
+        out = np.zeros((self.nMF, self.NZ))
+        self._check(self._L.ql_mf_real(self._h, _dp(out)))
+        return out
+
+    def synchronize(self):
+        self._check(self._L.ql_synchronize(self._h))
+
+    def profile_kernels(self, n: int, t: float = 0.0):
+        tt = ctypes.c_double(t)
+        avg = np.zeros(3)
+        cnt = (ctypes.c_int * 3)()
+        self._check(self._L.ql_profile_kernels(self._h, n, ctypes.byref(tt), _dp(avg), cnt))
+        return tt.value, avg, list(cnt)

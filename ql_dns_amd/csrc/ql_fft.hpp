@@ -1,0 +1,114 @@
+// ql_fft.hpp -- batched dealiased z-transforms in LDS for gfx950.
+//
+// Replaces the reference's per-line FFTW calls (Auxiliary/fftwPlans.h:33-59:
+// `inverse` = zero-pad nz -> NZ then unnormalised exp(+i) transform, `forward`
+// = unnormalised exp(-i) transform then truncate NZ -> nz).  Here a workgroup
+// transforms NL lines of length NZ at once, in place in LDS, with a Stockham
+// autosort schedule of radix-4 passes (and one radix-2 pass when log2 NZ is
+// odd).  Each pass reads every element of every line into registers, waits at
+// a barrier, and writes the butterflies back, so a single LDS buffer suffices.
+// Twiddles come from a table tw[k] = exp(-2 pi i k / NZ) staged in LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qlk {
+
+constexpr int NT = 256;  // threads per workgroup of every kernel here
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+// (i k) * a
+__device__ __forceinline__ double2 imul(double k, double2 a) { return make_double2(-k * a.y, k * a.x); }
+// Re(a * conj(b))
+__device__ __forceinline__ double redot(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
+
+template <bool INV>
+__device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, double2 &v3)
+{
+    const double2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = csub(v1, v3);
+    // forward: -i * a3, inverse: +i * a3
+    const double2 ja3 = INV ? make_double2(-a3.y, a3.x) : make_double2(a3.y, -a3.x);
+    v0 = cadd(a0, a2);
+    v2 = csub(a0, a2);
+    v1 = cadd(a1, ja3);
+    v3 = csub(a1, ja3);
+}
+
+// One Stockham pass of radix R at stride NS over NL lines held in buf.
+template <int NZ, int NL, bool INV, int R, int NS>
+__device__ __forceinline__ void fft_pass(double2 *buf, const double2 *tw)
+{
+    constexpr int M = NZ / R;
+    constexpr int TOT = NL * M;
+    constexpr int IT = (TOT + NT - 1) / NT;
+    double2 v[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int item = threadIdx.x + it * NT;
+        if (TOT % NT == 0 || item < TOT) {
+            const int line = item / M, j = item % M;
+            const double2 *src = buf + line * NZ + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = src[r * M];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int item = threadIdx.x + it * NT;
+        if (TOT % NT == 0 || item < TOT) {
+            const int line = item / M, j = item % M;
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    double2 w = tw[k * r * (NZ / (NS * R))];
+                    if (INV) w.y = -w.y;
+                    v[it][r] = cmul(v[it][r], w);
+                }
+            }
+            if constexpr (R == 4) {
+                dft4<INV>(v[it][0], v[it][1], v[it][2], v[it][3]);
+            } else {
+                const double2 a = v[it][0], b = v[it][1];
+                v[it][0] = cadd(a, b);
+                v[it][1] = csub(a, b);
+            }
+            double2 *dst = buf + line * NZ + (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * NS] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full length-NZ transform of NL lines (INV: exp(+i), else exp(-i)),
+// unnormalised.  The caller must have synchronised after filling buf.
+template <int NZ, int NL, bool INV, int NS = 1>
+__device__ __forceinline__ void fft_lines(double2 *buf, const double2 *tw)
+{
+    if constexpr (NS < NZ) {
+        constexpr int R = (NZ / NS >= 4) ? 4 : 2;
+        fft_pass<NZ, NL, INV, R, NS>(buf, tw);
+        fft_lines<NZ, NL, INV, NS * R>(buf, tw);
+    }
+}
+
+// dealiased sizes fixed by NZ (Models/Model.h:34: nz = 1 + 2*int(NZ/3))
+template <int NZ>
+struct Dealias {
+    static constexpr int nz = 1 + 2 * (NZ / 3);
+    static constexpr int nzl = (nz - 1) / 2;
+    static constexpr int half = nzl + 1;
+    // padded index of retained slot m, kz order (0..nzl, -nzl..-1)
+    __device__ static __forceinline__ int pad(int m) { return m <= nzl ? m : m + (NZ - nz); }
+    __device__ static __forceinline__ bool kept(int j) { return j <= nzl || j >= NZ - nzl; }
+    __device__ static __forceinline__ int slot(int j) { return j <= nzl ? j : j - (NZ - nz); }
+};
+
+}  // namespace qlk
