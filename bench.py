@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X quasilinear (QL) shearing-box time step.
+
+One "step" is one RK3 step (three substeps) of the QL MHD model over every
+(kx, ky) mode of the box, with the shearing-box remap after it, exactly the
+unit of the reference's main loop (main.cpp:108-123).  The default workload is
+the 64x128x128 box (4032 modes x 85 dealiased kz, Re = Rm = 8000, dt = 0.005)
+with a seeded synthetic state already resident in HBM when timing starts.
+
+Prints one JSON line.  Besides the throughput it reports
+  * roofline: the dominant kernel (the fused per-mode kernel) against HBM
+    bandwidth, from its algorithmic bytes per launch and its average launch
+    duration measured with HIP events on the library's stream;
+  * cpu_baseline: the CPU oracle (a C restatement of the reference algorithm)
+    timed on this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QL RK3-CN steps/s at 64x128x128; mixed/batched vs FP64 speedup at EMF err<=1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per mode-kernel launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", f"k_mode_full_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = re.search(r"^bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(p, lin, mf, budget_s: float = 10.0, max_steps: int = 20):
+    """Times the CPU oracle on this host: 1 warm-up step, then steps until the
+    budget is spent (at least 2, at most max_steps)."""
+    from oracle.oracle import Oracle, build
+    build(native=True)
+    cores = min(16, os.cpu_count() or 1)
+    o = Oracle.from_params(p, nthreads=cores, native=True)
+    o.set_state(lin, mf)
+    o.linear_ops_init(0.0)
+    t = o.steps(1, 0.0)
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_steps:
+        t = o.steps(1, t)
+        n += 1
+        if n >= 2 and time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    o.close()
+    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} steps of the same workload after 1 warm-up step "
+                      f"(oracle/ql_oracle.c, gcc -O3 -march=x86-64-v3, OpenMP mode blocks)"}
+
+
+def copy_calibration(nbytes: int = 1 << 30):
+    """A plain device-to-device copy of known size, for scaling the HBM byte
+    counters (FETCH_SIZE / WRITE_SIZE) that rocprofv3 reports per dispatch."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "kib": nbytes / 1024}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="box64x128x128")
+    ap.add_argument("--integrator", choices=["RK3CN_Elin", "RK3CN"], default="RK3CN_Elin")
+    ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs the QL step on one GPU only; the multi-GPU mode "
+              "sharding is a design (mode blocks + a 4*half complex all-reduce per substep) "
+              "that is not built here.", file=sys.stderr)
+        sys.exit(2)
+
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import INTEGRATOR_NAMES, workload
+
+    p = workload(args.workload, integrator=INTEGRATOR_NAMES[args.integrator])
+    lin, mf = initial_state(p)
+    ctx = QLContext(p)
+    ctx.set_state(lin, mf)
+    ctx.linear_ops_init(0.0)
+    t = ctx.steps(args.warmup, 0.0)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    t = ctx.steps(args.steps, t)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    steps_per_s = args.steps / el
+
+    # per-kernel launch durations (HIP events on the library stream)
+    _, avg_ms, counts = ctx.profile_kernels(args.profile_steps, t)
+    S = p.state_bytes
+    bytes_per_launch = 10.0 * S / 3.0  # u read+write, RHS write (s1), RHS read+write (s2), RHS read (s3)
+    achieved = bytes_per_launch / (avg_ms[0] * 1e-3) / 1e9
+    traffic = traffic_from_profiles(args.workload)
+    lin2, mf2, _ = ctx.get_state()
+    finite = bool(abs(lin2).max() < 1e300 and abs(mf2).max() < 1e300)
+    ctx.close()
+
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded divergence-free fluctuations and 4-harmonic mean fields",
+        "config": {"workload": args.workload, "NX": p.NX, "NY": p.NY, "NZ": p.NZ,
+                   "modes": p.nxy, "nz_dealiased": p.nz, "Re": 1.0 / p.nu, "Rm": 1.0 / p.eta,
+                   "dt": p.dt, "equations": "MHD_FullQlin", "integrator": args.integrator,
+                   "remap": bool(p.remap), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_mode_full", "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": avg_ms[0]},
+        "kernels_ms": {"k_mode_full": avg_ms[0], "k_meanfield": avg_ms[1],
+                       "launches_profiled": counts},
+        "state_finite": finite,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(p, lin, mf)
+        out["speedup_vs_cpu"] = steps_per_s / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -110,10 +110,10 @@ int ql_mf_real(ql_ctx *ctx, double *out);
 int ql_synchronize(ql_ctx *ctx);
 
 /* Runs nsteps like ql_steps with HIP events around every kernel launch and
- * returns, per kernel kind (0 mode kernel, 1 partial reduction, 2 mean-field
- * kernel), the average launch duration in ms and the launch count.  For
- * measurement only: the events serialise the launches. */
-int ql_profile_kernels(ql_ctx *ctx, int nsteps, double *t_inout, double avg_ms[3], int counts[3]);
+ * returns, per kernel kind (0 fused per-mode kernel, 1 mean-field kernel with
+ * the stress reduction), the average launch duration in ms and the launch
+ * count.  For measurement only: the events serialise the launches. */
+int ql_profile_kernels(ql_ctx *ctx, int nsteps, double *t_inout, double avg_ms[2], int counts[2]);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *ql_last_error(void);

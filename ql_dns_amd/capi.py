@@ -185,7 +185,7 @@ class QLContext:
 
     def profile_kernels(self, n: int, t: float = 0.0):
         tt = ctypes.c_double(t)
-        avg = np.zeros(3)
-        cnt = (ctypes.c_int * 3)()
+        avg = np.zeros(2)
+        cnt = (ctypes.c_int * 2)()
         self._check(self._L.ql_profile_kernels(self._h, n, ctypes.byref(tt), _dp(avg), cnt))
         return tt.value, avg, list(cnt)

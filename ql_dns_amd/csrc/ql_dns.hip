@@ -60,6 +60,7 @@ struct ql_ctx {
     double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr;
     double *part = nullptr, *part2 = nullptr, *lin_tmp = nullptr, *dpart = nullptr, *dout = nullptr;
     double *mfr_tmp = nullptr;
+    unsigned *counter = nullptr;
     // integrator bookkeeping: where the RK3CN "old" operator was evaluated
     int old_init = 1;
     double qt_old = 0.0, qdtl_old = 0.0;
@@ -68,15 +69,25 @@ struct ql_ctx {
     // optional per-kernel timing
     bool timing = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    double t_ms[3] = {0, 0, 0};
-    int t_cnt[3] = {0, 0, 0};
+    double t_ms[2] = {0, 0};
+    int t_cnt[2] = {0, 0};
 };
 
 namespace {
 
+// modes per workgroup of the per-mode kernels: MB * NZ = QL_MB_LINE points
+#ifndef QL_MB_LINE
+#define QL_MB_LINE 256
+#endif
+// upper bound on the per-mode kernel grid
+#ifndef QL_GMAX
+#define QL_GMAX 512
+#endif
+
 bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
 
-int mb_for(int NZ) { return NZ >= 128 ? 1 : 128 / NZ; }
+constexpr int mb_of(int NZ) { return NZ >= QL_MB_LINE ? 1 : QL_MB_LINE / NZ; }
+int mb_for(int NZ) { return mb_of(NZ); }
 
 int begin_timing(ql_ctx *c)
 {
@@ -99,7 +110,7 @@ int end_timing(ql_ctx *c, int kind)
 template <int NZ>
 void launch_mode(ql_ctx *c, const qlk::ModeArgs &a)
 {
-    constexpr int MB = NZ >= 128 ? 1 : 128 / NZ;
+    constexpr int MB = mb_of(NZ);
     if (c->p.model == QL_MODEL_MHD_FULLQLIN)
         qlk::k_mode_full<NZ, MB><<<c->G, qlk::NT, 0, c->s>>>(a);
     else
@@ -109,7 +120,8 @@ void launch_mode(ql_ctx *c, const qlk::ModeArgs &a)
 template <int NZ>
 void launch_mf(ql_ctx *c, const qlk::MFArgs &a)
 {
-    qlk::k_meanfield<NZ><<<1, qlk::NT, 0, c->s>>>(a);
+    const int grid = (a.mode == qlk::MODE_PROFILES) ? 1 : c->G2;
+    qlk::k_meanfield<NZ><<<grid, qlk::NT, 0, c->s>>>(a);
 }
 
 template <int NZ>
@@ -153,8 +165,11 @@ qlk::ModeArgs mode_args(ql_ctx *c)
 qlk::MFArgs mf_args(ql_ctx *c)
 {
     qlk::MFArgs m{};
+    m.part = c->part;
+    m.nrows = c->G;
+    m.rows_per = c->rows_per;
     m.part2 = c->part2;
-    m.nrows = c->G2;
+    m.counter = c->counter;
     m.mf = c->mf;
     m.drive = c->drive;
     m.mfz = c->mfz;
@@ -172,7 +187,7 @@ qlk::MFArgs mf_args(ql_ctx *c)
     return m;
 }
 
-// mode kernel -> partial reduction -> mean-field kernel
+// mode kernel -> mean-field kernel (which reduces the mode partials)
 int run_sequence(ql_ctx *c, const qlk::ModeArgs &a, const qlk::MFArgs &m)
 {
     int rc;
@@ -181,13 +196,9 @@ int run_sequence(ql_ctx *c, const qlk::ModeArgs &a, const qlk::MFArgs &m)
     QL_HIP(hipGetLastError());
     if ((rc = end_timing(c, 0))) return rc;
     if ((rc = begin_timing(c))) return rc;
-    qlk::k_reduce<<<c->G2, qlk::NT, 0, c->s>>>(c->part, c->G, c->rows_per, c->nMF * c->NZ, c->part2);
-    QL_HIP(hipGetLastError());
-    if ((rc = end_timing(c, 1))) return rc;
-    if ((rc = begin_timing(c))) return rc;
     QL_DISPATCH_NZ(c->NZ, launch_mf, c, m);
     QL_HIP(hipGetLastError());
-    return end_timing(c, 2);
+    return end_timing(c, 1);
 }
 
 int refresh_profiles(ql_ctx *c)
@@ -273,7 +284,8 @@ int dalloc(T **p, size_t n)
 void release(ql_ctx *c)
 {
     void *bufs[] = {c->u, c->N1, c->N2, c->mf, c->mfN1, c->mfN2, c->drive, c->tw, c->kx0, c->ky,
-                    c->kzv, c->mfz, c->part, c->part2, c->lin_tmp, c->dpart, c->dout, c->mfr_tmp};
+                    c->kzv, c->mfz, c->part, c->part2, c->lin_tmp, c->dpart, c->dout, c->mfr_tmp,
+                    c->counter};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &e : c->ev)
@@ -337,9 +349,8 @@ int ql_create(const ql_params *p, ql_ctx **out)
     c->kzfac = 2 * QL_PI / p->L[2] * (p->NZ / 3);
     c->MB = mb_for(p->NZ);
     const int ngroups = (c->nxy + c->MB - 1) / c->MB;
-    c->G = ngroups < 1024 ? ngroups : 1024;
-    c->G2 = c->G < 32 ? c->G : 32;
-    c->rows_per = (c->G + c->G2 - 1) / c->G2;
+    c->G = ngroups < QL_GMAX ? ngroups : QL_GMAX;
+    c->rows_per = 16;
     c->G2 = (c->G + c->rows_per - 1) / c->rows_per;
 
     // wavenumber tables, Kdata.cpp:28-80
@@ -386,6 +397,7 @@ int ql_create(const ql_params *p, ql_ctx **out)
     if (!rc) rc = dalloc(&c->dpart, (size_t)256 * 6);
     if (!rc) rc = dalloc(&c->dout, (size_t)18);
     if (!rc) rc = dalloc(&c->mfr_tmp, (size_t)4 * c->NZ);
+    if (!rc) rc = dalloc(&c->counter, (size_t)1);
     if (!rc && hipMemcpy(c->kx0, c->h_kx0.data(), sizeof(double) * c->nxy, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(QL_ERR_HIP, "table upload failed");
     if (!rc && hipMemcpy(c->ky, c->h_ky.data(), sizeof(double) * c->nxy, hipMemcpyHostToDevice) != hipSuccess)
@@ -623,10 +635,10 @@ int ql_synchronize(ql_ctx *c)
     return QL_OK;
 }
 
-int ql_profile_kernels(ql_ctx *c, int nsteps, double *t_inout, double avg_ms[3], int counts[3])
+int ql_profile_kernels(ql_ctx *c, int nsteps, double *t_inout, double avg_ms[2], int counts[2])
 {
     if (!c || !t_inout || !avg_ms || !counts) return fail(QL_ERR_ARG, "ql_profile_kernels: null argument");
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
         c->t_ms[k] = 0.0;
         c->t_cnt[k] = 0;
     }
@@ -634,7 +646,7 @@ int ql_profile_kernels(ql_ctx *c, int nsteps, double *t_inout, double avg_ms[3],
     int rc = ql_steps(c, nsteps, t_inout);
     c->timing = false;
     if (rc) return rc;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
         counts[k] = c->t_cnt[k];
         avg_ms[k] = c->t_cnt[k] ? c->t_ms[k] / c->t_cnt[k] : 0.0;
     }

@@ -4,8 +4,8 @@
 // `inverse` = zero-pad nz -> NZ then unnormalised exp(+i) transform, `forward`
 // = unnormalised exp(-i) transform then truncate NZ -> nz).  Here a workgroup
 // transforms NL lines of length NZ at once, in place in LDS, with a Stockham
-// autosort schedule of radix-4 passes (and one radix-2 pass when log2 NZ is
-// odd).  Each pass reads every element of every line into registers, waits at
+// autosort schedule of radix-8 passes (then one radix-4 or radix-2 pass for
+// the remaining factor).  Each pass reads every element of every line into registers, waits at
 // a barrier, and writes the butterflies back, so a single LDS buffer suffices.
 // Twiddles come from a table tw[k] = exp(-2 pi i k / NZ) staged in LDS.
 #pragma once
@@ -37,6 +37,37 @@ __device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, doub
     v2 = csub(a0, a2);
     v1 = cadd(a1, ja3);
     v3 = csub(a1, ja3);
+}
+
+// 8-point DFT as two 4-point DFTs (radix-2 split): even outputs from
+// v[k] + v[k+4], odd outputs from (v[k] - v[k+4]) * w8^k.
+template <bool INV>
+__device__ __forceinline__ void dft8(double2 *v)
+{
+    constexpr double H = 0.70710678118654752440;  // 1/sqrt(2)
+    double2 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = cadd(v[k], v[k + 4]);
+        b[k] = csub(v[k], v[k + 4]);
+    }
+    // w8^1 = (1 -/+ i)/sqrt2, w8^2 = -/+ i, w8^3 = (-1 -/+ i)/sqrt2
+    if (INV) {
+        b[1] = make_double2(H * (b[1].x - b[1].y), H * (b[1].x + b[1].y));
+        b[2] = make_double2(-b[2].y, b[2].x);
+        b[3] = make_double2(-H * (b[3].x + b[3].y), H * (b[3].x - b[3].y));
+    } else {
+        b[1] = make_double2(H * (b[1].x + b[1].y), H * (b[1].y - b[1].x));
+        b[2] = make_double2(b[2].y, -b[2].x);
+        b[3] = make_double2(H * (b[3].y - b[3].x), -H * (b[3].x + b[3].y));
+    }
+    dft4<INV>(a[0], a[1], a[2], a[3]);
+    dft4<INV>(b[0], b[1], b[2], b[3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[2 * k] = a[k];
+        v[2 * k + 1] = b[k];
+    }
 }
 
 // One Stockham pass of radix R at stride NS over NL lines held in buf.
@@ -72,7 +103,9 @@ __device__ __forceinline__ void fft_pass(double2 *buf, const double2 *tw)
                     v[it][r] = cmul(v[it][r], w);
                 }
             }
-            if constexpr (R == 4) {
+            if constexpr (R == 8) {
+                dft8<INV>(v[it]);
+            } else if constexpr (R == 4) {
                 dft4<INV>(v[it][0], v[it][1], v[it][2], v[it][3]);
             } else {
                 const double2 a = v[it][0], b = v[it][1];
@@ -93,7 +126,7 @@ template <int NZ, int NL, bool INV, int NS = 1>
 __device__ __forceinline__ void fft_lines(double2 *buf, const double2 *tw)
 {
     if constexpr (NS < NZ) {
-        constexpr int R = (NZ / NS >= 4) ? 4 : 2;
+        constexpr int R = (NZ / NS >= 8) ? 8 : ((NZ / NS >= 4) ? 4 : 2);
         fft_pass<NZ, NL, INV, R, NS>(buf, tw);
         fft_lines<NZ, NL, INV, NS * R>(buf, tw);
     }

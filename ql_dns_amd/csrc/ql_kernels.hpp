@@ -1,6 +1,6 @@
 // ql_kernels.hpp -- the per-substep kernels of the QL time step (FP64).
 //
-// One RK3 substep is three launches on one stream:
+// One RK3 substep is two launches on one stream:
 //   k_mode_*      fused per-(kx,ky) RHS + substep update.  A workgroup takes MB
 //                 modes at a time: builds the padded spectral lines of u, b and
 //                 their z-derivatives, inverse-transforms them in LDS, forms the
@@ -10,8 +10,8 @@
 //                 new state (and the RHS when a later substep needs it).  The
 //                 EMF / Reynolds-stress z-profiles of its modes are summed in
 //                 registers in a fixed order and written as one partial per WG.
-//   k_reduce      fixed-order sum of row blocks of the partials.
-//   k_meanfield   one workgroup: final sum, forward transform, x i kz,
+//   k_meanfield   fixed-order reduction of the partials (row blocks, then the
+//                 last block to finish adds the block sums), forward transform, x i kz,
 //                 Hermitian fill (the reference's MPI Allreduce + fill,
 //                 MHD_FullQlin.cpp:386-399), mean-field RHS and update, and the
 //                 real-space mean-field profiles the next substep reads.
@@ -21,6 +21,17 @@
 namespace qlk {
 
 enum { MODE_RHS = 0, MODE_ELIN = 1, MODE_CN = 2, MODE_PROFILES = 3 };
+
+// Occupancy request for the per-mode kernels (waves per SIMD); bounds their
+// VGPR budget.  0 leaves the choice to the compiler.
+#ifndef QL_MODE_WAVES
+#define QL_MODE_WAVES 0
+#endif
+#if QL_MODE_WAVES > 0
+#define QL_MODE_ATTR __attribute__((amdgpu_waves_per_eu(QL_MODE_WAVES)))
+#else
+#define QL_MODE_ATTR
+#endif
 
 struct ModeArgs {
     double2 *u;             // state [nxy][4][nz], read and (MODE_ELIN/CN) updated in place
@@ -59,11 +70,66 @@ __device__ __forceinline__ bool remap_hit(const ModeArgs &a, double kx0, double 
     return __dadd_rn(kx0, __dmul_rn(a.remap_qt, ky)) > a.remap_thr;
 }
 
+// Per-mode inverse Laplacians of assign_laplacians_ (MHD_FullQlin.cpp:722-738,
+// Kdata.cpp:100-120): ilap2 = 1/(-ky^2 - kz^2) with the (ky, kz) = (0, 0)
+// entry set to 1, ilapF = 1/(-kx(t)^2 + lap2) with the (0, 0, 0) entry set to 1.
+template <int NZD, int MB>
+__device__ __forceinline__ void mode_tables(const ModeArgs &a, const double *skx, const double *sky,
+                                            const int *sidx, double *sil2, double *silF)
+{
+    for (int k = threadIdx.x; k < MB * NZD; k += NT) {
+        const int mb = k / NZD, m = k % NZD;
+        const double kx = skx[mb], ky = sky[mb], kz = a.kzv[m];
+        const bool zero = (ky == 0.0 && m == 0);
+        const double lap2 = zero ? 0.0 : (-ky * ky + -(kz * kz));
+        sil2[k] = zero ? 1.0 : 1.0 / lap2;
+        silF[k] = (sidx[mb] == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
+    }
+}
+
+// Substep update of one (mode, kz) slot for a pair of fields sharing one
+// dissipation coefficient: RK3CN_Elin (RK3CN_Elin.cpp:303-305 and :335-337,
+// :368-370) or RK3CN (RK3CN.cpp:105-107, :141-143, :177-179).  Ln is the
+// operator at the end of the substep; Lo is only used by RK3CN.
+__device__ __forceinline__ void update_pair(const ModeArgs &a, size_t off0, size_t off1, double2 u0, double2 u1,
+                                            double2 N0, double2 N1, double Ln, double Lo, bool zero_out)
+{
+    double2 un0, un1;
+    if (a.mode == MODE_ELIN) {
+        const double e = exp(a.linC * Ln);
+        double2 s0 = cscale(a.nlc1, N0), s1 = cscale(a.nlc1, N1);
+        if (a.n_prev) {
+            s0 = cadd(s0, cscale(a.nlc2, a.n_prev[off0]));
+            s1 = cadd(s1, cscale(a.nlc2, a.n_prev[off1]));
+        }
+        un0 = cscale(e, cadd(s0, u0));
+        un1 = cscale(e, cadd(s1, u1));
+    } else {
+        const double denom = 1.0 / (1.0 - a.linC * Ln);
+        const double c1 = a.nlc1 * denom, co = (1 + a.linC * Lo) * denom;
+        double2 s0 = cscale(c1, N0), s1 = cscale(c1, N1);
+        if (a.n_prev) {
+            const double c2 = a.nlc2 * denom;
+            s0 = cadd(s0, cscale(c2, a.n_prev[off0]));
+            s1 = cadd(s1, cscale(c2, a.n_prev[off1]));
+        }
+        un0 = cadd(s0, cscale(co, u0));
+        un1 = cadd(s1, cscale(co, u1));
+    }
+    if (zero_out) un0 = un1 = make_double2(0.0, 0.0);
+    a.u[off0] = un0;
+    a.u[off1] = un1;
+    if (a.n_out) {
+        a.n_out[off0] = N0;
+        a.n_out[off1] = N1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // MHD_FullQlin (Models/MHD_FullQlin.cpp:171-427)
 // ---------------------------------------------------------------------------
 template <int NZ, int MB>
-__global__ __launch_bounds__(NT) void k_mode_full(ModeArgs a)
+__global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode_full(ModeArgs a)
 {
     using D = Dealias<NZ>;
     constexpr int NZD = D::nz, NZL = D::nzl;
@@ -75,6 +141,7 @@ __global__ __launch_bounds__(NT) void k_mode_full(ModeArgs a)
     __shared__ double2 tw[NZ];
     __shared__ double skx0[MB], sky[MB], skx[MB];
     __shared__ int sidx[MB];
+    __shared__ double sil2[MB * NZD], silF[MB * NZD];
     const int tid = threadIdx.x;
 
     for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
@@ -111,6 +178,8 @@ __global__ __launch_bounds__(NT) void k_mode_full(ModeArgs a)
             uin[k] = (i < a.nxy) ? a.u[(size_t)i * 4 * NZD + (k - mb * 4 * NZD)] : make_double2(0.0, 0.0);
         }
         __syncthreads();
+        mode_tables<NZD, MB>(a, skx, sky, sidx, sil2, silF);
+        __syncthreads();
 
         // padded spectral lines, fac1D folded in (MHD_FullQlin.cpp:220-265, :338-356):
         // 0 ux 1 uy 2 uz 3 bx 4 by 5 bz 6 dz ux 7 dz bx 8 dz uy 9 dz by
@@ -122,8 +191,7 @@ __global__ __launch_bounds__(NT) void k_mode_full(ModeArgs a)
                 const int m = D::slot(j);
                 const double2 *U = uin + mb * 4 * NZD;
                 const double kx = skx[mb], ky = sky[mb], kz = a.kzv[m];
-                const double kz2 = -(kz * kz);
-                const double ilap2 = (ky == 0.0 && m == 0) ? 1.0 : 1.0 / (-ky * ky + kz2);
+                const double ilap2 = sil2[mb * NZD + m];
                 const bool isb = (l == 3 || l == 4 || l == 5 || l == 7 || l == 9);
                 const double2 f1 = isb ? U[2 * NZD + m] : U[m];        // ux / bx
                 const double2 f2 = isb ? U[3 * NZD + m] : U[NZD + m];  // zeta / eta
@@ -206,75 +274,61 @@ __global__ __launch_bounds__(NT) void k_mode_full(ModeArgs a)
         fft_lines<NZ, 6 * MB, false>(buf, tw);
 
         // RHS assembly (MHD_FullQlin.cpp:283-305), linear operator (:371-376)
-        // and the substep update (RK3CN_Elin.cpp:99-112 / RK3CN.cpp:102-115)
-        for (int k = tid; k < MB * 4 * NZD; k += NT) {
-            const int mb = k / (4 * NZD), r = k % (4 * NZD);
-            const int f = r / NZD, m = r % NZD;
+        // and the substep update, per (mode, kz) slot and field pair
+        for (int k = tid; k < MB * 2 * NZD; k += NT) {
+            const int mb = k / (2 * NZD), r2 = k % (2 * NZD);
+            const int pr = r2 / NZD, m = r2 % NZD;
             const int i = sidx[mb];
             if (i >= a.nxy) continue;
             const double2 *T = buf + mb * 6 * NZ + D::pad(m);
             const double2 *U = uin + mb * 4 * NZD;
             const double kx = skx[mb], ky = sky[mb], kz = a.kzv[m];
             const double kz2 = -(kz * kz);
-            const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
             const double q = a.q, Om = a.omega, B0z = a.B0z;
-            double2 N;
-            if (f == 0) {
-                double ilapF = 1.0 / (-kx * kx + lap2);
-                if (i == 0 && m == 0) ilapF = 1.0;
-                const double2 ux = U[m], ze = U[NZD + m], bx = U[2 * NZD + m];
+            const double2 ux = U[m], ze = U[NZD + m], bx = U[2 * NZD + m], et = U[3 * NZD + m];
+            double2 N0, N1;
+            if (pr == 0) {
+                const double ilapF = silF[mb * NZD + m];
                 const double2 T1 = T[0], T2 = T[NZ], T3 = T[2 * NZ];
                 double2 s = cscale(2.0 * kx * ky * q, ux);
                 s = cadd(s, imul(2.0 * Om * kz, ze));
                 s = csub(s, cscale(kx * kx, T1));
                 s = csub(s, cscale(kx * ky, T2));
                 s = csub(s, cscale(kx * kz, T3));
-                N = csub(cadd(cscale(ilapF, s), imul(B0z * kz, bx)), T1);
-            } else if (f == 1) {
-                const double2 ux = U[m], et = U[3 * NZD + m];
-                const double2 T2 = T[NZ], T3 = T[2 * NZ];
-                N = cadd(imul((q - 2.0 * Om) * kz, ux), imul(B0z * kz, et));
-                N = csub(N, csub(imul(kz, T2), imul(ky, T3)));
-            } else if (f == 2) {
-                N = csub(imul(B0z * kz, U[m]), T[3 * NZ]);
+                N0 = csub(cadd(cscale(ilapF, s), imul(B0z * kz, bx)), T1);
+                N1 = cadd(imul((q - 2.0 * Om) * kz, ux), imul(B0z * kz, et));
+                N1 = csub(N1, csub(imul(kz, T2), imul(ky, T3)));
             } else {
-                const double2 ze = U[NZD + m], bx = U[2 * NZD + m];
                 const double2 T2 = T[4 * NZ], T3 = T[5 * NZ];
-                N = cadd(imul(-q * kz, bx), imul(B0z * kz, ze));
-                N = csub(N, csub(imul(kz, T2), imul(ky, T3)));
+                N0 = csub(imul(B0z * kz, ux), T[3 * NZ]);
+                N1 = cadd(imul(-q * kz, bx), imul(B0z * kz, ze));
+                N1 = csub(N1, csub(imul(kz, T2), imul(ky, T3)));
             }
-            const double coef = (f < 2) ? a.nu : a.eta;
+            const double coef = (pr == 0) ? a.nu : a.eta;
             const double kxn = kx + a.qdtl * ky;
             const double Ln = coef * neg_abs_pow((-kxn * kxn - ky * ky) + kz2, a.vo2);
-            const size_t off = (size_t)i * 4 * NZD + r;
+            const size_t off0 = (size_t)i * 4 * NZD + (2 * pr) * NZD + m, off1 = off0 + NZD;
             if (a.mode == MODE_RHS) {
-                if (a.n_out) a.n_out[off] = N;
-                if (a.lin_out) a.lin_out[off] = Ln;
+                if (a.n_out) {
+                    a.n_out[off0] = N0;
+                    a.n_out[off1] = N1;
+                }
+                if (a.lin_out) a.lin_out[off0] = a.lin_out[off1] = Ln;
                 continue;
             }
-            double2 un;
-            const double2 u = U[r];
-            if (a.mode == MODE_ELIN) {
-                double2 acc2 = cscale(a.nlc1, N);
-                if (a.n_prev) acc2 = cadd(acc2, cscale(a.nlc2, a.n_prev[off]));
-                un = cscale(exp(a.linC * Ln), cadd(acc2, u));
-            } else {
-                double Lo;
+            double Lo = 0.0;
+            if (a.mode == MODE_CN) {
                 if (a.old_init) {
                     const double kxo = __dadd_rn(skx0[mb], __dmul_rn(a.qt_old, ky));
+                    const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
                     Lo = coef * neg_abs_pow(-kxo * kxo + lap2, a.vo2);
                 } else {
                     const double kxo = __dadd_rn(skx0[mb], __dmul_rn(a.qt_old, ky)) + a.qdtl_old * ky;
                     Lo = coef * neg_abs_pow((-kxo * kxo - ky * ky) + kz2, a.vo2);
                 }
-                const double denom = 1.0 / (1.0 - a.linC * Ln);
-                double2 s = cscale(a.nlc1 * denom, N);
-                if (a.n_prev) s = cadd(s, cscale(a.nlc2 * denom, a.n_prev[off]));
-                un = cadd(s, cscale((1 + a.linC * Lo) * denom, u));
             }
-            if (a.do_remap && remap_hit(a, skx0[mb], ky)) un = make_double2(0.0, 0.0);
-            a.u[off] = un;
-            if (a.n_out) a.n_out[off] = N;
+            const bool zero_out = a.do_remap && remap_hit(a, skx0[mb], ky);
+            update_pair(a, off0, off1, U[2 * pr * NZD + m], U[(2 * pr + 1) * NZD + m], N0, N1, Ln, Lo, zero_out);
         }
         if (a.do_remap && tid < MB) {
             const int i = sidx[tid];
@@ -309,7 +363,7 @@ __global__ __launch_bounds__(NT) void k_mode_full(ModeArgs a)
 // Omega = 2q/3, plain Laplacian dissipation.
 // ---------------------------------------------------------------------------
 template <int NZ, int MB>
-__global__ __launch_bounds__(NT) void k_mode_bq(ModeArgs a)
+__global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode_bq(ModeArgs a)
 {
     using D = Dealias<NZ>;
     constexpr int NZD = D::nz;
@@ -321,6 +375,7 @@ __global__ __launch_bounds__(NT) void k_mode_bq(ModeArgs a)
     __shared__ double2 tw[NZ];
     __shared__ double skx0[MB], sky[MB], skx[MB];
     __shared__ int sidx[MB];
+    __shared__ double sil2[MB * NZD], silF[MB * NZD];
     const int tid = threadIdx.x;
     for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
 
@@ -355,6 +410,8 @@ __global__ __launch_bounds__(NT) void k_mode_bq(ModeArgs a)
             uin[k] = (i < a.nxy) ? a.u[(size_t)i * 4 * NZD + (k - mb * 4 * NZD)] : make_double2(0.0, 0.0);
         }
         __syncthreads();
+        mode_tables<NZD, MB>(a, skx, sky, sidx, sil2, silF);
+        __syncthreads();
         // lines (fac1D folded in): 0 ux 1 zeta 2 bx 3 eta 4 uy 5 uz 6 by 7 bz 8 dz uz
         for (int idx = tid; idx < NL * NZ; idx += NT) {
             const int line = idx / NZ, j = idx % NZ;
@@ -364,8 +421,7 @@ __global__ __launch_bounds__(NT) void k_mode_bq(ModeArgs a)
                 const int m = D::slot(j);
                 const double2 *U = uin + mb * 4 * NZD;
                 const double kx = skx[mb], ky = sky[mb], kz = a.kzv[m];
-                const double kz2 = -(kz * kz);
-                const double ilap2 = (ky == 0.0 && m == 0) ? 1.0 : 1.0 / (-ky * ky + kz2);
+                const double ilap2 = sil2[mb * NZD + m];
                 const double fac = a.fac;
                 if (l < 4) {
                     val = cscale(fac, U[l * NZD + m]);
@@ -436,63 +492,55 @@ __global__ __launch_bounds__(NT) void k_mode_bq(ModeArgs a)
         __syncthreads();
         fft_lines<NZ, 6 * MB, false>(buf, tw);
 
-        for (int k = tid; k < MB * 4 * NZD; k += NT) {
-            const int mb = k / (4 * NZD), r = k % (4 * NZD);
-            const int f = r / NZD, m = r % NZD;
+        // RHS assembly (MHD_BQlin.cpp:162-201), linear operator (:248-252) and
+        // the substep update, per (mode, kz) slot and field pair
+        for (int k = tid; k < MB * 2 * NZD; k += NT) {
+            const int mb = k / (2 * NZD), r2 = k % (2 * NZD);
+            const int pr = r2 / NZD, m = r2 % NZD;
             const int i = sidx[mb];
             if (i >= a.nxy) continue;
             const double2 *T = buf + mb * 6 * NZ + D::pad(m);
             const double2 *U = uin + mb * 4 * NZD;
             const double kx = skx[mb], ky = sky[mb], kz = a.kzv[m];
             const double kz2 = -(kz * kz);
-            const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
             const double q = a.q, Om = a.omega;
-            double2 N;
-            if (f == 0) {
-                double ilapF = 1.0 / (-kx * kx + lap2);
-                if (i == 0 && m == 0) ilapF = 1.0;
-                double2 s = cadd(cscale(2.0 * kx * ky * q, U[m]), imul(2.0 * Om * kz, U[NZD + m]));
-                N = cadd(T[0], cscale(ilapF, cadd(s, T[NZ])));
-            } else if (f == 1) {
-                N = cadd(T[3 * NZ], imul((q - 2.0 * Om) * kz, U[m]));
-            } else if (f == 2) {
-                N = T[2 * NZ];
+            double2 N0, N1;
+            if (pr == 0) {
+                const double ilapF = silF[mb * NZD + m];
+                const double2 s = cadd(cscale(2.0 * kx * ky * q, U[m]), imul(2.0 * Om * kz, U[NZD + m]));
+                N0 = cadd(T[0], cscale(ilapF, cadd(s, T[NZ])));
+                N1 = cadd(T[3 * NZ], imul((q - 2.0 * Om) * kz, U[m]));
             } else {
-                N = csub(T[4 * NZ], imul(q * kz, U[2 * NZD + m]));
+                N0 = T[2 * NZ];
+                N1 = csub(T[4 * NZ], imul(q * kz, U[2 * NZD + m]));
             }
             const double kxn = kx + a.qdtl * ky;
             const double L0 = a.nu * ((-kxn * kxn - ky * ky) + kz2);
-            const double Ln = (f < 2) ? L0 : (a.eta / a.nu) * L0;
-            const size_t off = (size_t)i * 4 * NZD + r;
+            const double Ln = (pr == 0) ? L0 : (a.eta / a.nu) * L0;
+            const size_t off0 = (size_t)i * 4 * NZD + (2 * pr) * NZD + m, off1 = off0 + NZD;
             if (a.mode == MODE_RHS) {
-                if (a.n_out) a.n_out[off] = N;
-                if (a.lin_out) a.lin_out[off] = Ln;
+                if (a.n_out) {
+                    a.n_out[off0] = N0;
+                    a.n_out[off1] = N1;
+                }
+                if (a.lin_out) a.lin_out[off0] = a.lin_out[off1] = Ln;
                 continue;
             }
-            double2 un;
-            const double2 u = U[r];
-            if (a.mode == MODE_ELIN) {
-                double2 acc2 = cscale(a.nlc1, N);
-                if (a.n_prev) acc2 = cadd(acc2, cscale(a.nlc2, a.n_prev[off]));
-                un = cscale(exp(a.linC * Ln), cadd(acc2, u));
-            } else {
+            double Lo = 0.0;
+            if (a.mode == MODE_CN) {
                 double L0o;
                 if (a.old_init) {
                     const double kxo = __dadd_rn(skx0[mb], __dmul_rn(a.qt_old, ky));
+                    const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
                     L0o = a.nu * (-kxo * kxo + lap2);
                 } else {
                     const double kxo = __dadd_rn(skx0[mb], __dmul_rn(a.qt_old, ky)) + a.qdtl_old * ky;
                     L0o = a.nu * ((-kxo * kxo - ky * ky) + kz2);
                 }
-                const double Lo = (f < 2) ? L0o : (a.eta / a.nu) * L0o;
-                const double denom = 1.0 / (1.0 - a.linC * Ln);
-                double2 s = cscale(a.nlc1 * denom, N);
-                if (a.n_prev) s = cadd(s, cscale(a.nlc2 * denom, a.n_prev[off]));
-                un = cadd(s, cscale((1 + a.linC * Lo) * denom, u));
+                Lo = (pr == 0) ? L0o : (a.eta / a.nu) * L0o;
             }
-            if (a.do_remap && remap_hit(a, skx0[mb], ky)) un = make_double2(0.0, 0.0);
-            a.u[off] = un;
-            if (a.n_out) a.n_out[off] = N;
+            const bool zero_out = a.do_remap && remap_hit(a, skx0[mb], ky);
+            update_pair(a, off0, off1, U[2 * pr * NZD + m], U[(2 * pr + 1) * NZD + m], N0, N1, Ln, Lo, zero_out);
         }
         if (a.do_remap && tid < MB) {
             const int i = sidx[tid];
@@ -518,26 +566,17 @@ __global__ __launch_bounds__(NT) void k_mode_bq(ModeArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// partial sums: block b adds rows [b*rows, (b+1)*rows) in order
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_reduce(const double *__restrict__ in, int nrows, int rows_per,
-                                               int ncols, double *__restrict__ out)
-{
-    const int r0 = blockIdx.x * rows_per;
-    const int r1 = min(nrows, r0 + rows_per);
-    for (int c = threadIdx.x; c < ncols; c += NT) {
-        double s = 0.0;
-        for (int r = r0; r < r1; ++r) s += in[(size_t)r * ncols + c];
-        out[(size_t)blockIdx.x * ncols + c] = s;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// mean fields: final reduction, drive, RHS, update, next profiles
+// mean fields: fixed-order reduction of the mode-kernel partials, drive, RHS,
+// update, next profiles.  Launched with gridDim.x blocks: each adds a block of
+// partial rows; the block that finishes last (atomic ticket, no waiting) adds
+// the block sums in index order and does the rest.  The ticket counter is
+// reset by that last block, so the kernel needs no memset between launches.
 // ---------------------------------------------------------------------------
 struct MFArgs {
-    const double *part2;  // [nrows][nMF][NZ]
-    int nrows;
+    const double *part;   // mode-kernel partials [nrows][nMF][NZ]
+    int nrows, rows_per;
+    double *part2;        // block sums [gridDim.x][nMF][NZ]
+    unsigned *counter;    // ticket, 0 between launches
     double2 *mf;              // [nMF][nz]
     double2 *mfn_out;         // mean-field RHS out [nMF][nz]
     const double2 *mfn_prev;  // previous substep mean-field RHS (nullable)
@@ -548,6 +587,21 @@ struct MFArgs {
     int model, nMF, quasilinear, mode;
     double q, omega, B0z, nu, eta, fac, linC, nlc1, nlc2;
 };
+
+// sum of rows [r0, r1) of column c, in row order, loads issued 32 at a time
+__device__ __forceinline__ double sum_rows(const double *p, int r0, int r1, int ncols, int c)
+{
+    constexpr int B = 32;
+    double s = 0.0;
+    for (int r = r0; r < r1; r += B) {
+        double v[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) v[k] = (r + k < r1) ? p[(size_t)(r + k) * ncols + c] : 0.0;
+#pragma unroll
+        for (int k = 0; k < B; ++k) s += v[k];
+    }
+    return s;
+}
 
 template <int NZ>
 __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
@@ -563,12 +617,20 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
 
     if (a.mode != MODE_PROFILES) {
-        // fixed-order sum of the partial rows -> real z-profiles
-        for (int k = tid; k < nMF * NZ; k += NT) {
-            double s = 0.0;
-            for (int r = 0; r < a.nrows; ++r) s += a.part2[(size_t)r * nMF * NZ + k];
-            buf[k] = make_double2(s, 0.0);
-        }
+        __shared__ int amlast;
+        const int ncols = nMF * NZ;
+        const int r0 = blockIdx.x * a.rows_per;
+        const int r1 = min(a.nrows, r0 + a.rows_per);
+        for (int c = tid; c < ncols; c += NT) a.part2[(size_t)blockIdx.x * ncols + c] = sum_rows(a.part, r0, r1, ncols, c);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) amlast = (atomicAdd(a.counter, 1u) == gridDim.x - 1);
+        __syncthreads();
+        if (!amlast) return;
+        __threadfence();
+        // fixed-order sum of the block sums -> real z-profiles
+        for (int c = tid; c < ncols; c += NT) buf[c] = make_double2(sum_rows(a.part2, 0, gridDim.x, ncols, c), 0.0);
+        if (tid == 0) *a.counter = 0u;
         __syncthreads();
         if (nMF == 4) fft_lines<NZ, 4, false>(buf, tw);
         else fft_lines<NZ, 2, false>(buf, tw);

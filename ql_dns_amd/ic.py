@@ -72,7 +72,8 @@ def initial_state(p: QLParams, seed: int | None = None, amp: float = 1e-2,
     """Return (lin, mf) complex128 arrays of shapes (nxy, 4, nz) and (nMF, nz).
 
     Fluctuations: each Cartesian component of u and b is complex N(0,1) times
-    amp*exp(-|k|^2/kdecay) with k = (kx0, ky, kz), projected divergence free,
+    amp*exp(-|k|^2/kdecay) (zero where |k|^2/kdecay > 300) with k = (kx0, ky, kz),
+    projected divergence free,
     converted to (ux, zeta, bx, eta), scaled to the storage convention, made
     Hermitian across the ky = 0 plane (the (kx, 0, kz) mode is the conjugate of
     (-kx, 0, -kz)), with the (kx, ky) = (0, 0) mode zero.
@@ -84,7 +85,9 @@ def initial_state(p: QLParams, seed: int | None = None, amp: float = 1e-2,
     kx0, ky, kxi, kyi, kz, kzi = mode_tables(p)
     K = np.stack(np.broadcast_arrays(kx0[:, None], ky[:, None], kz[None, :]), axis=-1)  # (nxy,nz,3)
     k2 = np.sum(K * K, axis=-1)
-    env = amp * np.exp(-k2 / kdecay)
+    # envelopes below amp*e^-300 are set to exact zero: such amplitudes carry no
+    # physical content and would only produce subnormal numbers
+    env = np.where(k2 / kdecay > 300.0, 0.0, amp * np.exp(-np.minimum(k2 / kdecay, 300.0)))
     cnt = (np.arange(nxy)[:, None] * nz + np.arange(nz)[None, :]).astype(np.uint64)
 
     def field(stream0: int) -> np.ndarray:

@@ -1,0 +1,54 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol the header
+declares, and rejects invalid parameters before touching a device."""
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "ql_dns.h")) as f:
+        src = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ql_\w+)\s*\(", src, re.M)))
+
+
+def test_header_matches_binding():
+    from ql_dns_amd.capi import EXPORTS
+    assert sorted(EXPORTS) == _declared()
+
+
+def test_library_exports_all_symbols():
+    from ql_dns_amd import capi
+    L = capi.load()
+    for name in _declared():
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("NZ", 48, -1), ("NZ", 512, -1), ("NX", 7, -1), ("NY", 0, -1), ("model", 5, -1),
+    ("integrator", 3, -1), ("precision", 7, -3), ("dt", -0.01, -3), ("dt", 0.0, -3),
+    ("viscosity_order", -2, -1)])
+def test_create_rejects_invalid_params(field, value, code):
+    import ctypes
+    from ql_dns_amd import capi
+    from ql_dns_amd.params import QLParams
+    L = capi.load()
+    p = QLParams().replace(**{field: value})
+    cp = capi.to_c_params(p)
+    h = ctypes.c_void_p()
+    rc = L.ql_create(ctypes.byref(cp), ctypes.byref(h))
+    assert rc == code
+    assert h.value is None
+    assert len(L.ql_last_error()) > 0
+
+
+def test_null_arguments_are_errors():
+    import ctypes
+    from ql_dns_amd import capi
+    L = capi.load()
+    assert L.ql_create(None, None) == -1
+    assert L.ql_destroy(None) == 0
+    assert L.ql_steps(None, 1, None) == -1
+    assert L.ql_dims(None, None) == -1
