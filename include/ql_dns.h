@@ -110,9 +110,8 @@ int ql_mf_real(ql_ctx *ctx, double *out);
 int ql_synchronize(ql_ctx *ctx);
 
 /* Runs nsteps like ql_steps with HIP events around every kernel launch and
- * returns, per kernel kind (0 fused per-mode kernel, 1 mean-field kernel with
- * the stress reduction), the average launch duration in ms and the launch
- * count.  For measurement only: the events serialise the launches. */
+ * returns, per kernel kind (0 fused per-mode kernel, 1 stress reduction +
+ * mean-field kernel), the average duration in ms and the count.  For measurement only: the events serialise the launches. */
 int ql_profile_kernels(ql_ctx *ctx, int nsteps, double *t_inout, double avg_ms[2], int counts[2]);
 
 /* Message of the last failed call on this thread ("" if none). */

@@ -16,8 +16,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libql_dns.so")
-SOURCES = ["ql_dns.hip"]
-HEADERS = ["ql_fft.hpp", "ql_kernels.hpp"]
+SOURCES = ["ql_dns.hip", "host/ql_model.cpp"]
+HEADERS = ["ql_fft.hpp", "ql_kernels.hpp", "ql_mode.hpp", "host/ql_model.hpp"]
+HOST_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_host_adapter.cpp")
+HOST_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_host_adapter")
 
 
 def hipcc() -> str:
@@ -41,7 +43,21 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libql_dns.so")
     os.replace(tmp, LIB)
+    build_host_test(verbose)
     return LIB
+
+
+def build_host_test(verbose: bool = False) -> str:
+    """g++ build of the C++ driver that exercises the Model/Integrator mirror."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), HOST_TEST_SRC,
+           "-L", HERE, "-lql_dns", "-Wl,-rpath,$ORIGIN/../../ql_dns_amd", "-o", HOST_TEST_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building tests/cpp/test_host_adapter")
+    return HOST_TEST_BIN
 
 
 if __name__ == "__main__":

@@ -1,0 +1,240 @@
+// ql_model.cpp -- the C++ Model / Integrator / solution mirror on the C ABI.
+#include "ql_model.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace qlgpu {
+
+namespace {
+
+int model_id(const std::string &s)
+{
+    if (s == "MHD_FullQlin") return QL_MODEL_MHD_FULLQLIN;
+    if (s == "MHD_BQlin") return QL_MODEL_MHD_BQLIN;
+    throw Error(QL_ERR_ARG, "model '" + s + "' is not available on the GPU path (MHD_FullQlin, MHD_BQlin)");
+}
+
+int integrator_id(const std::string &s)
+{
+    if (s == "RK3CN_Elin") return QL_INT_RK3CN_ELIN;
+    if (s == "RK3CN") return QL_INT_RK3CN;
+    throw Error(QL_ERR_ARG, "integrator '" + s + "' is not available on the GPU path (RK3CN, RK3CN_Elin)");
+}
+
+const double kPi = 3.14159265358979323846;
+
+}  // namespace
+
+void TimeVariables::Save_Data(double t)
+{
+    Record r;
+    r.t = t;
+    for (int k = 0; k < 4; ++k) {
+        r.energy[k] = energy_[k];
+        r.AM[k] = AM_[k];
+        r.diss[k] = diss_[k];
+        r.reynolds[k] = reynolds_[k];
+    }
+    records_.push_back(r);
+}
+
+GpuModel::GpuModel(const Inputs &sp) : Model(sp.NZ, sp.NXY, sp.L), sp_(sp), f_noise_(sp.f_noise)
+{
+    ql_params p{};
+    p.model = model_id(sp.equations_to_use);
+    p.integrator = integrator_id(sp.integrator);
+    p.precision = QL_PREC_FP64;
+    p.NX = sp.NXY[0];
+    p.NY = sp.NXY[1];
+    p.NZ = sp.NZ;
+    for (int i = 0; i < 3; ++i) p.L[i] = sp.L[i];
+    p.q = sp.q;
+    p.omega = sp.omega;
+    p.nu = sp.nu;
+    p.eta = sp.eta;
+    p.B0z = sp.B0z;
+    p.viscosity_order = sp.viscosity_order;
+    p.quasilinear = sp.QuasiLinearQ ? 1 : 0;
+    p.remap = sp.remapQ ? 1 : 0;
+    p.dt = sp.dt;
+    p.device = sp.device;
+    check(ql_create(&p, &ctx_));
+    check(ql_dims(ctx_, dims_));
+}
+
+GpuModel::~GpuModel()
+{
+    if (ctx_) ql_destroy(ctx_);
+}
+
+void GpuModel::check(int rc) const
+{
+    if (rc != QL_OK) throw Error(rc, ql_last_error());
+}
+
+void GpuModel::rhs(const double t, const double dt_lin, solution *SolIn, solution *SolOut, doubVec **linOpFluct)
+{
+    SolIn->push();
+    const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs();
+    std::vector<dcmplx> out((size_t)nxy * 4 * nz), omf((size_t)nMF * nz);
+    std::vector<double> L((size_t)nxy * 4 * nz);
+    check(ql_rhs(ctx_, t, dt_lin, reinterpret_cast<double *>(out.data()), reinterpret_cast<double *>(omf.data()),
+                 L.data()));
+    if (SolOut) SolOut->unpack(out, omf);
+    if (linOpFluct)
+        for (int i = 0; i < nxy; ++i)
+            for (int j = 0; j < 4; ++j)
+                for (int m = 0; m < nz; ++m) linOpFluct[i][j](m) = L[((size_t)i * 4 + j) * nz + m];
+}
+
+void GpuModel::linearOPs_Init(double t0, doubVec **linOpFluct, doubVec *linOpMF)
+{
+    const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs();
+    std::vector<double> L((size_t)nxy * 4 * nz), Lmf((size_t)nMF * nz);
+    check(ql_linear_ops_init(ctx_, t0, L.data(), Lmf.data()));
+    if (linOpFluct)
+        for (int i = 0; i < nxy; ++i)
+            for (int j = 0; j < 4; ++j)
+                for (int m = 0; m < nz; ++m) linOpFluct[i][j](m) = L[((size_t)i * 4 + j) * nz + m];
+    if (linOpMF)
+        for (int f = 0; f < nMF; ++f)
+            for (int m = 0; m < nz; ++m) linOpMF[f](m) = Lmf[(size_t)f * nz + m];
+}
+
+void GpuModel::ShearingBox_Remap(double qt, solution *sol)
+{
+    sol->push();
+    check(ql_remap(ctx_, qt));
+    sol->mark_device_newer();
+}
+
+void GpuModel::DrivingNoise(double, double, solution *)
+{
+    // MHD_FullQlin::DrivingNoise (MHD_FullQlin.cpp:786-1053) is not part of the
+    // GPU path; main.cpp:112 calls it only when f_noise != 0.
+    if (f_noise_ != 0.0)
+        throw Error(QL_ERR_UNSUPPORTED, "driving noise (f_noise != 0) is not available on the GPU path");
+}
+
+void GpuModel::ChangeNoiseRange(double fnoise, double, double) { f_noise_ = fnoise; }
+
+void GpuModel::Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *sol)
+{
+    const_cast<solution *>(sol)->push();
+    double out[18];
+    check(ql_diagnostics(ctx_, t, out));
+    for (int k = 0; k < 4; ++k) {
+        tv->current_energy()[k] = out[k];
+        tv->current_AM()[k] = out[4 + k];
+        tv->current_diss()[k] = out[8 + k];
+        tv->current_reynolds()[k] = out[12 + k];
+    }
+    tv->etaK_times_kmax[0] = out[16];
+    tv->etaK_times_kmax[1] = out[17];
+    tv->Save_Data(t);
+}
+
+double GpuModel::Calculate_CFL(const solution *sol)
+{
+    // MHD_FullQlin.cpp:769-780: kmax * (max|By| + max|Bx| + max|Ux| + max|Uy|) + q,
+    // kmax from :92-97; evaluated on the current mean fields.
+    const_cast<solution *>(sol)->push();
+    const int nMF = num_MFs(), NZf = NZfull();
+    std::vector<double> prof((size_t)nMF * NZf);
+    check(ql_mf_real(ctx_, prof.data()));
+    double kmax = 0;
+    const double dealiasfac[3] = {2.0, 2.0, 3.0};
+    for (int i = 0; i < 3; ++i) kmax = std::max(kmax, 2 * kPi / L(i) * (N(i) / dealiasfac[i]));
+    double s = 0;
+    for (int f = 0; f < nMF; ++f) {
+        double mx = 0;
+        for (int z = 0; z < NZf; ++z) mx = std::max(mx, std::fabs(prof[(size_t)f * NZf + z]));
+        s += mx;
+    }
+    return kmax * s + sp_.q;
+}
+
+std::vector<double> GpuModel::kx() const
+{
+    std::vector<double> k(Dimxy());
+    check(ql_get_state(ctx_, nullptr, nullptr, k.data()));
+    return k;
+}
+
+solution::solution(Model *eqs)
+{
+    gm_ = dynamic_cast<GpuModel *>(eqs);
+    if (!gm_) throw Error(QL_ERR_ARG, "solution: the model is not a GpuModel");
+    nxy_ = gm_->Dimxy();
+    nLin_ = gm_->num_Lin();
+    nMF_ = gm_->num_MFs();
+    nz_ = gm_->NZ();
+    lin_.assign((size_t)nxy_ * nLin_, dcmplxVec(nz_));
+    mf_.assign(nMF_, dcmplxVec(nz_));
+}
+
+void solution::pack(std::vector<dcmplx> &lin, std::vector<dcmplx> &mf) const
+{
+    lin.resize((size_t)nxy_ * nLin_ * nz_);
+    mf.resize((size_t)nMF_ * nz_);
+    for (size_t k = 0; k < lin_.size(); ++k) std::copy(lin_[k].data(), lin_[k].data() + nz_, lin.data() + k * nz_);
+    for (int f = 0; f < nMF_; ++f) std::copy(mf_[f].data(), mf_[f].data() + nz_, mf.data() + (size_t)f * nz_);
+}
+
+void solution::unpack(const std::vector<dcmplx> &lin, const std::vector<dcmplx> &mf)
+{
+    for (size_t k = 0; k < lin_.size(); ++k) std::copy(lin.data() + k * nz_, lin.data() + (k + 1) * nz_, lin_[k].data());
+    for (int f = 0; f < nMF_; ++f) std::copy(mf.data() + (size_t)f * nz_, mf.data() + (size_t)(f + 1) * nz_, mf_[f].data());
+    if (gm_->resident == this) gm_->resident = nullptr;  // host copy is now the newer one
+    stale_ = false;
+    dirty_ = true;
+}
+
+void solution::push()
+{
+    if (gm_->resident != this) {
+        if (gm_->resident) gm_->resident->pull();  // keep the previous resident's data
+        gm_->resident = this;
+        dirty_ = true;
+    }
+    if (!dirty_) return;
+    std::vector<dcmplx> lin, mf;
+    pack(lin, mf);
+    gm_->check(ql_set_state(gm_->ctx(), reinterpret_cast<const double *>(lin.data()),
+                            reinterpret_cast<const double *>(mf.data()), nullptr));
+    dirty_ = false;
+    stale_ = false;
+}
+
+void solution::pull() const
+{
+    if (gm_->resident != this || !stale_) return;
+    std::vector<dcmplx> lin((size_t)nxy_ * nLin_ * nz_), mf((size_t)nMF_ * nz_);
+    gm_->check(ql_get_state(gm_->ctx(), reinterpret_cast<double *>(lin.data()), reinterpret_cast<double *>(mf.data()),
+                            nullptr));
+    for (size_t k = 0; k < lin_.size(); ++k) std::copy(lin.data() + k * nz_, lin.data() + (k + 1) * nz_, lin_[k].data());
+    for (int f = 0; f < nMF_; ++f) std::copy(mf.data() + (size_t)f * nz_, mf.data() + (size_t)(f + 1) * nz_, mf_[f].data());
+    stale_ = false;
+}
+
+GpuIntegrator::GpuIntegrator(double t0, const Inputs &SP, GpuModel *mod) : model_(mod), dt_(SP.dt)
+{
+    if (!(SP.dt > 0))
+        throw Error(QL_ERR_UNSUPPORTED, "variable (CFL) time steps are not available on the GPU path");
+    // RK3CN.cpp:57 / RK3CN_Elin.cpp:257
+    model_->linearOPs_Init(t0, nullptr, nullptr);
+}
+
+double GpuIntegrator::Step(double t, solution *sol)
+{
+    sol->push();
+    double dt = 0.0;
+    model_->check(ql_step(model_->ctx(), t, &dt));
+    sol->mark_device_newer();
+    dt_mean_ += dt;
+    ++step_count_;
+    return dt;
+}
+
+}  // namespace qlgpu

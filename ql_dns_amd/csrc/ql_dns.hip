@@ -17,6 +17,7 @@
 
 #include "../../include/ql_dns.h"
 #include "ql_kernels.hpp"
+#include "ql_mode.hpp"
 
 namespace {
 
@@ -50,7 +51,7 @@ constexpr double QL_PI = 3.14159265358979323846;
 
 struct ql_ctx {
     ql_params p;
-    int nxy, nz, nzl, half, NZ, nMF, MB, G, G2, rows_per, vo2;
+    int nxy, nz, nzl, half, NZ, nMF, MB, G, vo2;
     double fac, rsf, kxLfac, kyLfac, kzLfac, omega_eff;
     double remap_thr, remap_shift, totalN2, kzfac;
     hipStream_t s = nullptr;
@@ -60,7 +61,6 @@ struct ql_ctx {
     double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr;
     double *part = nullptr, *part2 = nullptr, *lin_tmp = nullptr, *dpart = nullptr, *dout = nullptr;
     double *mfr_tmp = nullptr;
-    unsigned *counter = nullptr;
     // integrator bookkeeping: where the RK3CN "old" operator was evaluated
     int old_init = 1;
     double qt_old = 0.0, qdtl_old = 0.0;
@@ -112,16 +112,15 @@ void launch_mode(ql_ctx *c, const qlk::ModeArgs &a)
 {
     constexpr int MB = mb_of(NZ);
     if (c->p.model == QL_MODEL_MHD_FULLQLIN)
-        qlk::k_mode_full<NZ, MB><<<c->G, qlk::NT, 0, c->s>>>(a);
+        qlk::k_mode<NZ, MB, qlk::ModelFull><<<c->G, qlk::NT, 0, c->s>>>(a);
     else
-        qlk::k_mode_bq<NZ, MB><<<c->G, qlk::NT, 0, c->s>>>(a);
+        qlk::k_mode<NZ, MB, qlk::ModelBQ><<<c->G, qlk::NT, 0, c->s>>>(a);
 }
 
 template <int NZ>
 void launch_mf(ql_ctx *c, const qlk::MFArgs &a)
 {
-    const int grid = (a.mode == qlk::MODE_PROFILES) ? 1 : c->G2;
-    qlk::k_meanfield<NZ><<<grid, qlk::NT, 0, c->s>>>(a);
+    qlk::k_meanfield<NZ><<<1, qlk::NT, 0, c->s>>>(a);
 }
 
 template <int NZ>
@@ -165,11 +164,7 @@ qlk::ModeArgs mode_args(ql_ctx *c)
 qlk::MFArgs mf_args(ql_ctx *c)
 {
     qlk::MFArgs m{};
-    m.part = c->part;
-    m.nrows = c->G;
-    m.rows_per = c->rows_per;
-    m.part2 = c->part2;
-    m.counter = c->counter;
+    m.sums = c->part2;
     m.mf = c->mf;
     m.drive = c->drive;
     m.mfz = c->mfz;
@@ -187,7 +182,7 @@ qlk::MFArgs mf_args(ql_ctx *c)
     return m;
 }
 
-// mode kernel -> mean-field kernel (which reduces the mode partials)
+// mode kernel -> column reduction of its partials -> mean-field kernel
 int run_sequence(ql_ctx *c, const qlk::ModeArgs &a, const qlk::MFArgs &m)
 {
     int rc;
@@ -196,6 +191,9 @@ int run_sequence(ql_ctx *c, const qlk::ModeArgs &a, const qlk::MFArgs &m)
     QL_HIP(hipGetLastError());
     if ((rc = end_timing(c, 0))) return rc;
     if ((rc = begin_timing(c))) return rc;
+    const int ncols = c->nMF * c->NZ;
+    qlk::k_reduce_cols<<<(ncols + 15) / 16, qlk::NT, 0, c->s>>>(c->part, c->G, ncols, c->part2);
+    QL_HIP(hipGetLastError());
     QL_DISPATCH_NZ(c->NZ, launch_mf, c, m);
     QL_HIP(hipGetLastError());
     return end_timing(c, 1);
@@ -284,8 +282,7 @@ int dalloc(T **p, size_t n)
 void release(ql_ctx *c)
 {
     void *bufs[] = {c->u, c->N1, c->N2, c->mf, c->mfN1, c->mfN2, c->drive, c->tw, c->kx0, c->ky,
-                    c->kzv, c->mfz, c->part, c->part2, c->lin_tmp, c->dpart, c->dout, c->mfr_tmp,
-                    c->counter};
+                    c->kzv, c->mfz, c->part, c->part2, c->lin_tmp, c->dpart, c->dout, c->mfr_tmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &e : c->ev)
@@ -350,8 +347,6 @@ int ql_create(const ql_params *p, ql_ctx **out)
     c->MB = mb_for(p->NZ);
     const int ngroups = (c->nxy + c->MB - 1) / c->MB;
     c->G = ngroups < QL_GMAX ? ngroups : QL_GMAX;
-    c->rows_per = 16;
-    c->G2 = (c->G + c->rows_per - 1) / c->rows_per;
 
     // wavenumber tables, Kdata.cpp:28-80
     const int nx = p->NX / 2, ny = p->NY / 2;
@@ -393,11 +388,10 @@ int ql_create(const ql_params *p, ql_ctx **out)
     if (!rc) rc = dalloc(&c->kzv, (size_t)c->nz);
     if (!rc) rc = dalloc(&c->mfz, (size_t)8 * c->NZ);
     if (!rc) rc = dalloc(&c->part, (size_t)c->G * c->nMF * c->NZ);
-    if (!rc) rc = dalloc(&c->part2, (size_t)c->G2 * c->nMF * c->NZ);
+    if (!rc) rc = dalloc(&c->part2, (size_t)c->nMF * c->NZ);
     if (!rc) rc = dalloc(&c->dpart, (size_t)256 * 6);
     if (!rc) rc = dalloc(&c->dout, (size_t)18);
     if (!rc) rc = dalloc(&c->mfr_tmp, (size_t)4 * c->NZ);
-    if (!rc) rc = dalloc(&c->counter, (size_t)1);
     if (!rc && hipMemcpy(c->kx0, c->h_kx0.data(), sizeof(double) * c->nxy, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(QL_ERR_HIP, "table upload failed");
     if (!rc && hipMemcpy(c->ky, c->h_ky.data(), sizeof(double) * c->nxy, hipMemcpyHostToDevice) != hipSuccess)

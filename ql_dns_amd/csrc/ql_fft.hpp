@@ -132,6 +132,42 @@ __device__ __forceinline__ void fft_lines(double2 *buf, const double2 *tw)
     }
 }
 
+// As fft_lines, but the first pass takes its inputs from src(line, grid index)
+// instead of buf (so the input lines never have to be materialised in LDS).
+// The caller must have synchronised before buf may be overwritten.
+template <int NZ, int NL, bool INV, class Src>
+__device__ __forceinline__ void fft_lines_from(double2 *buf, const double2 *tw, const Src &src)
+{
+    constexpr int R = (NZ >= 8) ? 8 : ((NZ >= 4) ? 4 : 2);
+    constexpr int M = NZ / R;
+    constexpr int TOT = NL * M;
+    constexpr int IT = (TOT + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int item = threadIdx.x + it * NT;
+        if (TOT % NT == 0 || item < TOT) {
+            const int line = item / M, j = item % M;
+            double2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = src(line, j + r * M);
+            if constexpr (R == 8) {
+                dft8<INV>(v);
+            } else if constexpr (R == 4) {
+                dft4<INV>(v[0], v[1], v[2], v[3]);
+            } else {
+                const double2 x = v[0], y = v[1];
+                v[0] = cadd(x, y);
+                v[1] = csub(x, y);
+            }
+            double2 *dst = buf + line * NZ + j * R;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r] = v[r];
+        }
+    }
+    __syncthreads();
+    fft_lines<NZ, NL, INV, R>(buf, tw);
+}
+
 // dealiased sizes fixed by NZ (Models/Model.h:34: nz = 1 + 2*int(NZ/3))
 template <int NZ>
 struct Dealias {

@@ -1,0 +1,62 @@
+"""The C++ Model / Integrator / solution mirror (ql_dns_amd/csrc/host) driven
+like the reference's main loop, against the oracle (GPU)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, rel_err
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(ROOT, "tests", "cpp", "test_host_adapter")
+
+
+@pytest.mark.parametrize("model,integrator", [("MHD_FullQlin", "RK3CN_Elin"), ("MHD_FullQlin", "RK3CN"),
+                                              ("MHD_BQlin", "RK3CN_Elin")])
+def test_main_loop_through_cpp_adapter(tmp_path, model, integrator):
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import INTEGRATOR_NAMES, MODEL_NAMES, workload
+    from oracle.oracle import Oracle
+    p = workload("box16x16x32", model=MODEL_NAMES[model], integrator=INTEGRATOR_NAMES[integrator])
+    lin, mf = initial_state(p)
+    src = tmp_path / "state.bin"
+    out = tmp_path / "out.bin"
+    with open(src, "wb") as f:
+        f.write(np.ascontiguousarray(lin).tobytes())
+        f.write(np.ascontiguousarray(mf).tobytes())
+    nsteps = 50
+    r = subprocess.run([BIN, model, integrator, str(nsteps), str(src), str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    nl, nm = p.nxy * 4 * p.nz, p.nMF * p.nz
+    o_ = 0
+
+    def take(n, cplx=True):
+        nonlocal o_
+        k = 2 * n if cplx else n
+        a = raw[o_:o_ + k]
+        o_ += k
+        return a.view(np.complex128) if cplx else a
+    rhs_lin, rhs_mf, linop = take(nl), take(nm), take(nl, False)
+    fl, fm, kx, t, diag = take(nl), take(nm), take(p.nxy, False), take(1, False)[0], take(18, False)
+    assert o_ == raw.size
+
+    o = Oracle.from_params(p, nthreads=8)
+    o.set_state(lin, mf)
+    o.linear_ops_init(0.0)
+    ro = o.rhs(0.0, 0.0053)
+    assert rel_err(rhs_lin, ro[0].ravel()) < 1e-10
+    assert rel_err(rhs_mf, ro[1].ravel()) < 1e-10
+    assert rel_err(linop, ro[2].ravel()) < 1e-10
+    to = o.steps(nsteps, 0.0)
+    lo, mo, ko = o.get_state()
+    assert t == to
+    assert np.array_equal(kx, ko)
+    assert rel_err(fl, lo.ravel()) < 1e-10
+    assert rel_err(fm, mo.ravel()) < 1e-10
+    do = o.diagnostics(to)
+    fin = np.isfinite(do)
+    assert np.all(np.abs(diag[fin] - do[fin]) <= 1e-10 * np.abs(do[fin]) + 1e-300)
