@@ -58,6 +58,7 @@ struct ql_ctx {
     // device state and tables
     double2 *u = nullptr, *N1 = nullptr, *N2 = nullptr;
     double2 *mf = nullptr, *mfN1 = nullptr, *mfN2 = nullptr, *drive = nullptr, *tw = nullptr;
+    float2 *tw32 = nullptr;
     double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr;
     double *part = nullptr, *part2 = nullptr, *lin_tmp = nullptr, *dpart = nullptr, *dout = nullptr;
     double *mfr_tmp = nullptr;
@@ -111,10 +112,14 @@ template <int NZ>
 void launch_mode(ql_ctx *c, const qlk::ModeArgs &a)
 {
     constexpr int MB = mb_of(NZ);
-    if (c->p.model == QL_MODEL_MHD_FULLQLIN)
-        qlk::k_mode<NZ, MB, qlk::ModelFull><<<c->G, qlk::NT, 0, c->s>>>(a);
-    else
-        qlk::k_mode<NZ, MB, qlk::ModelBQ><<<c->G, qlk::NT, 0, c->s>>>(a);
+    const bool full = (c->p.model == QL_MODEL_MHD_FULLQLIN);
+    if (c->p.precision == QL_PREC_MIXED) {
+        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull><<<c->G, qlk::NT, 0, c->s>>>(a);
+        else qlk::k_mode32<NZ, MB, qlk::ModelBQ><<<c->G, qlk::NT, 0, c->s>>>(a);
+    } else {
+        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull><<<c->G, qlk::NT, 0, c->s>>>(a);
+        else qlk::k_mode64<NZ, MB, qlk::ModelBQ><<<c->G, qlk::NT, 0, c->s>>>(a);
+    }
 }
 
 template <int NZ>
@@ -145,6 +150,7 @@ qlk::ModeArgs mode_args(ql_ctx *c)
     a.ky = c->ky;
     a.kzv = c->kzv;
     a.tw = c->tw;
+    a.tw32 = c->tw32;
     a.mfz = c->mfz;
     a.partials = c->part;
     a.nxy = c->nxy;
@@ -281,7 +287,7 @@ int dalloc(T **p, size_t n)
 
 void release(ql_ctx *c)
 {
-    void *bufs[] = {c->u, c->N1, c->N2, c->mf, c->mfN1, c->mfN2, c->drive, c->tw, c->kx0, c->ky,
+    void *bufs[] = {c->u, c->N1, c->N2, c->mf, c->mfN1, c->mfN2, c->drive, c->tw, c->tw32, c->kx0, c->ky,
                     c->kzv, c->mfz, c->part, c->part2, c->lin_tmp, c->dpart, c->dout, c->mfr_tmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -304,7 +310,7 @@ int ql_create(const ql_params *p, ql_ctx **out)
         return fail(QL_ERR_ARG, "ql_create: unknown model %d", p->model);
     if (p->integrator != QL_INT_RK3CN && p->integrator != QL_INT_RK3CN_ELIN)
         return fail(QL_ERR_ARG, "ql_create: unknown integrator %d", p->integrator);
-    if (p->precision != QL_PREC_FP64)
+    if (p->precision != QL_PREC_FP64 && p->precision != QL_PREC_MIXED)
         return fail(QL_ERR_UNSUPPORTED, "ql_create: precision %d is not available in this build", p->precision);
     if (!valid_NZ(p->NZ)) return fail(QL_ERR_ARG, "ql_create: NZ=%d not in {32,64,128,256}", p->NZ);
     if (p->NX < 2 || p->NY < 2 || (p->NX & 1) || (p->NY & 1))
@@ -366,9 +372,11 @@ int ql_create(const ql_params *p, ql_ctx **out)
     for (int i = 0; i < c->nzl + 1; ++i) c->h_kz[i] = i * c->kzLfac;
     for (int i = c->nzl + 1; i < 2 * c->nzl + 1; ++i) c->h_kz[i] = (-2 * c->nzl + i - 1) * c->kzLfac;
     std::vector<double2> htw(c->NZ);
+    std::vector<float2> htw32(c->NZ);
     for (int k = 0; k < c->NZ; ++k) {
         const double ang = 2.0 * QL_PI * (double)k / (double)c->NZ;
         htw[k] = make_double2(cos(ang), -sin(ang));
+        htw32[k] = make_float2((float)htw[k].x, (float)htw[k].y);
     }
 
     int rc = QL_OK;
@@ -383,6 +391,7 @@ int ql_create(const ql_params *p, ql_ctx **out)
     if (!rc) rc = dalloc(&c->mfN2, (size_t)c->nMF * c->nz);
     if (!rc) rc = dalloc(&c->drive, (size_t)c->nMF * c->nz);
     if (!rc) rc = dalloc(&c->tw, (size_t)c->NZ);
+    if (!rc) rc = dalloc(&c->tw32, (size_t)c->NZ);
     if (!rc) rc = dalloc(&c->kx0, (size_t)c->nxy);
     if (!rc) rc = dalloc(&c->ky, (size_t)c->nxy);
     if (!rc) rc = dalloc(&c->kzv, (size_t)c->nz);
@@ -399,6 +408,8 @@ int ql_create(const ql_params *p, ql_ctx **out)
     if (!rc && hipMemcpy(c->kzv, c->h_kz.data(), sizeof(double) * c->nz, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(QL_ERR_HIP, "table upload failed");
     if (!rc && hipMemcpy(c->tw, htw.data(), sizeof(double2) * c->NZ, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(QL_ERR_HIP, "table upload failed");
+    if (!rc && hipMemcpy(c->tw32, htw32.data(), sizeof(float2) * c->NZ, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(QL_ERR_HIP, "table upload failed");
     if (!rc && (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess))
         rc = fail(QL_ERR_HIP, "event create failed");

@@ -7,7 +7,9 @@
 // autosort schedule of radix-8 passes (then one radix-4 or radix-2 pass for
 // the remaining factor).  Each pass reads every element of every line into registers, waits at
 // a barrier, and writes the butterflies back, so a single LDS buffer suffices.
-// Twiddles come from a table tw[k] = exp(-2 pi i k / NZ) staged in LDS.
+// Twiddles come from a table tw[k] = exp(-2 pi i k / NZ) staged in LDS.  The
+// complex type C is double2 (FP64 path) or float2 (FP32 transforms of the
+// mixed-precision path).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -15,24 +17,47 @@ namespace qlk {
 
 constexpr int NT = 256;  // threads per workgroup of every kernel here
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
-{
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
-// (i k) * a
-__device__ __forceinline__ double2 imul(double k, double2 a) { return make_double2(-k * a.y, k * a.x); }
-// Re(a * conj(b))
-__device__ __forceinline__ double redot(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
+// complex arithmetic on double2 (FP64 path) and float2 (FP32 transforms)
+template <class C>
+struct Cx;
+template <>
+struct Cx<double2> {
+    typedef double R;
+    __device__ static __forceinline__ double2 make(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct Cx<float2> {
+    typedef float R;
+    __device__ static __forceinline__ float2 make(float x, float y) { return make_float2(x, y); }
+};
+template <class C>
+using RealOf = typename Cx<C>::R;
 
-template <bool INV>
-__device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, double2 &v3)
+template <class C>
+__device__ __forceinline__ C cadd(C a, C b) { return Cx<C>::make(a.x + b.x, a.y + b.y); }
+template <class C>
+__device__ __forceinline__ C csub(C a, C b) { return Cx<C>::make(a.x - b.x, a.y - b.y); }
+template <class C>
+__device__ __forceinline__ C cmul(C a, C b) { return Cx<C>::make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <class C>
+__device__ __forceinline__ C cscale(RealOf<C> s, C a) { return Cx<C>::make(s * a.x, s * a.y); }
+// (i k) * a
+template <class C>
+__device__ __forceinline__ C imul(RealOf<C> k, C a) { return Cx<C>::make(-k * a.y, k * a.x); }
+// Re(a * conj(b))
+template <class C>
+__device__ __forceinline__ RealOf<C> redot(C a, C b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ double2 to_d(float2 a) { return make_double2(a.x, a.y); }
+__device__ __forceinline__ double2 to_d(double2 a) { return a; }
+template <class C>
+__device__ __forceinline__ C from_d(double2 a) { return Cx<C>::make((RealOf<C>)a.x, (RealOf<C>)a.y); }
+
+template <bool INV, class C>
+__device__ __forceinline__ void dft4(C &v0, C &v1, C &v2, C &v3)
 {
-    const double2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = csub(v1, v3);
+    const C a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = csub(v1, v3);
     // forward: -i * a3, inverse: +i * a3
-    const double2 ja3 = INV ? make_double2(-a3.y, a3.x) : make_double2(a3.y, -a3.x);
+    const C ja3 = INV ? Cx<C>::make(-a3.y, a3.x) : Cx<C>::make(a3.y, -a3.x);
     v0 = cadd(a0, a2);
     v2 = csub(a0, a2);
     v1 = cadd(a1, ja3);
@@ -41,11 +66,12 @@ __device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, doub
 
 // 8-point DFT as two 4-point DFTs (radix-2 split): even outputs from
 // v[k] + v[k+4], odd outputs from (v[k] - v[k+4]) * w8^k.
-template <bool INV>
-__device__ __forceinline__ void dft8(double2 *v)
+template <bool INV, class C>
+__device__ __forceinline__ void dft8(C *v)
 {
-    constexpr double H = 0.70710678118654752440;  // 1/sqrt(2)
-    double2 a[4], b[4];
+    typedef RealOf<C> R;
+    const R H = (R)0.70710678118654752440;  // 1/sqrt(2)
+    C a[4], b[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         a[k] = cadd(v[k], v[k + 4]);
@@ -53,13 +79,13 @@ __device__ __forceinline__ void dft8(double2 *v)
     }
     // w8^1 = (1 -/+ i)/sqrt2, w8^2 = -/+ i, w8^3 = (-1 -/+ i)/sqrt2
     if (INV) {
-        b[1] = make_double2(H * (b[1].x - b[1].y), H * (b[1].x + b[1].y));
-        b[2] = make_double2(-b[2].y, b[2].x);
-        b[3] = make_double2(-H * (b[3].x + b[3].y), H * (b[3].x - b[3].y));
+        b[1] = Cx<C>::make(H * (b[1].x - b[1].y), H * (b[1].x + b[1].y));
+        b[2] = Cx<C>::make(-b[2].y, b[2].x);
+        b[3] = Cx<C>::make(-H * (b[3].x + b[3].y), H * (b[3].x - b[3].y));
     } else {
-        b[1] = make_double2(H * (b[1].x + b[1].y), H * (b[1].y - b[1].x));
-        b[2] = make_double2(b[2].y, -b[2].x);
-        b[3] = make_double2(H * (b[3].y - b[3].x), -H * (b[3].x + b[3].y));
+        b[1] = Cx<C>::make(H * (b[1].x + b[1].y), H * (b[1].y - b[1].x));
+        b[2] = Cx<C>::make(b[2].y, -b[2].x);
+        b[3] = Cx<C>::make(H * (b[3].y - b[3].x), -H * (b[3].x + b[3].y));
     }
     dft4<INV>(a[0], a[1], a[2], a[3]);
     dft4<INV>(b[0], b[1], b[2], b[3]);
@@ -71,19 +97,19 @@ __device__ __forceinline__ void dft8(double2 *v)
 }
 
 // One Stockham pass of radix R at stride NS over NL lines held in buf.
-template <int NZ, int NL, bool INV, int R, int NS>
-__device__ __forceinline__ void fft_pass(double2 *buf, const double2 *tw)
+template <int NZ, int NL, bool INV, int R, int NS, class C>
+__device__ __forceinline__ void fft_pass(C *buf, const C *tw)
 {
     constexpr int M = NZ / R;
     constexpr int TOT = NL * M;
     constexpr int IT = (TOT + NT - 1) / NT;
-    double2 v[IT][R];
+    C v[IT][R];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int item = threadIdx.x + it * NT;
         if (TOT % NT == 0 || item < TOT) {
             const int line = item / M, j = item % M;
-            const double2 *src = buf + line * NZ + j;
+            const C *src = buf + line * NZ + j;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[it][r] = src[r * M];
         }
@@ -98,7 +124,7 @@ __device__ __forceinline__ void fft_pass(double2 *buf, const double2 *tw)
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    double2 w = tw[k * r * (NZ / (NS * R))];
+                    C w = tw[k * r * (NZ / (NS * R))];
                     if (INV) w.y = -w.y;
                     v[it][r] = cmul(v[it][r], w);
                 }
@@ -108,11 +134,11 @@ __device__ __forceinline__ void fft_pass(double2 *buf, const double2 *tw)
             } else if constexpr (R == 4) {
                 dft4<INV>(v[it][0], v[it][1], v[it][2], v[it][3]);
             } else {
-                const double2 a = v[it][0], b = v[it][1];
+                const C a = v[it][0], b = v[it][1];
                 v[it][0] = cadd(a, b);
                 v[it][1] = csub(a, b);
             }
-            double2 *dst = buf + line * NZ + (j - k) * R + k;
+            C *dst = buf + line * NZ + (j - k) * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[r * NS] = v[it][r];
         }
@@ -122,8 +148,8 @@ __device__ __forceinline__ void fft_pass(double2 *buf, const double2 *tw)
 
 // Full length-NZ transform of NL lines (INV: exp(+i), else exp(-i)),
 // unnormalised.  The caller must have synchronised after filling buf.
-template <int NZ, int NL, bool INV, int NS = 1>
-__device__ __forceinline__ void fft_lines(double2 *buf, const double2 *tw)
+template <int NZ, int NL, bool INV, int NS = 1, class C>
+__device__ __forceinline__ void fft_lines(C *buf, const C *tw)
 {
     if constexpr (NS < NZ) {
         constexpr int R = (NZ / NS >= 8) ? 8 : ((NZ / NS >= 4) ? 4 : 2);
@@ -135,8 +161,8 @@ __device__ __forceinline__ void fft_lines(double2 *buf, const double2 *tw)
 // As fft_lines, but the first pass takes its inputs from src(line, grid index)
 // instead of buf (so the input lines never have to be materialised in LDS).
 // The caller must have synchronised before buf may be overwritten.
-template <int NZ, int NL, bool INV, class Src>
-__device__ __forceinline__ void fft_lines_from(double2 *buf, const double2 *tw, const Src &src)
+template <int NZ, int NL, bool INV, class C, class Src>
+__device__ __forceinline__ void fft_lines_from(C *buf, const C *tw, const Src &src)
 {
     constexpr int R = (NZ >= 8) ? 8 : ((NZ >= 4) ? 4 : 2);
     constexpr int M = NZ / R;
@@ -147,7 +173,7 @@ __device__ __forceinline__ void fft_lines_from(double2 *buf, const double2 *tw, 
         const int item = threadIdx.x + it * NT;
         if (TOT % NT == 0 || item < TOT) {
             const int line = item / M, j = item % M;
-            double2 v[R];
+            C v[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) v[r] = src(line, j + r * M);
             if constexpr (R == 8) {
@@ -155,11 +181,11 @@ __device__ __forceinline__ void fft_lines_from(double2 *buf, const double2 *tw, 
             } else if constexpr (R == 4) {
                 dft4<INV>(v[0], v[1], v[2], v[3]);
             } else {
-                const double2 x = v[0], y = v[1];
+                const C x = v[0], y = v[1];
                 v[0] = cadd(x, y);
                 v[1] = csub(x, y);
             }
-            double2 *dst = buf + line * NZ + j * R;
+            C *dst = buf + line * NZ + j * R;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[r] = v[r];
         }

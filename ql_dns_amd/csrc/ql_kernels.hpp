@@ -22,16 +22,7 @@ namespace qlk {
 
 enum { MODE_RHS = 0, MODE_ELIN = 1, MODE_CN = 2, MODE_PROFILES = 3 };
 
-// Occupancy request for the per-mode kernels (waves per SIMD); bounds their
-// VGPR budget (2 waves: 256 VGPRs).  0 leaves the choice to the compiler.
-#ifndef QL_MODE_WAVES
-#define QL_MODE_WAVES 2
-#endif
-#if QL_MODE_WAVES > 0
-#define QL_MODE_ATTR __attribute__((amdgpu_waves_per_eu(QL_MODE_WAVES)))
-#else
-#define QL_MODE_ATTR
-#endif
+
 
 struct ModeArgs {
     double2 *u;             // state [nxy][4][nz], read and (MODE_ELIN/CN) updated in place
@@ -42,6 +33,7 @@ struct ModeArgs {
     const double *ky;       // [nxy]
     const double *kzv;      // [nz] physical kz
     const double2 *tw;      // [NZ] exp(-2 pi i k/NZ)
+    const float2 *tw32;     // the same, rounded to FP32
     const double *mfz;      // real-space mean-field profiles [.][NZ]
     double *partials;       // [gridDim.x][nMF][NZ]
     int nxy, vo2, mode, old_init, do_remap;
@@ -72,35 +64,48 @@ __device__ __forceinline__ bool remap_hit(const ModeArgs &a, double kx0, double 
 
 // ---------------------------------------------------------------------------
 // stress / EMF reduction: column c of the mode-kernel partials summed over all
-// rows.  A block owns 16 columns; its 16 row chunks are summed in row order by
-// separate threads and then added in chunk order, so the result does not
-// depend on scheduling.  No cross-block communication (no fences, no atomics).
+// rows with Neumaier compensation.  A block owns 16 columns; its 16 row chunks
+// are summed in row order by separate threads and then added in chunk order,
+// so the result does not depend on scheduling.  No cross-block communication
+// (no fences, no atomics).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, int nrows, int ncols,
                                                     double *__restrict__ sums)
 {
-    __shared__ double red[16][17];
+    __shared__ double red[16][17], red2[16][17];
     const int cl = threadIdx.x % 16, chunk = threadIdx.x / 16;
     const int c = blockIdx.x * 16 + cl;
     const int per = (nrows + 15) / 16;
     const int r0 = chunk * per, r1 = min(nrows, r0 + per);
     constexpr int B = 16;
-    double s = 0.0;
+    // Neumaier-compensated running sum (s, e)
+    double s = 0.0, e = 0.0;
     if (c < ncols) {
         for (int r = r0; r < r1; r += B) {
             double v[B];
 #pragma unroll
             for (int k = 0; k < B; ++k) v[k] = (r + k < r1) ? part[(size_t)(r + k) * ncols + c] : 0.0;
 #pragma unroll
-            for (int k = 0; k < B; ++k) s += v[k];
+            for (int k = 0; k < B; ++k) {
+                const double t = s + v[k];
+                e += (fabs(s) >= fabs(v[k])) ? ((s - t) + v[k]) : ((v[k] - t) + s);
+                s = t;
+            }
         }
     }
     red[chunk][cl] = s;
+    red2[chunk][cl] = e;
     __syncthreads();
     if (chunk == 0 && c < ncols) {
-        double t = 0.0;
-        for (int k = 0; k < 16; ++k) t += red[k][cl];
-        sums[c] = t;
+        double t = 0.0, te = 0.0;
+        for (int k = 0; k < 16; ++k) {
+            const double v = red[k][cl];
+            const double u = t + v;
+            te += (fabs(t) >= fabs(v)) ? ((t - u) + v) : ((v - u) + t);
+            te += red2[k][cl];
+            t = u;
+        }
+        sums[c] = t + te;
     }
 }
 

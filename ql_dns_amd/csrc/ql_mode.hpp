@@ -31,8 +31,21 @@
 #ifndef QL_PREFETCH_NPREV
 #define QL_PREFETCH_NPREV 0
 #endif
+#ifndef QL_WAVES64
+#define QL_WAVES64 2
+#endif
+#ifndef QL_WAVES32
+#define QL_WAVES32 2
+#endif
 
 namespace qlk {
+
+// Re(a conj b) formed in FP64 (exact for FP32 inputs)
+template <class C>
+__device__ __forceinline__ double redot64(C a, C b)
+{
+    return (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+}
 
 // y and z components of a divergence-free field from (fx, zeta-like):
 // ((ky kx) f + i kz z) ilap2 and ((kx kz) f - i ky z) ilap2
@@ -53,58 +66,68 @@ struct ModelFull {
     static constexpr int NACC = 4;   // EMF x, EMF y, stress x, stress y
     __device__ static int base_of(int l) { return l < 6 ? l : (((l & 1) ? 3 : 0) + (l >= 8 ? 1 : 0)); }
     __device__ static bool dz_of(int l) { return l >= 6; }
+    template <class C>
     __device__ static void bases(double2 ux, double2 ze, double2 bx, double2 et, double kx, double ky, double kz,
-                                 double ilap2, double fac, double2 *out, int stride)
+                                 double ilap2, double fac, C *out, int stride)
     {
         double2 uy, uz, by, bz;
         yz_from(ux, ze, kx, ky, kz, ilap2, uy, uz);
         yz_from(bx, et, kx, ky, kz, ilap2, by, bz);
-        out[0] = cscale(fac, ux);
-        out[stride] = cscale(fac, uy);
-        out[2 * stride] = cscale(fac, uz);
-        out[3 * stride] = cscale(fac, bx);
-        out[4 * stride] = cscale(fac, by);
-        out[5 * stride] = cscale(fac, bz);
+        out[0] = from_d<C>(cscale(fac, ux));
+        out[stride] = from_d<C>(cscale(fac, uy));
+        out[2 * stride] = from_d<C>(cscale(fac, uz));
+        out[3 * stride] = from_d<C>(cscale(fac, bx));
+        out[4 * stride] = from_d<C>(cscale(fac, by));
+        out[5 * stride] = from_d<C>(cscale(fac, bz));
     }
     __device__ static int nprod(int) { return 3; }
     __device__ static int out_line(int part, int c) { return part * 3 + c; }
     __device__ static int acc_slot(int part, int c) { return part * 2 + c; }
     // MHD_FullQlin.cpp:271-300 (products) and :322-358 (EMF, stresses)
-    __device__ static void products(int part, const double2 *X, int ld, const double *mf, double kx, double ky,
-                                    double2 *prod, double *racc)
+    template <class C>
+    __device__ static void products(int part, const C *X, int ld, const RealOf<C> *mf, RealOf<C> kx, RealOf<C> ky,
+                                    C *prod, double *racc)
     {
-        const double2 u0 = X[0], u1 = X[ld], u2 = X[2 * ld];
-        const double2 b0 = X[3 * ld], b1 = X[4 * ld], b2 = X[5 * ld];
-        const double MBx = mf[0], dzMBx = mf[1], MBy = mf[2], dzMBy = mf[3];
-        const double MUx = mf[4], dzMUx = mf[5], MUy = mf[6], dzMUy = mf[7];
-        const double aU = kx * MUx + ky * MUy, aB = kx * MBx + ky * MBy;
+        typedef RealOf<C> R;
+        const C u0 = X[0], u1 = X[ld], u2 = X[2 * ld];
+        const C b0 = X[3 * ld], b1 = X[4 * ld], b2 = X[5 * ld];
+        const R MBx = mf[0], dzMBx = mf[1], MBy = mf[2], dzMBy = mf[3];
+        const R MUx = mf[4], dzMUx = mf[5], MUy = mf[6], dzMUy = mf[7];
+        const R aU = kx * MUx + ky * MUy, aB = kx * MBx + ky * MBy;
         if (part == 0) {
             // U.grad(u) + uz dzU - (B.grad(b) + bz dzB)
             prod[0] = csub(cadd(imul(aU, u0), cscale(dzMUx, u2)), cadd(imul(aB, b0), cscale(dzMBx, b2)));
             prod[1] = csub(cadd(imul(aU, u1), cscale(dzMUy, u2)), cadd(imul(aB, b1), cscale(dzMBy, b2)));
             prod[2] = csub(imul(aU, u2), imul(aB, b2));
-            racc[0] = redot(b2, u0) - redot(u2, b0);
-            racc[1] = redot(b2, u1) - redot(u2, b1);
+            racc[0] = redot64(b2, u0) - redot64(u2, b0);
+            racc[1] = redot64(b2, u1) - redot64(u2, b1);
         } else {
             // U.grad(b) + uz dzB - (B.grad(u) + bz dzU)
             prod[0] = csub(cadd(imul(aU, b0), cscale(dzMBx, u2)), cadd(imul(aB, u0), cscale(dzMUx, b2)));
             prod[1] = csub(cadd(imul(aU, b1), cscale(dzMBy, u2)), cadd(imul(aB, u1), cscale(dzMUy, b2)));
             prod[2] = csub(imul(aU, b2), imul(aB, u2));
-            const double2 dzux = X[6 * ld], dzbx = X[7 * ld], dzuy = X[8 * ld], dzby = X[9 * ld];
-            const double2 u3 = imul(kx, u0), u6 = imul(ky, u0), b3 = imul(kx, b0), b6 = imul(ky, b0);
-            const double2 u4 = imul(kx, u1), u7 = imul(ky, u1), b4 = imul(kx, b1), b7 = imul(ky, b1);
-            racc[0] = -(redot(u0, u3) + redot(u1, u6) + redot(u2, dzux)) + (redot(b0, b3) + redot(b1, b6) + redot(b2, dzbx));
-            racc[1] = -(redot(u0, u4) + redot(u1, u7) + redot(u2, dzuy)) + (redot(b0, b4) + redot(b1, b7) + redot(b2, dzby));
+            const C dzux = X[6 * ld], dzbx = X[7 * ld], dzuy = X[8 * ld], dzby = X[9 * ld];
+            // x and y derivatives i kx f, i ky f formed in FP64
+            const double kxd = kx, kyd = ky;
+            const double2 U0 = to_d(u0), U1 = to_d(u1), B0 = to_d(b0), B1 = to_d(b1);
+            const double2 u3 = imul(kxd, U0), u6 = imul(kyd, U0), b3 = imul(kxd, B0), b6 = imul(kyd, B0);
+            const double2 u4 = imul(kxd, U1), u7 = imul(kyd, U1), b4 = imul(kxd, B1), b7 = imul(kyd, B1);
+            const double2 U2 = to_d(u2), B2 = to_d(b2);
+            racc[0] = -(redot(U0, u3) + redot(U1, u6) + redot64(u2, dzux)) + (redot(B0, b3) + redot(B1, b6) + redot64(b2, dzbx));
+            racc[1] = -(redot(U0, u4) + redot(U1, u7) + redot64(u2, dzuy)) + (redot(B0, b4) + redot(B1, b7) + redot64(b2, dzby));
+            (void)U2;
+            (void)B2;
         }
     }
     // MHD_FullQlin.cpp:283-285 (pair 0: ux, zeta) and :303-305 (pair 1: bx, eta)
-    __device__ static void rhs_pair(int pr, const double2 *T, int ld, double2 ux, double2 ze, double2 bx, double2 et,
+    template <class C>
+    __device__ static void rhs_pair(int pr, const C *T, int ld, double2 ux, double2 ze, double2 bx, double2 et,
                                     double kx, double ky, double kz, double ilapF, const ModeArgs &a, double2 &N0,
                                     double2 &N1)
     {
         const double q = a.q, Om = a.omega, B0z = a.B0z;
         if (pr == 0) {
-            const double2 T1 = T[0], T2 = T[ld], T3 = T[2 * ld];
+            const double2 T1 = to_d(T[0]), T2 = to_d(T[ld]), T3 = to_d(T[2 * ld]);
             double2 s = cscale(2.0 * kx * ky * q, ux);
             s = cadd(s, imul(2.0 * Om * kz, ze));
             s = csub(s, cscale(kx * kx, T1));
@@ -114,8 +137,8 @@ struct ModelFull {
             N1 = cadd(imul((q - 2.0 * Om) * kz, ux), imul(B0z * kz, et));
             N1 = csub(N1, csub(imul(kz, T2), imul(ky, T3)));
         } else {
-            const double2 T2 = T[4 * ld], T3 = T[5 * ld];
-            N0 = csub(imul(B0z * kz, ux), T[3 * ld]);
+            const double2 T2 = to_d(T[4 * ld]), T3 = to_d(T[5 * ld]);
+            N0 = csub(imul(B0z * kz, ux), to_d(T[3 * ld]));
             N1 = cadd(imul(-q * kz, bx), imul(B0z * kz, ze));
             N1 = csub(N1, csub(imul(kz, T2), imul(ky, T3)));
         }
@@ -139,60 +162,64 @@ struct ModelBQ {
     static constexpr int NACC = 2;  // EMF x, EMF y
     __device__ static int base_of(int l) { return l < 8 ? l : 5; }
     __device__ static bool dz_of(int l) { return l == 8; }
+    template <class C>
     __device__ static void bases(double2 ux, double2 ze, double2 bx, double2 et, double kx, double ky, double kz,
-                                 double ilap2, double fac, double2 *out, int stride)
+                                 double ilap2, double fac, C *out, int stride)
     {
         double2 uy, uz, by, bz;
         yz_from(ux, ze, kx, ky, kz, ilap2, uy, uz);  // MHD_BQlin.cpp:143-146
         yz_from(bx, et, kx, ky, kz, ilap2, by, bz);
-        out[0] = cscale(fac, ux);
-        out[stride] = cscale(fac, ze);
-        out[2 * stride] = cscale(fac, bx);
-        out[3 * stride] = cscale(fac, et);
-        out[4 * stride] = cscale(fac, uy);
-        out[5 * stride] = cscale(fac, uz);
-        out[6 * stride] = cscale(fac, by);
-        out[7 * stride] = cscale(fac, bz);
+        out[0] = from_d<C>(cscale(fac, ux));
+        out[stride] = from_d<C>(cscale(fac, ze));
+        out[2 * stride] = from_d<C>(cscale(fac, bx));
+        out[3 * stride] = from_d<C>(cscale(fac, et));
+        out[4 * stride] = from_d<C>(cscale(fac, uy));
+        out[5 * stride] = from_d<C>(cscale(fac, uz));
+        out[6 * stride] = from_d<C>(cscale(fac, by));
+        out[7 * stride] = from_d<C>(cscale(fac, bz));
     }
     __device__ static int nprod(int part) { return part == 0 ? 3 : 2; }
     __device__ static int out_line(int part, int c) { return part * 3 + c; }
     __device__ static int acc_slot(int part, int) { return part; }
     // MHD_BQlin.cpp:153-201 (products) and :219-240 (EMF)
-    __device__ static void products(int part, const double2 *X, int ld, const double *mf, double kx, double ky,
-                                    double2 *prod, double *racc)
+    template <class C>
+    __device__ static void products(int part, const C *X, int ld, const RealOf<C> *mf, RealOf<C> kx, RealOf<C> ky,
+                                    C *prod, double *racc)
     {
-        const double2 ux = X[0], ze = X[ld], bx = X[2 * ld], et = X[3 * ld];
-        const double2 uy = X[4 * ld], uz = X[5 * ld], by = X[6 * ld], bz = X[7 * ld], dzuz = X[8 * ld];
-        const double By = mf[0], dzBy = mf[1], dzdzBy = mf[2];
+        typedef RealOf<C> R;
+        const C ux = X[0], ze = X[ld], bx = X[2 * ld], et = X[3 * ld];
+        const C uy = X[4 * ld], uz = X[5 * ld], by = X[6 * ld], bz = X[7 * ld], dzuz = X[8 * ld];
+        const R By = mf[0], dzBy = mf[1], dzdzBy = mf[2];
         if (part == 0) {
             prod[0] = cscale(By, imul(ky, bx));                 // u, first part
-            prod[1] = cscale(dzBy, cscale(2.0 * kx * ky, bz));  // u, part under 1/lapF
+            prod[1] = cscale(dzBy, cscale((R)2 * kx * ky, bz));  // u, part under 1/lapF
             prod[2] = cscale(By, imul(ky, ux));                 // b
-            racc[0] = redot(bz, ux) - redot(uz, bx);
+            racc[0] = redot64(bz, ux) - redot64(uz, bx);
         } else {
             // zeta: By (i ky eta) + dzBy (-i kx bx) + dzdzBy bz
             prod[0] = cadd(cadd(cscale(By, imul(ky, et)), cscale(dzBy, imul(-kx, bx))), cscale(dzdzBy, bz));
             // eta: By (i ky zeta) + dzBy (-2 dz uz - i kx ux) + dzdzBy (-uz)
-            const double2 t2 = csub(cscale(-2.0, dzuz), imul(kx, ux));
+            const C t2 = csub(cscale((R)-2, dzuz), imul(kx, ux));
             prod[1] = cadd(cadd(cscale(By, imul(ky, ze)), cscale(dzBy, t2)), cscale(-dzdzBy, uz));
-            prod[2] = make_double2(0.0, 0.0);
-            racc[0] = redot(bz, uy) - redot(uz, by);
+            prod[2] = Cx<C>::make(0, 0);
+            racc[0] = redot64(bz, uy) - redot64(uz, by);
         }
-        racc[1] = 0.0;
+        racc[1] = 0;
     }
     // MHD_BQlin.cpp:160-162, :177-178 (pair 0), :185, :200-201 (pair 1)
-    __device__ static void rhs_pair(int pr, const double2 *T, int ld, double2 ux, double2 ze, double2 bx, double2,
+    template <class C>
+    __device__ static void rhs_pair(int pr, const C *T, int ld, double2 ux, double2 ze, double2 bx, double2,
                                     double kx, double ky, double kz, double ilapF, const ModeArgs &a, double2 &N0,
                                     double2 &N1)
     {
         const double q = a.q, Om = a.omega;
         if (pr == 0) {
             const double2 s = cadd(cscale(2.0 * kx * ky * q, ux), imul(2.0 * Om * kz, ze));
-            N0 = cadd(T[0], cscale(ilapF, cadd(s, T[ld])));
-            N1 = cadd(T[3 * ld], imul((q - 2.0 * Om) * kz, ux));
+            N0 = cadd(to_d(T[0]), cscale(ilapF, cadd(s, to_d(T[ld]))));
+            N1 = cadd(to_d(T[3 * ld]), imul((q - 2.0 * Om) * kz, ux));
         } else {
-            N0 = T[2 * ld];
-            N1 = csub(T[4 * ld], imul(q * kz, bx));
+            N0 = to_d(T[2 * ld]);
+            N1 = csub(to_d(T[4 * ld]), imul(q * kz, bx));
         }
     }
     // nu (-k^2) for u, zeta; (eta/nu) times that for b, eta (:248-252, :289-295)
@@ -210,24 +237,25 @@ struct ModelBQ {
 
 // inputs of the first inverse-transform pass: padded line `line` at grid
 // index g, from the base table (zero in the dealiased band)
-template <int NZ, class P>
+template <int NZ, class P, class C>
 struct LineSrc {
-    const double2 *sbase;
-    const double *skz;
-    __device__ double2 operator()(int line, int g) const
+    const C *sbase;
+    const RealOf<C> *skz;
+    __device__ C operator()(int line, int g) const
     {
         using D = Dealias<NZ>;
-        if (!D::kept(g)) return make_double2(0.0, 0.0);
+        if (!D::kept(g)) return Cx<C>::make(0, 0);
         const int m = D::slot(g);
         const int mb = line / P::NLINE, l = line - mb * P::NLINE;
-        const double2 v = sbase[(mb * P::NB + P::base_of(l)) * D::nz + m];
+        const C v = sbase[(mb * P::NB + P::base_of(l)) * D::nz + m];
         return P::dz_of(l) ? imul(skz[m], v) : v;
     }
 };
 
-template <int NZ, int MB, class P>
-__global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode(ModeArgs a)
+template <int NZ, int MB, class P, class C>
+__device__ __forceinline__ void mode_body(const ModeArgs &a)
 {
+    typedef RealOf<C> R;
     using D = Dealias<NZ>;
     constexpr int NZD = D::nz;
     constexpr int NL = P::NLINE * MB;
@@ -237,31 +265,36 @@ __global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode(ModeArgs a)
     constexpr int UW = (NU + NT - 1) / NT;
     constexpr int AI = MB * 2 * NZD;  // update items: (mode slot, field pair, kz slot)
     constexpr int AW = (AI + NT - 1) / NT;
-    __shared__ double2 buf[NL * NZ];
+    __shared__ C buf[NL * NZ];
     __shared__ double2 uin[NU];
-    __shared__ double2 sbase[MB * P::NB * NZD];
-    __shared__ double2 tw[NZ];
+    __shared__ C sbase[MB * P::NB * NZD];
+    __shared__ C tw[NZ];
     __shared__ double skz[NZD];
+    __shared__ R skzr[NZD];
     __shared__ double silF[MB * NZD];
     __shared__ double skx0[MB], sky[MB], skx[MB];
     __shared__ int sidx[MB];
     // mean-field profiles as [z][profile]: in LDS up to NZ = 128, in
     // registers (the thread's fixed z points) at NZ = 256 where LDS is short
-    constexpr bool MF_LDS = (NZ <= 128);
-    __shared__ double smfz[MF_LDS ? NZ * P::NMFZ : 1];
-    double mfr[MF_LDS ? 1 : PW][P::NMFZ];
+    constexpr bool MF_LDS = (NZ <= 128) || sizeof(C) == 8;
+    __shared__ R smfz[MF_LDS ? NZ * P::NMFZ : 1];
+    R mfr[MF_LDS ? 1 : PW][P::NMFZ];
     const int tid = threadIdx.x;
 
-    for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
-    for (int k = tid; k < NZD; k += NT) skz[k] = a.kzv[k];
+    const C *twg = reinterpret_cast<const C *>(sizeof(C) == 8 ? a.tw32 : (const void *)a.tw);
+    for (int k = tid; k < NZ; k += NT) tw[k] = twg[k];
+    for (int k = tid; k < NZD; k += NT) {
+        skz[k] = a.kzv[k];
+        skzr[k] = (R)a.kzv[k];
+    }
     if constexpr (MF_LDS) {
-        for (int k = tid; k < NZ * P::NMFZ; k += NT) smfz[(k % NZ) * P::NMFZ + k / NZ] = a.mfz[k];
+        for (int k = tid; k < NZ * P::NMFZ; k += NT) smfz[(k % NZ) * P::NMFZ + k / NZ] = (R)a.mfz[k];
     } else {
 #pragma unroll
         for (int p = 0; p < PW; ++p) {
             const int item = tid + p * NT;
 #pragma unroll
-            for (int c = 0; c < P::NMFZ; ++c) mfr[p][c] = (item < PI_) ? a.mfz[c * NZ + item % NZ] : 0.0;
+            for (int c = 0; c < P::NMFZ; ++c) mfr[p][c] = (item < PI_) ? (R)a.mfz[c * NZ + item % NZ] : (R)0;
         }
     }
 
@@ -346,10 +379,10 @@ __global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode(ModeArgs a)
         __syncthreads();
 
         // C: inverse transforms
-        fft_lines_from<NZ, NL, true>(buf, tw, LineSrc<NZ, P>{sbase, skz});
+        fft_lines_from<NZ, NL, true>(buf, tw, LineSrc<NZ, P, C>{sbase, skzr});
 
         // D: products on the z grid and EMF / stress sums
-        double2 prod[PW][3];
+        C prod[PW][3];
 #pragma unroll
         for (int p = 0; p < PW; ++p) {
             const int item = tid + p * NT;
@@ -358,10 +391,10 @@ __global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode(ModeArgs a)
             const int part = rem / NZ, z = rem % NZ;
             const double ky = sky[mb];
             double racc[2];
-            const double *mfp;
+            const R *mfp;
             if constexpr (MF_LDS) mfp = smfz + z * P::NMFZ;
             else mfp = mfr[p];
-            P::products(part, buf + mb * P::NLINE * NZ + z, NZ, mfp, skx[mb], ky, prod[p], racc);
+            P::products(part, buf + mb * P::NLINE * NZ + z, NZ, mfp, (R)skx[mb], (R)ky, prod[p], racc);
             if (sidx[mb] < a.nxy) {
                 const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * a.rsf;  // MHD_FullQlin.cpp:314-318
                 acc[p][0] += ftfac * racc[0];
@@ -375,7 +408,7 @@ __global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode(ModeArgs a)
             if (PI_ % NT != 0 && item >= PI_) continue;
             const int mb = item / (2 * NZ), rem = item % (2 * NZ);
             const int part = rem / NZ, z = rem % NZ;
-            double2 *Y = buf + mb * P::NOUT * NZ + z;
+            C *Y = buf + mb * P::NOUT * NZ + z;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 if (c < P::nprod(part)) Y[P::out_line(part, c) * NZ] = prod[p][c];
@@ -486,6 +519,19 @@ __global__ __launch_bounds__(NT) QL_MODE_ATTR void k_mode(ModeArgs a)
         for (int mb = 0; mb < MB; ++mb) s += red[mb * P::NACC * NZ + k];
         a.partials[(size_t)blockIdx.x * P::NACC * NZ + k] = s;
     }
+}
+
+// FP64 path and mixed path (FP32 transforms); each with its own occupancy
+// request (waves per SIMD), which bounds its VGPR budget
+template <int NZ, int MB, class P>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QL_WAVES64))) void k_mode64(ModeArgs a)
+{
+    mode_body<NZ, MB, P, double2>(a);
+}
+template <int NZ, int MB, class P>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QL_WAVES32))) void k_mode32(ModeArgs a)
+{
+    mode_body<NZ, MB, P, float2>(a);
 }
 
 }  // namespace qlk
