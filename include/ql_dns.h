@@ -2,8 +2,10 @@
  * ql_dns.h -- C ABI of the MI355X (gfx950) quasilinear shearing-box time step.
  *
  * One context (ql_ctx) holds one run: its parameters, its device-resident
- * state and its HIP stream.  Host buffers passed in are copied and never
- * retained.  Every entry point returns QL_OK (0) or a negative QL_ERR_* code;
+ * state and its HIP stream.  A run set (ql_batch) holds many runs of one model,
+ * integrator, precision and dt, stepped together; a context is a set of one
+ * (ql_batch_of).  Host buffers passed in are copied and never retained.
+ * Every entry point returns QL_OK (0) or a negative QL_ERR_* code;
  * ql_last_error() gives the message of the last failure on the calling
  * thread.  Calls on one context are not thread-safe (as the reference's
  * objects are not); separate contexts are independent.
@@ -111,11 +113,51 @@ int ql_synchronize(ql_ctx *ctx);
 
 /* Runs nsteps like ql_steps with HIP events around every kernel launch and
  * returns, per kernel kind (0 fused per-mode kernel, 1 stress reduction +
- * mean-field kernel), the average duration in ms and the count.  For measurement only: the events serialise the launches. */
+ * mean-field kernel), the average duration in ms and the count.  For
+ * measurement only: the events serialise the launches. */
 int ql_profile_kernels(ql_ctx *ctx, int nsteps, double *t_inout, double avg_ms[2], int counts[2]);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *ql_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Run sets.  The reference runs one simulation per MPI job (main.cpp:20-160);
+ * a parameter study is many such jobs.  A run set steps many runs of one
+ * model / integrator / precision / dt (each with its own NX, NY, NZ, box, q,
+ * omega, nu, eta, B0z, viscosity order, quasilinear and remap flags; runs of
+ * one NZ share Lz) in the same kernel launches: runs are grouped by NZ, and a
+ * whole step of every group is one replayed HIP graph.
+ * ------------------------------------------------------------------------ */
+typedef struct ql_batch ql_batch;
+
+int ql_batch_create(const ql_params *runs, int nruns, ql_batch **out);
+int ql_batch_destroy(ql_batch *b);
+/* the run set of a context (owned by the context) */
+ql_batch *ql_batch_of(ql_ctx *ctx);
+int ql_batch_size(const ql_batch *b, int *nruns);
+/* per-run counterparts of ql_dims / ql_set_state / ql_get_state /
+ * ql_diagnostics / ql_get_drive, same layouts, run in [0, nruns) */
+int ql_batch_dims(const ql_batch *b, int run, int dims[6]);
+int ql_batch_set_state(ql_batch *b, int run, const double *lin, const double *mf, const double *kx0);
+int ql_batch_get_state(ql_batch *b, int run, double *lin, double *mf, double *kx0);
+int ql_batch_diagnostics(ql_batch *b, int run, double t, double out[18]);
+int ql_batch_get_drive(ql_batch *b, int run, double *drive);
+/* integrator priming at t0 of every run (RK3CN.cpp:57, RK3CN_Elin.cpp:257) */
+int ql_batch_linear_ops_init(ql_batch *b, double t0);
+/* nsteps of main.cpp:108-123 for every run (as ql_steps), one clock */
+int ql_batch_steps(ql_batch *b, int nsteps, double *t_inout);
+/* Launch-shape tuner: for each NZ group, times every candidate shape (z
+ * points per workgroup pass 128 or 256, target workgroup count) over
+ * trial_steps steps and keeps the fastest; the state is restored afterwards.
+ * *step_ms (may be NULL) receives the sum over groups of the best step times.
+ * The shape sets the summation order of the stress reduction: results of two
+ * shapes agree to rounding, results of one shape are bitwise reproducible. */
+int ql_batch_tune(ql_batch *b, int trial_steps, double *step_ms);
+/* current shape per group: cfg[5k..5k+4] = NZ, line, target, modes per
+ * workgroup pass, workgroups; *ngroups receives the group count */
+int ql_batch_get_config(const ql_batch *b, int max_groups, int *ngroups, int *cfg);
+/* sets the shape of the NZ group: line in {128, 256}, target >= 1 */
+int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
 #ifdef __cplusplus
 }

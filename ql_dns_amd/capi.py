@@ -1,6 +1,7 @@
 """ctypes binding of the C ABI in include/ql_dns.h (libql_dns.so).
 
-`QLContext` is a thin object over one ``ql_ctx``: every method is one C-ABI
+`QLContext` is a thin object over one ``ql_ctx`` and `QLBatch` over one
+``ql_batch`` (a set of runs stepped together): every method is one C-ABI
 call.  Loading fails loudly when the library is missing; there is no fallback.
 """
 from __future__ import annotations
@@ -18,7 +19,11 @@ LIB_PATH = os.path.join(HERE, "libql_dns.so")
 # every symbol include/ql_dns.h declares
 EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_linear_ops_init", "ql_rhs", "ql_step", "ql_steps", "ql_remap", "ql_diagnostics",
-           "ql_get_drive", "ql_mf_real", "ql_synchronize", "ql_profile_kernels", "ql_last_error")
+           "ql_get_drive", "ql_mf_real", "ql_synchronize", "ql_profile_kernels", "ql_last_error",
+           "ql_batch_create", "ql_batch_destroy", "ql_batch_of", "ql_batch_size", "ql_batch_dims",
+           "ql_batch_set_state", "ql_batch_get_state", "ql_batch_diagnostics", "ql_batch_get_drive",
+           "ql_batch_linear_ops_init", "ql_batch_steps", "ql_batch_tune", "ql_batch_get_config",
+           "ql_batch_set_config")
 
 
 class QLError(RuntimeError):
@@ -66,6 +71,20 @@ def load(path: str = LIB_PATH):
         "ql_synchronize": ([P], ctypes.c_int),
         "ql_profile_kernels": ([P, ctypes.c_int, D, D, I], ctypes.c_int),
         "ql_last_error": ([], ctypes.c_char_p),
+        "ql_batch_create": ([ctypes.POINTER(ql_params), ctypes.c_int, ctypes.POINTER(P)], ctypes.c_int),
+        "ql_batch_destroy": ([P], ctypes.c_int),
+        "ql_batch_of": ([P], P),
+        "ql_batch_size": ([P, I], ctypes.c_int),
+        "ql_batch_dims": ([P, ctypes.c_int, I], ctypes.c_int),
+        "ql_batch_set_state": ([P, ctypes.c_int, D, D, D], ctypes.c_int),
+        "ql_batch_get_state": ([P, ctypes.c_int, D, D, D], ctypes.c_int),
+        "ql_batch_diagnostics": ([P, ctypes.c_int, ctypes.c_double, D], ctypes.c_int),
+        "ql_batch_get_drive": ([P, ctypes.c_int, D], ctypes.c_int),
+        "ql_batch_linear_ops_init": ([P, ctypes.c_double], ctypes.c_int),
+        "ql_batch_steps": ([P, ctypes.c_int, D], ctypes.c_int),
+        "ql_batch_tune": ([P, ctypes.c_int, D], ctypes.c_int),
+        "ql_batch_get_config": ([P, ctypes.c_int, I, I], ctypes.c_int),
+        "ql_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -77,6 +96,35 @@ def load(path: str = LIB_PATH):
 
 def _dp(a):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _checked(a, dt, n):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=dt)
+    if a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _raise(L, rc: int):
+    if rc != 0:
+        raise QLError(rc, L.ql_last_error().decode())
+
+
+def _get_config(L, h):
+    n = ctypes.c_int(0)
+    _raise(L, L.ql_batch_get_config(h, 0, ctypes.byref(n), None))
+    cfg = (ctypes.c_int * (5 * max(1, n.value)))()
+    _raise(L, L.ql_batch_get_config(h, n.value, ctypes.byref(n), cfg))
+    keys = ("NZ", "line", "target", "modes_per_pass", "workgroups")
+    return [dict(zip(keys, cfg[5 * k:5 * k + 5])) for k in range(n.value)]
+
+
+def _tune(L, h, trial_steps: int) -> float:
+    ms = ctypes.c_double(0.0)
+    _raise(L, L.ql_batch_tune(h, trial_steps, ctypes.byref(ms)))
+    return ms.value
 
 
 def to_c_params(p: QLParams, device: int = 0) -> ql_params:
@@ -120,16 +168,9 @@ class QLContext:
         self.close()
 
     def set_state(self, lin=None, mf=None, kx0=None):
-        def c(a, dt, n):
-            if a is None:
-                return None
-            a = np.ascontiguousarray(a, dtype=dt)
-            if a.size != n:
-                raise ValueError(f"expected {n} values, got {a.size}")
-            return a
-        lin = c(lin, np.complex128, self.nxy * 4 * self.nz)
-        mf = c(mf, np.complex128, self.nMF * self.nz)
-        kx0 = c(kx0, np.float64, self.nxy)
+        lin = _checked(lin, np.complex128, self.nxy * 4 * self.nz)
+        mf = _checked(mf, np.complex128, self.nMF * self.nz)
+        kx0 = _checked(kx0, np.float64, self.nxy)
         self._check(self._L.ql_set_state(self._h, _dp(lin), _dp(mf), _dp(kx0)))
 
     def get_state(self):
@@ -189,3 +230,97 @@ class QLContext:
         cnt = (ctypes.c_int * 2)()
         self._check(self._L.ql_profile_kernels(self._h, n, ctypes.byref(tt), _dp(avg), cnt))
         return tt.value, avg, list(cnt)
+
+    # launch shape (the context's run set of one)
+    def tune(self, trial_steps: int = 5) -> float:
+        return _tune(self._L, self._L.ql_batch_of(self._h), trial_steps)
+
+    def config(self):
+        return _get_config(self._L, self._L.ql_batch_of(self._h))
+
+    def set_config(self, NZ: int, line: int, target: int):
+        self._check(self._L.ql_batch_set_config(self._L.ql_batch_of(self._h), NZ, line, target))
+
+
+class QLBatch:
+    """A set of runs (one model, integrator, precision and dt) stepped together
+    on one GPU.  Run r has the sizes of ``params[r]``."""
+
+    def __init__(self, params, device: int = 0):
+        self._L = load()
+        self.params = list(params)
+        n = len(self.params)
+        arr = (ql_params * n)(*[to_c_params(p, device) for p in self.params])
+        self._h = ctypes.c_void_p()
+        self._check(self._L.ql_batch_create(arr, n, ctypes.byref(self._h)))
+        self.dims = []
+        for r in range(n):
+            d = (ctypes.c_int * 6)()
+            self._check(self._L.ql_batch_dims(self._h, r, d))
+            self.dims.append(dict(zip(("nxy", "nz", "nMF", "nLin", "half", "NZ"), list(d))))
+
+    def __len__(self):
+        return len(self.params)
+
+    def _check(self, rc: int):
+        _raise(self._L, rc)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.ql_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_state(self, run: int, lin=None, mf=None, kx0=None):
+        d = self.dims[run]
+        lin = _checked(lin, np.complex128, d["nxy"] * 4 * d["nz"])
+        mf = _checked(mf, np.complex128, d["nMF"] * d["nz"])
+        kx0 = _checked(kx0, np.float64, d["nxy"])
+        self._check(self._L.ql_batch_set_state(self._h, run, _dp(lin), _dp(mf), _dp(kx0)))
+
+    def get_state(self, run: int):
+        d = self.dims[run]
+        lin = np.zeros((d["nxy"], 4, d["nz"]), dtype=np.complex128)
+        mf = np.zeros((d["nMF"], d["nz"]), dtype=np.complex128)
+        kx0 = np.zeros(d["nxy"])
+        self._check(self._L.ql_batch_get_state(self._h, run, _dp(lin), _dp(mf), _dp(kx0)))
+        return lin, mf, kx0
+
+    def linear_ops_init(self, t0: float = 0.0):
+        self._check(self._L.ql_batch_linear_ops_init(self._h, t0))
+
+    def steps(self, n: int, t: float = 0.0) -> float:
+        tt = ctypes.c_double(t)
+        self._check(self._L.ql_batch_steps(self._h, n, ctypes.byref(tt)))
+        return tt.value
+
+    def diagnostics(self, run: int, t: float):
+        out = np.zeros(18)
+        self._check(self._L.ql_batch_diagnostics(self._h, run, t, _dp(out)))
+        return out
+
+    def drive(self, run: int):
+        d = self.dims[run]
+        out = np.zeros((d["nMF"], d["nz"]), dtype=np.complex128)
+        self._check(self._L.ql_batch_get_drive(self._h, run, _dp(out)))
+        return out
+
+    def tune(self, trial_steps: int = 5) -> float:
+        return _tune(self._L, self._h, trial_steps)
+
+    def config(self):
+        return _get_config(self._L, self._h)
+
+    def set_config(self, NZ: int, line: int, target: int):
+        self._check(self._L.ql_batch_set_config(self._h, NZ, line, target))

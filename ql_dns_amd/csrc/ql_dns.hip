@@ -1,13 +1,22 @@
-// ql_dns.hip -- host side of the MI355X quasilinear time step: the run context,
-// the per-substep launch sequence and the C ABI declared in include/ql_dns.h.
+// ql_dns.hip -- host side of the MI355X quasilinear time step: run sets,
+// the per-substep launch sequence, HIP-graph replay of whole steps, the
+// launch-shape tuner and the C ABI declared in include/ql_dns.h.
 //
 // Reference counterparts: the Model objects (Models/MHD_FullQlin.cpp,
 // Models/MHD_BQlin.cpp) become kernels plus device-resident tables; the
 // Integrator objects (Integrators/RK3CN.cpp, Integrators/RK3CN_Elin.cpp) become
-// the substep sequence in run_substep(); the MPI Allreduce of the stresses
-// (MHD_FullQlin.cpp:386) becomes the on-device fixed-order reduction.
+// the substep sequence in enqueue_substep(); the MPI Allreduce of the stresses
+// (MHD_FullQlin.cpp:386) becomes the on-device fixed-order reduction; the
+// driver loop (main.cpp:108-123) becomes a replayed HIP graph.
+//
+// Every handle holds a set of runs (a context is a set of one).  Runs are
+// grouped by NZ; a group's runs share one kernel instantiation and are laid
+// out one after another in the group's arrays, each with its own parameters,
+// mean fields and reductions.  Groups of different NZ run on their own streams,
+// forked from and joined back into the handle's stream at every substep.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -40,6 +49,12 @@ int fail(int code, const char *fmt, ...)
         if (e_ != hipSuccess) return fail(QL_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+#define QL_RC(call)                 \
+    do {                            \
+        int rc_ = (call);           \
+        if (rc_ != QL_OK) return rc_; \
+    } while (0)
+
 // Integrator coefficients (Integrators/RK3CN.cpp:19-20, RK3CN_Elin.cpp:19-20)
 constexpr double A0 = 8.0 / 15.0, A1 = 5.0 / 12.0, A2 = 3.0 / 4.0;
 constexpr double B1 = -17.0 / 60.0, B2 = -5.0 / 12.0;
@@ -47,235 +62,84 @@ constexpr double P1 = 8.0 / 15.0, P2 = 2.0 / 3.0;
 // General_Definitions.h:236: PI = atan(1)*4, which rounds to this double
 constexpr double QL_PI = 3.14159265358979323846;
 
+// steps captured in the long replay graph
+constexpr int GRAPH_STEPS = 16;
+// launch-shape candidates of the tuner: z points per workgroup pass
+// (modes per workgroup MB = line / NZ) and the target workgroup count
+constexpr int LINES[] = {128, 256};
+constexpr int TARGETS[] = {256, 512, 1024, 2048, 4096, 1 << 30};
+constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
+
+bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
+int mb_for(int NZ, int line) { return NZ >= line ? 1 : line / NZ; }
+
 }  // namespace
 
-struct ql_ctx {
-    ql_params p;
-    int nxy, nz, nzl, half, NZ, nMF, MB, G, vo2;
-    double fac, rsf, kxLfac, kyLfac, kzLfac, omega_eff;
-    double remap_thr, remap_shift, totalN2, kzfac;
-    hipStream_t s = nullptr;
-    // device state and tables
+namespace qlh {
+
+// one run: parameters, sizes and its place in its group
+struct RunInfo {
+    ql_params p{};
+    int group = 0, gi = 0, mode_off = 0;
+    int nxy = 0, nz = 0, nzl = 0, half = 0;
+    double omega_eff = 0, rsf = 0, remap_thr = 0, remap_shift = 0, totalN2 = 0, kzfac = 0;
+    std::vector<double> h_kx0, h_ky, h_kz;
+};
+
+// runs of one NZ
+struct Group {
+    int NZ = 0, nz = 0, nruns = 0, modes = 0;
+    std::vector<int> run_ids;
+    int line = DEFAULT_LINE, target = DEFAULT_TARGET;
+    int MB = 1, W = 0, part_cap = 0, wg_cap = 0;
+    hipStream_t s = nullptr;  // own stream (sets with several groups)
+    hipEvent_t join = nullptr;
     double2 *u = nullptr, *N1 = nullptr, *N2 = nullptr;
     double2 *mf = nullptr, *mfN1 = nullptr, *mfN2 = nullptr, *drive = nullptr, *tw = nullptr;
     float2 *tw32 = nullptr;
-    double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr;
-    double *part = nullptr, *part2 = nullptr, *lin_tmp = nullptr, *dpart = nullptr, *dout = nullptr;
-    double *mfr_tmp = nullptr;
-    // integrator bookkeeping: where the RK3CN "old" operator was evaluated
+    double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr, *part = nullptr, *sums = nullptr;
+    double *lin_tmp = nullptr;
+    qlk::RunParams *rp = nullptr;
+    int *wg_run = nullptr, *wg_g0 = nullptr, *wg_g1 = nullptr;
+    std::vector<qlk::RunParams> h_rp;
+};
+
+struct Batch {
+    int model = 0, integrator = 0, precision = 0, device = 0, nMF = 4;
+    double dt = 0;
+    std::vector<RunInfo> runs;
+    std::vector<Group> groups;
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr;
+    qlk::Clock *clock = nullptr;
+    // host view of the clock: start of the previous step, linearOPs_Init time
+    double t_prev = 0.0, t_init = 0.0;
     int old_init = 1;
-    double qt_old = 0.0, qdtl_old = 0.0;
-    // host copies of the tables
-    std::vector<double> h_kx0, h_ky, h_kz;
-    // optional per-kernel timing
+    // diagnostics scratch
+    double *dpart = nullptr, *dout = nullptr, *mfr_tmp = nullptr;
+    // replay graphs of 1 and GRAPH_STEPS steps (null until first use)
+    hipGraphExec_t g1 = nullptr, gK = nullptr;
+    // per-kernel timing (ql_profile_kernels)
     bool timing = false;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     double t_ms[2] = {0, 0};
     int t_cnt[2] = {0, 0};
 };
 
+}  // namespace qlh
+
+struct ql_batch {
+    qlh::Batch b;
+};
+struct ql_ctx {
+    ql_batch bt;
+};
+
 namespace {
 
-// modes per workgroup of the per-mode kernels: MB * NZ = QL_MB_LINE points
-#ifndef QL_MB_LINE
-#define QL_MB_LINE 256
-#endif
-// upper bound on the per-mode kernel grid
-#ifndef QL_GMAX
-#define QL_GMAX 512
-#endif
-
-bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
-
-constexpr int mb_of(int NZ) { return NZ >= QL_MB_LINE ? 1 : QL_MB_LINE / NZ; }
-int mb_for(int NZ) { return mb_of(NZ); }
-
-int begin_timing(ql_ctx *c)
-{
-    if (c->timing) QL_HIP(hipEventRecord(c->ev[0], c->s));
-    return QL_OK;
-}
-
-int end_timing(ql_ctx *c, int kind)
-{
-    if (!c->timing) return QL_OK;
-    QL_HIP(hipEventRecord(c->ev[1], c->s));
-    QL_HIP(hipEventSynchronize(c->ev[1]));
-    float ms = 0.f;
-    QL_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    c->t_ms[kind] += ms;
-    c->t_cnt[kind] += 1;
-    return QL_OK;
-}
-
-template <int NZ>
-void launch_mode(ql_ctx *c, const qlk::ModeArgs &a)
-{
-    constexpr int MB = mb_of(NZ);
-    const bool full = (c->p.model == QL_MODEL_MHD_FULLQLIN);
-    if (c->p.precision == QL_PREC_MIXED) {
-        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull><<<c->G, qlk::NT, 0, c->s>>>(a);
-        else qlk::k_mode32<NZ, MB, qlk::ModelBQ><<<c->G, qlk::NT, 0, c->s>>>(a);
-    } else {
-        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull><<<c->G, qlk::NT, 0, c->s>>>(a);
-        else qlk::k_mode64<NZ, MB, qlk::ModelBQ><<<c->G, qlk::NT, 0, c->s>>>(a);
-    }
-}
-
-template <int NZ>
-void launch_mf(ql_ctx *c, const qlk::MFArgs &a)
-{
-    qlk::k_meanfield<NZ><<<1, qlk::NT, 0, c->s>>>(a);
-}
-
-template <int NZ>
-void launch_mfreal(ql_ctx *c, double *out)
-{
-    qlk::k_mf_real<NZ><<<1, qlk::NT, 0, c->s>>>(c->mf, c->nMF, c->tw, c->fac, out);
-}
-
-#define QL_DISPATCH_NZ(NZv, FN, ...)               \
-    switch (NZv) {                                 \
-    case 32: FN<32>(__VA_ARGS__); break;           \
-    case 64: FN<64>(__VA_ARGS__); break;           \
-    case 128: FN<128>(__VA_ARGS__); break;         \
-    default: FN<256>(__VA_ARGS__); break;          \
-    }
-
-qlk::ModeArgs mode_args(ql_ctx *c)
-{
-    qlk::ModeArgs a{};
-    a.u = c->u;
-    a.kx0 = c->kx0;
-    a.ky = c->ky;
-    a.kzv = c->kzv;
-    a.tw = c->tw;
-    a.tw32 = c->tw32;
-    a.mfz = c->mfz;
-    a.partials = c->part;
-    a.nxy = c->nxy;
-    a.vo2 = c->vo2;
-    a.q = c->p.q;
-    a.omega = c->omega_eff;
-    a.B0z = c->p.B0z;
-    a.nu = c->p.nu;
-    a.eta = c->p.eta;
-    a.fac = c->fac;
-    a.rsf = c->rsf;
-    a.remap_thr = c->remap_thr;
-    a.remap_shift = c->remap_shift;
-    return a;
-}
-
-qlk::MFArgs mf_args(ql_ctx *c)
-{
-    qlk::MFArgs m{};
-    m.sums = c->part2;
-    m.mf = c->mf;
-    m.drive = c->drive;
-    m.mfz = c->mfz;
-    m.kzv = c->kzv;
-    m.tw = c->tw;
-    m.model = c->p.model;
-    m.nMF = c->nMF;
-    m.quasilinear = c->p.quasilinear;
-    m.q = c->p.q;
-    m.omega = c->omega_eff;
-    m.B0z = c->p.B0z;
-    m.nu = c->p.nu;
-    m.eta = c->p.eta;
-    m.fac = c->fac;
-    return m;
-}
-
-// mode kernel -> column reduction of its partials -> mean-field kernel
-int run_sequence(ql_ctx *c, const qlk::ModeArgs &a, const qlk::MFArgs &m)
-{
-    int rc;
-    if ((rc = begin_timing(c))) return rc;
-    QL_DISPATCH_NZ(c->NZ, launch_mode, c, a);
-    QL_HIP(hipGetLastError());
-    if ((rc = end_timing(c, 0))) return rc;
-    if ((rc = begin_timing(c))) return rc;
-    const int ncols = c->nMF * c->NZ;
-    qlk::k_reduce_cols<<<(ncols + 15) / 16, qlk::NT, 0, c->s>>>(c->part, c->G, ncols, c->part2);
-    QL_HIP(hipGetLastError());
-    QL_DISPATCH_NZ(c->NZ, launch_mf, c, m);
-    QL_HIP(hipGetLastError());
-    return end_timing(c, 1);
-}
-
-int refresh_profiles(ql_ctx *c)
-{
-    qlk::MFArgs m = mf_args(c);
-    m.mode = qlk::MODE_PROFILES;
-    m.mfn_out = c->mfN1;
-    QL_DISPATCH_NZ(c->NZ, launch_mf, c, m);
-    QL_HIP(hipGetLastError());
-    return QL_OK;
-}
-
-// One substep s (1..3) of RK3CN_Elin::Step (RK3CN_Elin.cpp:290-378) or
-// RK3CN::Step (RK3CN.cpp:92-193) at step start t with step dt.
-int run_substep(ql_ctx *c, int s, double t, double dt, bool remap_after, double t_new)
-{
-    const bool cn = (c->p.integrator == QL_INT_RK3CN);
-    double lc, nlc1, nlc2 = 0.0, ts, dtl;
-    if (s == 1) {
-        lc = cn ? dt / 2 * (A0) : dt * (A0);
-        nlc1 = A0 * dt;
-        ts = t;
-        dtl = dt * P1;
-    } else if (s == 2) {
-        lc = cn ? dt / 2 * (A1 + B1) : dt * (A1 + B1);
-        nlc1 = A1 * dt;
-        nlc2 = B1 * dt;
-        ts = t + dt * P1;
-        dtl = dt * (P2 - P1);
-    } else {
-        lc = cn ? dt / 2 * (A2 + B2) : dt * (A2 + B2);
-        nlc1 = A2 * dt;
-        nlc2 = B2 * dt;
-        ts = t + dt * P2;
-        dtl = dt * (1.0 - P2);
-    }
-    qlk::ModeArgs a = mode_args(c);
-    a.mode = cn ? qlk::MODE_CN : qlk::MODE_ELIN;
-    a.qt = c->p.q * ts;
-    a.qdtl = c->p.q * dtl;
-    a.linC = lc;
-    a.nlc1 = nlc1;
-    a.nlc2 = nlc2;
-    a.n_prev = (s == 1) ? nullptr : (s == 2 ? c->N1 : c->N2);
-    a.n_out = (s == 1) ? c->N1 : (s == 2 ? c->N2 : nullptr);
-    a.old_init = c->old_init;
-    a.qt_old = c->qt_old;
-    a.qdtl_old = c->qdtl_old;
-    a.do_remap = remap_after ? 1 : 0;
-    a.remap_qt = c->p.q * t_new;
-
-    qlk::MFArgs m = mf_args(c);
-    m.mode = a.mode;
-    m.linC = lc;
-    m.nlc1 = nlc1;
-    m.nlc2 = nlc2;
-    m.mfn_prev = (s == 1) ? nullptr : (s == 2 ? c->mfN1 : c->mfN2);
-    m.mfn_out = (s == 2) ? c->mfN2 : c->mfN1;
-    int rc = run_sequence(c, a, m);
-    if (rc) return rc;
-    c->old_init = 0;
-    c->qt_old = a.qt;
-    c->qdtl_old = a.qdtl;
-    return QL_OK;
-}
-
-int do_step(ql_ctx *c, double t, bool remap, double t_new)
-{
-    int rc;
-    for (int s = 1; s <= 3; ++s)
-        if ((rc = run_substep(c, s, t, c->p.dt, remap && s == 3, t_new))) return rc;
-    return QL_OK;
-}
+using qlh::Batch;
+using qlh::Group;
+using qlh::RunInfo;
 
 template <typename T>
 int dalloc(T **p, size_t n)
@@ -285,215 +149,598 @@ int dalloc(T **p, size_t n)
     return QL_OK;
 }
 
-void release(ql_ctx *c)
+template <typename T>
+void dfree(T *&p)
 {
-    void *bufs[] = {c->u, c->N1, c->N2, c->mf, c->mfN1, c->mfN2, c->drive, c->tw, c->tw32, c->kx0, c->ky,
-                    c->kzv, c->mfz, c->part, c->part2, c->lin_tmp, c->dpart, c->dout, c->mfr_tmp};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (c->s) (void)hipStreamDestroy(c->s);
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char *ql_last_error(void) { return g_err.c_str(); }
-
-int ql_create(const ql_params *p, ql_ctx **out)
+void drop_graphs(Batch &b)
 {
-    if (!p || !out) return fail(QL_ERR_ARG, "ql_create: null argument");
-    *out = nullptr;
+    if (b.g1) (void)hipGraphExecDestroy(b.g1);
+    if (b.gK) (void)hipGraphExecDestroy(b.gK);
+    b.g1 = b.gK = nullptr;
+}
+
+void release(Batch &b)
+{
+    drop_graphs(b);
+    for (Group &g : b.groups) {
+        dfree(g.u), dfree(g.N1), dfree(g.N2), dfree(g.mf), dfree(g.mfN1), dfree(g.mfN2), dfree(g.drive);
+        dfree(g.tw), dfree(g.tw32), dfree(g.kx0), dfree(g.ky), dfree(g.kzv), dfree(g.mfz), dfree(g.part);
+        dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg_run), dfree(g.wg_g0), dfree(g.wg_g1);
+        if (g.join) (void)hipEventDestroy(g.join);
+        if (g.s) (void)hipStreamDestroy(g.s);
+        g.join = nullptr;
+        g.s = nullptr;
+    }
+    dfree(b.clock), dfree(b.dpart), dfree(b.dout), dfree(b.mfr_tmp);
+    for (auto &e : b.ev)
+        if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (b.fork) (void)hipEventDestroy(b.fork);
+    if (b.s) (void)hipStreamDestroy(b.s);
+    b.fork = nullptr;
+    b.s = nullptr;
+}
+
+// waits for everything queued on the handle's streams
+int sync_all(Batch &b)
+{
+    if (b.s) QL_HIP(hipStreamSynchronize(b.s));
+    for (Group &g : b.groups)
+        if (g.s) QL_HIP(hipStreamSynchronize(g.s));
+    return QL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// validation and tables
+// ---------------------------------------------------------------------------
+int check_params(const ql_params *p, const char *who)
+{
     if (p->model != QL_MODEL_MHD_FULLQLIN && p->model != QL_MODEL_MHD_BQLIN)
-        return fail(QL_ERR_ARG, "ql_create: unknown model %d", p->model);
+        return fail(QL_ERR_ARG, "%s: unknown model %d", who, p->model);
     if (p->integrator != QL_INT_RK3CN && p->integrator != QL_INT_RK3CN_ELIN)
-        return fail(QL_ERR_ARG, "ql_create: unknown integrator %d", p->integrator);
+        return fail(QL_ERR_ARG, "%s: unknown integrator %d", who, p->integrator);
     if (p->precision != QL_PREC_FP64 && p->precision != QL_PREC_MIXED)
-        return fail(QL_ERR_UNSUPPORTED, "ql_create: precision %d is not available in this build", p->precision);
-    if (!valid_NZ(p->NZ)) return fail(QL_ERR_ARG, "ql_create: NZ=%d not in {32,64,128,256}", p->NZ);
+        return fail(QL_ERR_UNSUPPORTED, "%s: precision %d is not available in this build", who, p->precision);
+    if (!valid_NZ(p->NZ)) return fail(QL_ERR_ARG, "%s: NZ=%d not in {32,64,128,256}", who, p->NZ);
     if (p->NX < 2 || p->NY < 2 || (p->NX & 1) || (p->NY & 1))
-        return fail(QL_ERR_ARG, "ql_create: NX=%d, NY=%d must be even and >= 2", p->NX, p->NY);
-    if (!(p->L[0] > 0) || !(p->L[1] > 0) || !(p->L[2] > 0)) return fail(QL_ERR_ARG, "ql_create: box sizes must be > 0");
+        return fail(QL_ERR_ARG, "%s: NX=%d, NY=%d must be even and >= 2", who, p->NX, p->NY);
+    if (p->NX > 4096 || p->NY > 4096) return fail(QL_ERR_ARG, "%s: NX=%d, NY=%d above 4096", who, p->NX, p->NY);
+    if (!(p->L[0] > 0) || !(p->L[1] > 0) || !(p->L[2] > 0)) return fail(QL_ERR_ARG, "%s: box sizes must be > 0", who);
     if (!(p->dt > 0))
-        return fail(QL_ERR_UNSUPPORTED, "ql_create: dt=%g; variable (CFL) time steps are not supported", p->dt);
-    if (p->viscosity_order < 0) return fail(QL_ERR_ARG, "ql_create: viscosity_order < 0");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(QL_ERR_HIP, "ql_create: no HIP device");
-    if (p->device < 0 || p->device >= ndev) return fail(QL_ERR_ARG, "ql_create: device %d of %d", p->device, ndev);
-    QL_HIP(hipSetDevice(p->device));
+        return fail(QL_ERR_UNSUPPORTED, "%s: dt=%g; variable (CFL) time steps are not supported", who, p->dt);
+    if (p->viscosity_order < 0) return fail(QL_ERR_ARG, "%s: viscosity_order < 0", who);
+    return QL_OK;
+}
 
-    ql_ctx *c = new ql_ctx();
-    c->p = *p;
-    c->NZ = p->NZ;
-    c->nxy = (p->NX - 1) * (p->NY / 2);         // Model.h:32-36
-    c->nz = 1 + 2 * (int)(p->NZ / 3.0);         // Model.h:34
-    c->nzl = (c->nz - 1) / 2;
-    c->half = c->nzl + 1;
-    c->nMF = (p->model == QL_MODEL_MHD_BQLIN) ? 2 : 4;
-    c->vo2 = p->viscosity_order / 2;
-    c->fac = 1.0 / p->NZ;                        // fftwPlans.cpp:30
-    c->rsf = 1.0 / (p->NY * p->NX) / (p->NY * p->NX);  // MHD_FullQlin.cpp:67
-    c->kxLfac = 2 * QL_PI / p->L[0];
-    c->kyLfac = 2 * QL_PI / p->L[1];
-    c->kzLfac = 2 * QL_PI / p->L[2];
-    c->omega_eff = (p->model == QL_MODEL_MHD_BQLIN) ? 2.0 / 3.0 * p->q : p->omega;  // MHD_BQlin.cpp:19
-    {
-        const int nxr = p->NX - 1;                                  // MHD_FullQlin.cpp:466
-        c->remap_thr = nxr / 2.0 * (2 * QL_PI / p->L[0]);
-        c->remap_shift = nxr * (2 * QL_PI / p->L[0]);
-    }
-    {
-        long tn = (long)(p->NX * p->NY * p->NZ);
-        tn = tn * tn;
-        c->totalN2 = (double)tn;
-    }
-    c->kzfac = 2 * QL_PI / p->L[2] * (p->NZ / 3);
-    c->MB = mb_for(p->NZ);
-    const int ngroups = (c->nxy + c->MB - 1) / c->MB;
-    c->G = ngroups < QL_GMAX ? ngroups : QL_GMAX;
-
-    // wavenumber tables, Kdata.cpp:28-80
-    const int nx = p->NX / 2, ny = p->NY / 2;
-    c->h_kx0.assign(c->nxy, 0.0);
-    c->h_ky.assign(c->nxy, 0.0);
+// sizes and wavenumber tables of one run (Model.h:32-36, Kdata.cpp:28-80)
+void fill_run(RunInfo &r, const ql_params &p)
+{
+    r.p = p;
+    r.nxy = (p.NX - 1) * (p.NY / 2);
+    r.nz = 1 + 2 * (int)(p.NZ / 3.0);
+    r.nzl = (r.nz - 1) / 2;
+    r.half = r.nzl + 1;
+    r.rsf = 1.0 / (p.NY * p.NX) / (p.NY * p.NX);  // MHD_FullQlin.cpp:67
+    r.omega_eff = (p.model == QL_MODEL_MHD_BQLIN) ? 2.0 / 3.0 * p.q : p.omega;  // MHD_BQlin.cpp:19
+    const int nxr = p.NX - 1;                                                      // MHD_FullQlin.cpp:466
+    r.remap_thr = nxr / 2.0 * (2 * QL_PI / p.L[0]);
+    r.remap_shift = nxr * (2 * QL_PI / p.L[0]);
+    long tn = (long)p.NX * p.NY * p.NZ;
+    r.totalN2 = (double)(tn * tn);
+    r.kzfac = 2 * QL_PI / p.L[2] * (p.NZ / 3);
+    const double kxL = 2 * QL_PI / p.L[0], kyL = 2 * QL_PI / p.L[1], kzL = 2 * QL_PI / p.L[2];
+    const int nx = p.NX / 2, ny = p.NY / 2;
+    r.h_kx0.assign(r.nxy, 0.0);
+    r.h_ky.assign(r.nxy, 0.0);
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j) {
-            c->h_kx0[j + ny * i] = i * c->kxLfac;
-            c->h_ky[j + ny * i] = j * c->kyLfac;
+            r.h_kx0[j + ny * i] = i * kxL;
+            r.h_ky[j + ny * i] = j * kyL;
         }
     for (int i = nx + 1; i < 2 * nx; ++i)
         for (int j = 0; j < ny; ++j) {
-            c->h_kx0[j + ny * (i - 1)] = (-2 * nx + i) * c->kxLfac;
-            c->h_ky[j + ny * (i - 1)] = j * c->kyLfac;
+            r.h_kx0[j + ny * (i - 1)] = (-2 * nx + i) * kxL;
+            r.h_ky[j + ny * (i - 1)] = j * kyL;
         }
-    c->h_kz.assign(c->nz, 0.0);
-    for (int i = 0; i < c->nzl + 1; ++i) c->h_kz[i] = i * c->kzLfac;
-    for (int i = c->nzl + 1; i < 2 * c->nzl + 1; ++i) c->h_kz[i] = (-2 * c->nzl + i - 1) * c->kzLfac;
-    std::vector<double2> htw(c->NZ);
-    std::vector<float2> htw32(c->NZ);
-    for (int k = 0; k < c->NZ; ++k) {
-        const double ang = 2.0 * QL_PI * (double)k / (double)c->NZ;
-        htw[k] = make_double2(cos(ang), -sin(ang));
-        htw32[k] = make_float2((float)htw[k].x, (float)htw[k].y);
+    r.h_kz.assign(r.nz, 0.0);
+    for (int i = 0; i < r.nzl + 1; ++i) r.h_kz[i] = i * kzL;
+    for (int i = r.nzl + 1; i < 2 * r.nzl + 1; ++i) r.h_kz[i] = (-2 * r.nzl + i - 1) * kzL;
+}
+
+qlk::RunParams run_params(const RunInfo &r)
+{
+    qlk::RunParams rp{};
+    rp.ph.q = r.p.q;
+    rp.ph.omega = r.omega_eff;
+    rp.ph.B0z = r.p.B0z;
+    rp.ph.nu = r.p.nu;
+    rp.ph.eta = r.p.eta;
+    rp.ph.vo2 = r.p.viscosity_order / 2;
+    rp.ph.quasilinear = r.p.quasilinear;
+    rp.rsf = r.rsf;
+    rp.remap_thr = r.remap_thr;
+    rp.remap_shift = r.remap_shift;
+    rp.remap = r.p.remap;
+    rp.nxy = r.nxy;
+    rp.mode_off = r.mode_off;
+    return rp;
+}
+
+// workgroup -> (run, mode-group range) tables for a launch shape: modes per
+// workgroup pass MB = line/NZ; consecutive passes per workgroup chosen so that
+// the group launches about `target` workgroups; each run's passes split into
+// near-equal contiguous ranges
+int configure(Batch &b, Group &g, int line, int target)
+{
+    // the tables are rewritten below: nothing queued may still read them
+    QL_RC(sync_all(b));
+    const int MB = mb_for(g.NZ, line);
+    long total = 0;
+    for (int id : g.run_ids) total += (b.runs[id].nxy + MB - 1) / MB;
+    const long gpw = std::max<long>(1, (total + target - 1) / target);
+    std::vector<int> wr, w0, w1;
+    for (int k = 0; k < g.nruns; ++k) {
+        qlk::RunParams &rp = g.h_rp[k];
+        const int ng = (rp.nxy + MB - 1) / MB;
+        const int nw = (int)((ng + gpw - 1) / gpw);
+        rp.wg_off = (int)wr.size();
+        rp.nwg = nw;
+        for (int j = 0; j < nw; ++j) {
+            wr.push_back(k);
+            w0.push_back((int)((long)j * ng / nw));
+            w1.push_back((int)((long)(j + 1) * ng / nw));
+        }
+    }
+    const int W = (int)wr.size();
+    const int ncols = (b.nMF) * g.NZ;
+    if (W > g.wg_cap) {
+        dfree(g.wg_run), dfree(g.wg_g0), dfree(g.wg_g1), dfree(g.part);
+        QL_RC(dalloc(&g.wg_run, W));
+        QL_RC(dalloc(&g.wg_g0, W));
+        QL_RC(dalloc(&g.wg_g1, W));
+        QL_RC(dalloc(&g.part, (size_t)W * ncols));
+        g.wg_cap = W;
+    }
+    QL_HIP(hipMemcpy(g.wg_run, wr.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+    QL_HIP(hipMemcpy(g.wg_g0, w0.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+    QL_HIP(hipMemcpy(g.wg_g1, w1.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+    QL_HIP(hipMemcpy(g.rp, g.h_rp.data(), sizeof(qlk::RunParams) * g.nruns, hipMemcpyHostToDevice));
+    g.line = line;
+    g.target = target;
+    g.MB = MB;
+    g.W = W;
+    drop_graphs(b);
+    return QL_OK;
+}
+
+int create_batch(Batch &b, const ql_params *ps, int nruns)
+{
+    const ql_params &p0 = ps[0];
+    b.model = p0.model;
+    b.integrator = p0.integrator;
+    b.precision = p0.precision;
+    b.device = p0.device;
+    b.dt = p0.dt;
+    b.nMF = (p0.model == QL_MODEL_MHD_BQLIN) ? 2 : 4;
+    b.runs.resize(nruns);
+    std::vector<int> nzs;
+    for (int r = 0; r < nruns; ++r) {
+        fill_run(b.runs[r], ps[r]);
+        if (std::find(nzs.begin(), nzs.end(), ps[r].NZ) == nzs.end()) nzs.push_back(ps[r].NZ);
+    }
+    std::sort(nzs.begin(), nzs.end());
+    b.groups.resize(nzs.size());
+    for (size_t gi = 0; gi < nzs.size(); ++gi) {
+        Group &g = b.groups[gi];
+        g.NZ = nzs[gi];
+        for (int r = 0; r < nruns; ++r)
+            if (ps[r].NZ == g.NZ) {
+                RunInfo &ri = b.runs[r];
+                ri.group = (int)gi;
+                ri.gi = g.nruns++;
+                ri.mode_off = g.modes;
+                g.modes += ri.nxy;
+                g.nz = ri.nz;
+                g.run_ids.push_back(r);
+                g.h_rp.push_back(run_params(ri));
+            }
     }
 
+    QL_HIP(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
+    QL_HIP(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming));
+    for (auto &e : b.ev) QL_HIP(hipEventCreate(&e));
+    QL_RC(dalloc(&b.clock, 1));
+    QL_RC(dalloc(&b.dpart, (size_t)256 * 6));
+    QL_RC(dalloc(&b.dout, (size_t)18));
+    QL_RC(dalloc(&b.mfr_tmp, (size_t)4 * 256));
+    for (Group &g : b.groups) {
+        if (b.groups.size() > 1) {
+            QL_HIP(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+            QL_HIP(hipEventCreateWithFlags(&g.join, hipEventDisableTiming));
+        }
+        const size_t nl = (size_t)g.modes * 4 * g.nz, nm = (size_t)g.nruns * b.nMF * g.nz;
+        QL_RC(dalloc(&g.u, nl));
+        QL_RC(dalloc(&g.N1, nl));
+        QL_RC(dalloc(&g.N2, nl));
+        QL_RC(dalloc(&g.mf, nm));
+        QL_RC(dalloc(&g.mfN1, nm));
+        QL_RC(dalloc(&g.mfN2, nm));
+        QL_RC(dalloc(&g.drive, nm));
+        QL_RC(dalloc(&g.tw, g.NZ));
+        QL_RC(dalloc(&g.tw32, g.NZ));
+        QL_RC(dalloc(&g.kx0, g.modes));
+        QL_RC(dalloc(&g.ky, g.modes));
+        QL_RC(dalloc(&g.kzv, g.nz));
+        QL_RC(dalloc(&g.mfz, (size_t)g.nruns * 8 * g.NZ));
+        QL_RC(dalloc(&g.sums, (size_t)g.nruns * b.nMF * g.NZ));
+        QL_RC(dalloc(&g.rp, g.nruns));
+        std::vector<double> kx0(g.modes), ky(g.modes);
+        for (int id : g.run_ids) {
+            const RunInfo &ri = b.runs[id];
+            std::copy(ri.h_kx0.begin(), ri.h_kx0.end(), kx0.begin() + ri.mode_off);
+            std::copy(ri.h_ky.begin(), ri.h_ky.end(), ky.begin() + ri.mode_off);
+        }
+        std::vector<double2> htw(g.NZ);
+        std::vector<float2> htw32(g.NZ);
+        for (int k = 0; k < g.NZ; ++k) {
+            const double ang = 2.0 * QL_PI * (double)k / (double)g.NZ;
+            htw[k] = make_double2(cos(ang), -sin(ang));
+            htw32[k] = make_float2((float)htw[k].x, (float)htw[k].y);
+        }
+        const RunInfo &r0 = b.runs[g.run_ids[0]];
+        QL_HIP(hipMemcpy(g.kx0, kx0.data(), sizeof(double) * g.modes, hipMemcpyHostToDevice));
+        QL_HIP(hipMemcpy(g.ky, ky.data(), sizeof(double) * g.modes, hipMemcpyHostToDevice));
+        // kz depends on Lz: runs of a group share it (checked by the caller)
+        QL_HIP(hipMemcpy(g.kzv, r0.h_kz.data(), sizeof(double) * g.nz, hipMemcpyHostToDevice));
+        QL_HIP(hipMemcpy(g.tw, htw.data(), sizeof(double2) * g.NZ, hipMemcpyHostToDevice));
+        QL_HIP(hipMemcpy(g.tw32, htw32.data(), sizeof(float2) * g.NZ, hipMemcpyHostToDevice));
+        QL_RC(configure(b, g, DEFAULT_LINE, DEFAULT_TARGET));
+    }
+    return QL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+template <int NZ, int MB>
+void launch_mode_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    const bool full = (b.model == QL_MODEL_MHD_FULLQLIN);
+    if (b.precision == QL_PREC_MIXED) {
+        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull><<<g.W, qlk::NT, 0, st>>>(a);
+        else qlk::k_mode32<NZ, MB, qlk::ModelBQ><<<g.W, qlk::NT, 0, st>>>(a);
+    } else {
+        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull><<<g.W, qlk::NT, 0, st>>>(a);
+        else qlk::k_mode64<NZ, MB, qlk::ModelBQ><<<g.W, qlk::NT, 0, st>>>(a);
+    }
+}
+
+void launch_mode(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    switch (g.NZ) {
+    case 32: g.MB == 4 ? launch_mode_t<32, 4>(b, g, a, st) : launch_mode_t<32, 8>(b, g, a, st); break;
+    case 64: g.MB == 2 ? launch_mode_t<64, 2>(b, g, a, st) : launch_mode_t<64, 4>(b, g, a, st); break;
+    case 128: g.MB == 1 ? launch_mode_t<128, 1>(b, g, a, st) : launch_mode_t<128, 2>(b, g, a, st); break;
+    default: launch_mode_t<256, 1>(b, g, a, st); break;
+    }
+}
+
+void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st)
+{
+    switch (g.NZ) {
+    case 32: qlk::k_meanfield<32><<<g.nruns, qlk::NT, 0, st>>>(m); break;
+    case 64: qlk::k_meanfield<64><<<g.nruns, qlk::NT, 0, st>>>(m); break;
+    case 128: qlk::k_meanfield<128><<<g.nruns, qlk::NT, 0, st>>>(m); break;
+    default: qlk::k_meanfield<256><<<g.nruns, qlk::NT, 0, st>>>(m); break;
+    }
+}
+
+qlk::ModeArgs mode_args(const Batch &b, const Group &g)
+{
+    qlk::ModeArgs a{};
+    a.u = g.u;
+    a.kx0 = g.kx0;
+    a.ky = g.ky;
+    a.kzv = g.kzv;
+    a.tw = g.tw;
+    a.tw32 = g.tw32;
+    a.mfz = g.mfz;
+    a.partials = g.part;
+    a.runs = g.rp;
+    a.wg_run = g.wg_run;
+    a.wg_g0 = g.wg_g0;
+    a.wg_g1 = g.wg_g1;
+    a.clock = b.clock;
+    const double dt = b.dt;
+    a.dt = dt;
+    a.dtP1 = dt * P1;
+    a.dtP2 = dt * P2;
+    a.dtl1 = dt * P1;
+    a.dtl2 = dt * (P2 - P1);
+    a.dtl3 = dt * (1.0 - P2);
+    a.fac = 1.0 / g.NZ;  // fftwPlans.cpp:30
+    return a;
+}
+
+qlk::MFArgs mf_args(const Batch &b, const Group &g)
+{
+    qlk::MFArgs m{};
+    m.sums = g.sums;
+    m.mf = g.mf;
+    m.drive = g.drive;
+    m.mfz = g.mfz;
+    m.runs = g.rp;
+    m.kzv = g.kzv;
+    m.tw = g.tw;
+    m.model = b.model;
+    m.nMF = b.nMF;
+    m.fac = 1.0 / g.NZ;
+    return m;
+}
+
+// mode kernel -> column reduction of its partials -> mean-field kernel
+int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFArgs &m, hipStream_t st)
+{
+    if (b.timing) QL_HIP(hipEventRecord(b.ev[0], st));
+    launch_mode(b, g, a, st);
+    QL_HIP(hipGetLastError());
+    if (b.timing) QL_HIP(hipEventRecord(b.ev[1], st));
+    const int ncols = b.nMF * g.NZ;
+    qlk::k_reduce_cols<<<dim3((ncols + 15) / 16, g.nruns), qlk::NT, 0, st>>>(g.part, g.rp, ncols, g.sums);
+    QL_HIP(hipGetLastError());
+    launch_mf(g, m, st);
+    QL_HIP(hipGetLastError());
+    if (b.timing) {
+        QL_HIP(hipEventRecord(b.ev[2], st));
+        QL_HIP(hipEventSynchronize(b.ev[2]));
+        float ms0 = 0.f, ms1 = 0.f;
+        QL_HIP(hipEventElapsedTime(&ms0, b.ev[0], b.ev[1]));
+        QL_HIP(hipEventElapsedTime(&ms1, b.ev[1], b.ev[2]));
+        b.t_ms[0] += ms0;
+        b.t_ms[1] += ms1;
+        b.t_cnt[0] += 1;
+        b.t_cnt[1] += 1;
+    }
+    return QL_OK;
+}
+
+int refresh_profiles(Batch &b, Group &g, hipStream_t st)
+{
+    qlk::MFArgs m = mf_args(b, g);
+    m.mode = qlk::MODE_PROFILES;
+    m.mfn_out = g.mfN1;
+    launch_mf(g, m, st);
+    QL_HIP(hipGetLastError());
+    return QL_OK;
+}
+
+// One substep s (1..3) of RK3CN_Elin::Step (RK3CN_Elin.cpp:290-378) or
+// RK3CN::Step (RK3CN.cpp:92-193) for one group; times come from the clock.
+int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, hipStream_t st)
+{
+    const bool cn = (b.integrator == QL_INT_RK3CN);
+    const double dt = b.dt;
+    double lc, nlc1, nlc2 = 0.0;
+    if (s == 1) {
+        lc = cn ? dt / 2 * (A0) : dt * (A0);
+        nlc1 = A0 * dt;
+    } else if (s == 2) {
+        lc = cn ? dt / 2 * (A1 + B1) : dt * (A1 + B1);
+        nlc1 = A1 * dt;
+        nlc2 = B1 * dt;
+    } else {
+        lc = cn ? dt / 2 * (A2 + B2) : dt * (A2 + B2);
+        nlc1 = A2 * dt;
+        nlc2 = B2 * dt;
+    }
+    qlk::ModeArgs a = mode_args(b, g);
+    a.s = s;
+    a.mode = cn ? qlk::MODE_CN : qlk::MODE_ELIN;
+    a.linC = lc;
+    a.nlc1 = nlc1;
+    a.nlc2 = nlc2;
+    a.n_prev = (s == 1) ? nullptr : (s == 2 ? g.N1 : g.N2);
+    a.n_out = (s == 1) ? g.N1 : (s == 2 ? g.N2 : nullptr);
+    a.do_remap = remap_after ? 1 : 0;
+
+    qlk::MFArgs m = mf_args(b, g);
+    m.mode = a.mode;
+    m.linC = lc;
+    m.nlc1 = nlc1;
+    m.nlc2 = nlc2;
+    m.mfn_prev = (s == 1) ? nullptr : (s == 2 ? g.mfN1 : g.mfN2);
+    m.mfn_out = (s == 2) ? g.mfN2 : g.mfN1;
+    return enqueue_sequence(b, g, a, m, st);
+}
+
+// substep s of every group: groups fork from the handle's stream and join it
+int enqueue_substep_all(Batch &b, int s, bool remap_after)
+{
+    if (b.groups.size() == 1) return enqueue_substep(b, b.groups[0], s, remap_after, b.s);
+    QL_HIP(hipEventRecord(b.fork, b.s));
+    for (Group &g : b.groups) {
+        QL_HIP(hipStreamWaitEvent(g.s, b.fork, 0));
+        QL_RC(enqueue_substep(b, g, s, remap_after, g.s));
+        QL_HIP(hipEventRecord(g.join, g.s));
+    }
+    for (Group &g : b.groups) QL_HIP(hipStreamWaitEvent(b.s, g.join, 0));
+    return QL_OK;
+}
+
+// one step of main.cpp:108-123 (noise off): Step, t += dt, remap
+int enqueue_step(Batch &b, bool remap)
+{
+    for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, remap && s == 3));
+    qlk::k_clock_advance<<<1, 64, 0, b.s>>>(b.clock, b.dt);
+    QL_HIP(hipGetLastError());
+    return QL_OK;
+}
+
+int set_clock(Batch &b, double t)
+{
+    qlk::Clock c{};
+    c.t = t;
+    c.t_prev = b.t_prev;
+    c.t_init = b.t_init;
+    c.old_init = b.old_init;
+    qlk::k_clock_set<<<1, 64, 0, b.s>>>(b.clock, c);
+    QL_HIP(hipGetLastError());
+    return QL_OK;
+}
+
+int capture(Batch &b, int nsteps, hipGraphExec_t *out)
+{
+    hipGraph_t graph = nullptr;
+    QL_HIP(hipStreamBeginCapture(b.s, hipStreamCaptureModeThreadLocal));
     int rc = QL_OK;
-    const size_t nl = (size_t)c->nxy * 4 * c->nz;
-    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) rc = fail(QL_ERR_HIP, "stream create failed");
-    if (!rc) rc = dalloc(&c->u, nl);
-    if (!rc) rc = dalloc(&c->N1, nl);
-    if (!rc) rc = dalloc(&c->N2, nl);
-    if (!rc) rc = dalloc(&c->lin_tmp, nl);
-    if (!rc) rc = dalloc(&c->mf, (size_t)c->nMF * c->nz);
-    if (!rc) rc = dalloc(&c->mfN1, (size_t)c->nMF * c->nz);
-    if (!rc) rc = dalloc(&c->mfN2, (size_t)c->nMF * c->nz);
-    if (!rc) rc = dalloc(&c->drive, (size_t)c->nMF * c->nz);
-    if (!rc) rc = dalloc(&c->tw, (size_t)c->NZ);
-    if (!rc) rc = dalloc(&c->tw32, (size_t)c->NZ);
-    if (!rc) rc = dalloc(&c->kx0, (size_t)c->nxy);
-    if (!rc) rc = dalloc(&c->ky, (size_t)c->nxy);
-    if (!rc) rc = dalloc(&c->kzv, (size_t)c->nz);
-    if (!rc) rc = dalloc(&c->mfz, (size_t)8 * c->NZ);
-    if (!rc) rc = dalloc(&c->part, (size_t)c->G * c->nMF * c->NZ);
-    if (!rc) rc = dalloc(&c->part2, (size_t)c->nMF * c->NZ);
-    if (!rc) rc = dalloc(&c->dpart, (size_t)256 * 6);
-    if (!rc) rc = dalloc(&c->dout, (size_t)18);
-    if (!rc) rc = dalloc(&c->mfr_tmp, (size_t)4 * c->NZ);
-    if (!rc && hipMemcpy(c->kx0, c->h_kx0.data(), sizeof(double) * c->nxy, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(QL_ERR_HIP, "table upload failed");
-    if (!rc && hipMemcpy(c->ky, c->h_ky.data(), sizeof(double) * c->nxy, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(QL_ERR_HIP, "table upload failed");
-    if (!rc && hipMemcpy(c->kzv, c->h_kz.data(), sizeof(double) * c->nz, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(QL_ERR_HIP, "table upload failed");
-    if (!rc && hipMemcpy(c->tw, htw.data(), sizeof(double2) * c->NZ, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(QL_ERR_HIP, "table upload failed");
-    if (!rc && hipMemcpy(c->tw32, htw32.data(), sizeof(float2) * c->NZ, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(QL_ERR_HIP, "table upload failed");
-    if (!rc && (hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess))
-        rc = fail(QL_ERR_HIP, "event create failed");
-    if (rc) {
-        release(c);
-        delete c;
+    for (int n = 0; n < nsteps && rc == QL_OK; ++n) rc = enqueue_step(b, true);
+    hipError_t e = hipStreamEndCapture(b.s, &graph);
+    if (rc != QL_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
     }
-    *out = c;
+    if (e != hipSuccess) return fail(QL_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(QL_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     return QL_OK;
 }
 
-int ql_destroy(ql_ctx *c)
+// nsteps of the driver loop with remap as each run asks; graph replay unless
+// per-kernel timing is on
+int run_steps(Batch &b, int nsteps, double *t_inout)
 {
-    if (!c) return QL_OK;
-    if (c->s) (void)hipStreamSynchronize(c->s);
-    release(c);
-    delete c;
+    double t = *t_inout;
+    QL_RC(set_clock(b, t));
+    if (b.timing) {
+        for (int n = 0; n < nsteps; ++n) QL_RC(enqueue_step(b, true));
+    } else {
+        int left = nsteps;
+        if (left >= GRAPH_STEPS) {
+            if (!b.gK) QL_RC(capture(b, GRAPH_STEPS, &b.gK));
+            for (; left >= GRAPH_STEPS; left -= GRAPH_STEPS) QL_HIP(hipGraphLaunch(b.gK, b.s));
+        }
+        if (left > 0) {
+            if (!b.g1) QL_RC(capture(b, 1, &b.g1));
+            for (; left > 0; --left) QL_HIP(hipGraphLaunch(b.g1, b.s));
+        }
+    }
+    // the same accumulation as the device clock (main.cpp:116)
+    for (int n = 0; n < nsteps; ++n) {
+        b.t_prev = t;
+        t = t + b.dt;
+    }
+    if (nsteps > 0) b.old_init = 0;
+    QL_HIP(hipStreamSynchronize(b.s));
+    *t_inout = t;
     return QL_OK;
 }
 
-int ql_dims(const ql_ctx *c, int d[6])
+int check_run(const Batch &b, int run, const char *who)
 {
-    if (!c || !d) return fail(QL_ERR_ARG, "ql_dims: null argument");
-    d[0] = c->nxy;
-    d[1] = c->nz;
-    d[2] = c->nMF;
-    d[3] = 4;
-    d[4] = c->half;
-    d[5] = c->NZ;
+    if (run < 0 || run >= (int)b.runs.size()) return fail(QL_ERR_ARG, "%s: run %d of %d", who, run, (int)b.runs.size());
     return QL_OK;
 }
 
-int ql_set_state(ql_ctx *c, const double *lin, const double *mf, const double *kx0)
+int set_state(Batch &b, int run, const double *lin, const double *mf, const double *kx0)
 {
-    if (!c) return fail(QL_ERR_ARG, "ql_set_state: null context");
+    const RunInfo &r = b.runs[run];
+    Group &g = b.groups[r.group];
+    const size_t mo = (size_t)r.gi * b.nMF * g.nz;
     if (lin)
-        QL_HIP(hipMemcpyAsync(c->u, lin, sizeof(double2) * (size_t)c->nxy * 4 * c->nz, hipMemcpyHostToDevice, c->s));
-    if (mf) QL_HIP(hipMemcpyAsync(c->mf, mf, sizeof(double2) * (size_t)c->nMF * c->nz, hipMemcpyHostToDevice, c->s));
-    if (kx0) QL_HIP(hipMemcpyAsync(c->kx0, kx0, sizeof(double) * (size_t)c->nxy, hipMemcpyHostToDevice, c->s));
-    int rc = refresh_profiles(c);
-    if (rc) return rc;
-    QL_HIP(hipStreamSynchronize(c->s));
+        QL_HIP(hipMemcpyAsync(g.u + (size_t)r.mode_off * 4 * g.nz, lin, sizeof(double2) * (size_t)r.nxy * 4 * g.nz,
+                              hipMemcpyHostToDevice, b.s));
+    if (mf) QL_HIP(hipMemcpyAsync(g.mf + mo, mf, sizeof(double2) * b.nMF * g.nz, hipMemcpyHostToDevice, b.s));
+    if (kx0)
+        QL_HIP(hipMemcpyAsync(g.kx0 + r.mode_off, kx0, sizeof(double) * r.nxy, hipMemcpyHostToDevice, b.s));
+    QL_RC(refresh_profiles(b, g, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
 
-int ql_get_state(ql_ctx *c, double *lin, double *mf, double *kx0)
+int get_state(Batch &b, int run, double *lin, double *mf, double *kx0)
 {
-    if (!c) return fail(QL_ERR_ARG, "ql_get_state: null context");
+    const RunInfo &r = b.runs[run];
+    Group &g = b.groups[r.group];
+    const size_t mo = (size_t)r.gi * b.nMF * g.nz;
     if (lin)
-        QL_HIP(hipMemcpyAsync(lin, c->u, sizeof(double2) * (size_t)c->nxy * 4 * c->nz, hipMemcpyDeviceToHost, c->s));
-    if (mf) QL_HIP(hipMemcpyAsync(mf, c->mf, sizeof(double2) * (size_t)c->nMF * c->nz, hipMemcpyDeviceToHost, c->s));
-    if (kx0) QL_HIP(hipMemcpyAsync(kx0, c->kx0, sizeof(double) * (size_t)c->nxy, hipMemcpyDeviceToHost, c->s));
-    QL_HIP(hipStreamSynchronize(c->s));
+        QL_HIP(hipMemcpyAsync(lin, g.u + (size_t)r.mode_off * 4 * g.nz, sizeof(double2) * (size_t)r.nxy * 4 * g.nz,
+                              hipMemcpyDeviceToHost, b.s));
+    if (mf) QL_HIP(hipMemcpyAsync(mf, g.mf + mo, sizeof(double2) * b.nMF * g.nz, hipMemcpyDeviceToHost, b.s));
+    if (kx0)
+        QL_HIP(hipMemcpyAsync(kx0, g.kx0 + r.mode_off, sizeof(double) * r.nxy, hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
 
-int ql_linear_ops_init(ql_ctx *c, double t0, double *linop, double *linop_mf)
+int diagnostics(Batch &b, int run, double t, double out[18])
 {
-    if (!c) return fail(QL_ERR_ARG, "ql_linear_ops_init: null context");
-    c->old_init = 1;
-    c->qt_old = c->p.q * t0;
-    c->qdtl_old = 0.0;
-    const int nz = c->nz;
+    const RunInfo &r = b.runs[run];
+    Group &g = b.groups[r.group];
+    qlk::DiagArgs d{};
+    d.u = g.u;
+    d.kx0 = g.kx0;
+    d.ky = g.ky;
+    d.kzv = g.kzv;
+    d.m0 = r.mode_off;
+    d.nxy = r.nxy;
+    d.nz = g.nz;
+    d.qt = r.p.q * t;
+    d.nu = r.p.nu;
+    d.eta = r.p.eta;
+    d.part = b.dpart;
+    const int grid = r.nxy < 256 ? r.nxy : 256;
+    qlk::k_diag_modes<<<grid, qlk::NT, 0, b.s>>>(d);
+    QL_HIP(hipGetLastError());
+    const size_t mo = (size_t)r.gi * b.nMF * g.nz;
+    qlk::DiagFinalArgs f{};
+    f.part = b.dpart;
+    f.nparts = grid;
+    f.mf = g.mf + mo;
+    f.drive = g.drive + mo;
+    f.kzv = g.kzv;
+    f.nz = g.nz;
+    f.nMF = b.nMF;
+    f.model = b.model;
+    f.NZ = g.NZ;
+    f.Lz = r.p.L[2];
+    f.totalN2 = r.totalN2;
+    f.nu = r.p.nu;
+    f.eta = r.p.eta;
+    f.kzfac = r.kzfac;
+    f.out = b.dout;
+    qlk::k_diag_final<<<1, 64, 0, b.s>>>(f);
+    QL_HIP(hipGetLastError());
+    QL_HIP(hipMemcpyAsync(out, b.dout, sizeof(double) * 18, hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
+    return QL_OK;
+}
+
+void linear_ops_host(const Batch &b, const RunInfo &r, const std::vector<double> &k0, double qt,
+                     double *linop, double *linop_mf)
+{
+    const int nz = r.nz;
+    const ql_params &p = r.p;
     if (linop) {
         // MHD_FullQlin.cpp:432-440 / MHD_BQlin.cpp:289-295 on the current kx0
-        std::vector<double> k0(c->nxy);
-        QL_HIP(hipMemcpyAsync(k0.data(), c->kx0, sizeof(double) * c->nxy, hipMemcpyDeviceToHost, c->s));
-        QL_HIP(hipStreamSynchronize(c->s));
-        for (int i = 0; i < c->nxy; ++i) {
-            const double ky = c->h_ky[i];
-            const double kx = k0[i] + c->qt_old * ky;
+        for (int i = 0; i < r.nxy; ++i) {
+            const double ky = r.h_ky[i];
+            const double kx = k0[i] + qt * ky;
             for (int m = 0; m < nz; ++m) {
-                const double kz2 = -(c->h_kz[m] * c->h_kz[m]);
+                const double kz2 = -(r.h_kz[m] * r.h_kz[m]);
                 const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
                 const double lapF = -kx * kx + lap2;
                 double L0, L2;
-                if (c->p.model == QL_MODEL_MHD_BQLIN) {
-                    L0 = c->p.nu * lapF;
-                    L2 = (c->p.eta / c->p.nu) * L0;
+                if (p.model == QL_MODEL_MHD_BQLIN) {
+                    L0 = p.nu * lapF;
+                    L2 = (p.eta / p.nu) * L0;
                 } else {
-                    const double lp = -pow(fabs(lapF), (double)c->vo2);
-                    L0 = c->p.nu * lp;
-                    L2 = c->p.eta * lp;
+                    const double lp = -pow(fabs(lapF), (double)(p.viscosity_order / 2));
+                    L0 = p.nu * lp;
+                    L2 = p.eta * lp;
                 }
                 double *o = linop + (size_t)i * 4 * nz;
                 o[m] = L0;
@@ -504,13 +751,192 @@ int ql_linear_ops_init(ql_ctx *c, double t0, double *linop, double *linop_mf)
         }
     }
     if (linop_mf) {
-        for (int f = 0; f < c->nMF; ++f)
+        for (int f = 0; f < b.nMF; ++f)
             for (int m = 0; m < nz; ++m) {
-                const double kz2 = -(c->h_kz[m] * c->h_kz[m]);
+                const double kz2 = -(r.h_kz[m] * r.h_kz[m]);
                 double v = 0.0;
-                if (c->p.quasilinear) v = (c->p.model == QL_MODEL_MHD_BQLIN || f < 2 ? c->p.eta : c->p.nu) * kz2;
+                if (p.quasilinear) v = (p.model == QL_MODEL_MHD_BQLIN || f < 2 ? p.eta : p.nu) * kz2;
                 linop_mf[(size_t)f * nz + m] = v;
             }
+    }
+}
+
+// launch-shape tuner: per group, time each candidate shape over trial_steps
+// steps of that group alone and keep the fastest; the state is restored
+// afterwards.  Shapes change the summation order of the stress reduction, so
+// results agree across shapes to rounding, and bitwise for one shape.
+int tune(Batch &b, int trial_steps, double *best_ms_out)
+{
+    if (trial_steps < 1) return fail(QL_ERR_ARG, "tune: trial_steps < 1");
+    QL_HIP(hipStreamSynchronize(b.s));
+    // snapshot of what the trial steps overwrite
+    struct Snap {
+        double2 *u = nullptr, *mf = nullptr, *drive = nullptr;
+        double *kx0 = nullptr;
+    };
+    std::vector<Snap> snaps(b.groups.size());
+    int rc = QL_OK;
+    for (size_t gi = 0; gi < b.groups.size() && rc == QL_OK; ++gi) {
+        Group &g = b.groups[gi];
+        Snap &sn = snaps[gi];
+        const size_t nl = (size_t)g.modes * 4 * g.nz, nm = (size_t)g.nruns * b.nMF * g.nz;
+        if ((rc = dalloc(&sn.u, nl)) || (rc = dalloc(&sn.mf, nm)) || (rc = dalloc(&sn.drive, nm)) ||
+            (rc = dalloc(&sn.kx0, g.modes)))
+            break;
+        if (hipMemcpyAsync(sn.u, g.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+            hipMemcpyAsync(sn.mf, g.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+            hipMemcpyAsync(sn.drive, g.drive, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+            hipMemcpyAsync(sn.kx0, g.kx0, sizeof(double) * g.modes, hipMemcpyDeviceToDevice, b.s) != hipSuccess)
+            rc = fail(QL_ERR_HIP, "tune: state snapshot failed");
+    }
+    double total_best = 0.0;
+    const bool timing = b.timing;
+    b.timing = false;
+    if (rc == QL_OK) rc = set_clock(b, 0.0);
+    for (size_t gi = 0; gi < b.groups.size() && rc == QL_OK; ++gi) {
+        Group &g = b.groups[gi];
+        double best = 1e300;
+        int best_line = g.line, best_target = g.target;
+        std::vector<std::pair<int, int>> seen;  // (MB, W) already timed
+        for (int line : LINES) {
+            for (int target : TARGETS) {
+                if ((rc = configure(b, g, line, target))) break;
+                bool dup = false;
+                for (auto &sw : seen) dup = dup || (sw.first == g.MB && sw.second == g.W);
+                if (dup) continue;
+                seen.emplace_back(g.MB, g.W);
+                for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, b.s);
+                if (rc) break;
+                if (hipEventRecord(b.ev[0], b.s) != hipSuccess) {
+                    rc = fail(QL_ERR_HIP, "tune: event record failed");
+                    break;
+                }
+                for (int n = 0; n < trial_steps && rc == QL_OK; ++n)
+                    for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, b.s);
+                if (rc) break;
+                float ms = 0.f;
+                if (hipEventRecord(b.ev[1], b.s) != hipSuccess || hipEventSynchronize(b.ev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, b.ev[0], b.ev[1]) != hipSuccess) {
+                    rc = fail(QL_ERR_HIP, "tune: event timing failed");
+                    break;
+                }
+                const double per = ms / trial_steps;
+                if (per < best) {
+                    best = per;
+                    best_line = line;
+                    best_target = target;
+                }
+            }
+            if (rc) break;
+        }
+        if (rc == QL_OK) rc = configure(b, g, best_line, best_target);
+        total_best += best;
+    }
+    b.timing = timing;
+    // restore
+    for (size_t gi = 0; gi < b.groups.size(); ++gi) {
+        Group &g = b.groups[gi];
+        Snap &sn = snaps[gi];
+        const size_t nl = (size_t)g.modes * 4 * g.nz, nm = (size_t)g.nruns * b.nMF * g.nz;
+        if (sn.u && sn.mf && sn.drive && sn.kx0) {
+            if (hipMemcpyAsync(g.u, sn.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+                hipMemcpyAsync(g.mf, sn.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+                hipMemcpyAsync(g.drive, sn.drive, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+                hipMemcpyAsync(g.kx0, sn.kx0, sizeof(double) * g.modes, hipMemcpyDeviceToDevice, b.s) != hipSuccess)
+                if (rc == QL_OK) rc = fail(QL_ERR_HIP, "tune: state restore failed");
+            if (rc == QL_OK) rc = refresh_profiles(b, g, b.s);
+        }
+    }
+    if (hipStreamSynchronize(b.s) != hipSuccess && rc == QL_OK) rc = fail(QL_ERR_HIP, "tune: synchronize failed");
+    for (Snap &sn : snaps) dfree(sn.u), dfree(sn.mf), dfree(sn.drive), dfree(sn.kx0);
+    if (rc == QL_OK && best_ms_out) *best_ms_out = total_best;
+    return rc;
+}
+
+int init_batch(Batch &b, const ql_params *ps, int nruns, const char *who)
+{
+    if (nruns < 1) return fail(QL_ERR_ARG, "%s: nruns=%d", who, nruns);
+    for (int r = 0; r < nruns; ++r) {
+        QL_RC(check_params(&ps[r], who));
+        const ql_params &a = ps[0], &c = ps[r];
+        if (c.model != a.model || c.integrator != a.integrator || c.precision != a.precision || c.device != a.device)
+            return fail(QL_ERR_ARG, "%s: run %d: runs of a set share model, integrator, precision and device", who, r);
+        if (c.dt != a.dt) return fail(QL_ERR_ARG, "%s: run %d: runs of a set share dt", who, r);
+        for (int q = 0; q < r; ++q)
+            if (ps[q].NZ == c.NZ && ps[q].L[2] != c.L[2])
+                return fail(QL_ERR_ARG, "%s: runs %d and %d: runs of one NZ share Lz", who, q, r);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(QL_ERR_HIP, "%s: no HIP device", who);
+    if (ps[0].device < 0 || ps[0].device >= ndev) return fail(QL_ERR_ARG, "%s: device %d of %d", who, ps[0].device, ndev);
+    QL_HIP(hipSetDevice(ps[0].device));
+    int rc = create_batch(b, ps, nruns);
+    if (rc) release(b);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ql_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// single run
+// ---------------------------------------------------------------------------
+int ql_create(const ql_params *p, ql_ctx **out)
+{
+    if (!p || !out) return fail(QL_ERR_ARG, "ql_create: null argument");
+    *out = nullptr;
+    ql_ctx *c = new ql_ctx();
+    const int rc = init_batch(c->bt.b, p, 1, "ql_create");
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return QL_OK;
+}
+
+int ql_destroy(ql_ctx *c)
+{
+    if (!c) return QL_OK;
+    (void)sync_all(c->bt.b);
+    release(c->bt.b);
+    delete c;
+    return QL_OK;
+}
+
+ql_batch *ql_batch_of(ql_ctx *c) { return c ? &c->bt : nullptr; }
+
+int ql_dims(const ql_ctx *c, int d[6])
+{
+    if (!c || !d) return fail(QL_ERR_ARG, "ql_dims: null argument");
+    return ql_batch_dims(&c->bt, 0, d);
+}
+
+int ql_set_state(ql_ctx *c, const double *lin, const double *mf, const double *kx0)
+{
+    if (!c) return fail(QL_ERR_ARG, "ql_set_state: null context");
+    return set_state(c->bt.b, 0, lin, mf, kx0);
+}
+
+int ql_get_state(ql_ctx *c, double *lin, double *mf, double *kx0)
+{
+    if (!c) return fail(QL_ERR_ARG, "ql_get_state: null context");
+    return get_state(c->bt.b, 0, lin, mf, kx0);
+}
+
+int ql_linear_ops_init(ql_ctx *c, double t0, double *linop, double *linop_mf)
+{
+    if (!c) return fail(QL_ERR_ARG, "ql_linear_ops_init: null context");
+    Batch &b = c->bt.b;
+    QL_RC(ql_batch_linear_ops_init(&c->bt, t0));
+    if (linop || linop_mf) {
+        const RunInfo &r = b.runs[0];
+        std::vector<double> k0(r.nxy);
+        QL_RC(get_state(b, 0, nullptr, nullptr, k0.data()));
+        linear_ops_host(b, r, k0, r.p.q * t0, linop, linop_mf);
     }
     return QL_OK;
 }
@@ -518,144 +944,249 @@ int ql_linear_ops_init(ql_ctx *c, double t0, double *linop, double *linop_mf)
 int ql_rhs(ql_ctx *c, double t, double dt_lin, double *rhs_lin, double *rhs_mf, double *linop)
 {
     if (!c) return fail(QL_ERR_ARG, "ql_rhs: null context");
-    int rc = refresh_profiles(c);
-    if (rc) return rc;
-    qlk::ModeArgs a = mode_args(c);
+    Batch &b = c->bt.b;
+    Group &g = b.groups[0];
+    const size_t nl = (size_t)g.modes * 4 * g.nz;
+    if (!g.lin_tmp) QL_RC(dalloc(&g.lin_tmp, nl));
+    QL_RC(refresh_profiles(b, g, b.s));
+    qlk::ModeArgs a = mode_args(b, g);
+    a.s = 0;
     a.mode = qlk::MODE_RHS;
-    a.qt = c->p.q * t;
-    a.qdtl = c->p.q * dt_lin;
-    a.n_out = c->N1;
-    a.lin_out = c->lin_tmp;
-    qlk::MFArgs m = mf_args(c);
+    a.ts_x = t;
+    a.dtl_x = dt_lin;
+    a.n_out = g.N1;
+    a.lin_out = g.lin_tmp;
+    qlk::MFArgs m = mf_args(b, g);
     m.mode = qlk::MODE_RHS;
-    m.mfn_out = c->mfN1;
-    if ((rc = run_sequence(c, a, m))) return rc;
-    const size_t nl = (size_t)c->nxy * 4 * c->nz;
-    if (rhs_lin) QL_HIP(hipMemcpyAsync(rhs_lin, c->N1, sizeof(double2) * nl, hipMemcpyDeviceToHost, c->s));
-    if (rhs_mf) QL_HIP(hipMemcpyAsync(rhs_mf, c->mfN1, sizeof(double2) * c->nMF * c->nz, hipMemcpyDeviceToHost, c->s));
-    if (linop) QL_HIP(hipMemcpyAsync(linop, c->lin_tmp, sizeof(double) * nl, hipMemcpyDeviceToHost, c->s));
-    QL_HIP(hipStreamSynchronize(c->s));
+    m.mfn_out = g.mfN1;
+    QL_RC(enqueue_sequence(b, g, a, m, b.s));
+    if (rhs_lin) QL_HIP(hipMemcpyAsync(rhs_lin, g.N1, sizeof(double2) * nl, hipMemcpyDeviceToHost, b.s));
+    if (rhs_mf) QL_HIP(hipMemcpyAsync(rhs_mf, g.mfN1, sizeof(double2) * b.nMF * g.nz, hipMemcpyDeviceToHost, b.s));
+    if (linop) QL_HIP(hipMemcpyAsync(linop, g.lin_tmp, sizeof(double) * nl, hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
 
 int ql_step(ql_ctx *c, double t, double *dt_out)
 {
     if (!c) return fail(QL_ERR_ARG, "ql_step: null context");
-    int rc = do_step(c, t, false, t);
-    if (rc) return rc;
-    QL_HIP(hipStreamSynchronize(c->s));
-    if (dt_out) *dt_out = c->p.dt;
+    Batch &b = c->bt.b;
+    QL_RC(set_clock(b, t));
+    for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, false));
+    b.t_prev = t;
+    b.old_init = 0;
+    QL_HIP(hipStreamSynchronize(b.s));
+    if (dt_out) *dt_out = b.dt;
     return QL_OK;
 }
 
 int ql_steps(ql_ctx *c, int nsteps, double *t_inout)
 {
     if (!c || !t_inout) return fail(QL_ERR_ARG, "ql_steps: null argument");
-    if (nsteps < 0) return fail(QL_ERR_ARG, "ql_steps: nsteps < 0");
-    double t = *t_inout;
-    for (int n = 0; n < nsteps; ++n) {
-        const double dt = c->p.dt;
-        const double t_new = t + dt;  // main.cpp:116
-        int rc = do_step(c, t, c->p.remap != 0, t_new);
-        if (rc) return rc;
-        t = t_new;
-    }
-    QL_HIP(hipStreamSynchronize(c->s));
-    *t_inout = t;
-    return QL_OK;
+    return ql_batch_steps(&c->bt, nsteps, t_inout);
 }
 
 int ql_remap(ql_ctx *c, double qt)
 {
     if (!c) return fail(QL_ERR_ARG, "ql_remap: null context");
-    const int grid = c->nxy < 1024 ? c->nxy : 1024;
-    qlk::k_remap<<<grid, qlk::NT, 0, c->s>>>(c->u, c->kx0, c->ky, c->nxy, c->nz, qt, c->remap_thr, c->remap_shift);
+    Batch &b = c->bt.b;
+    const RunInfo &r = b.runs[0];
+    Group &g = b.groups[r.group];
+    const int grid = r.nxy < 1024 ? r.nxy : 1024;
+    qlk::k_remap<<<grid, qlk::NT, 0, b.s>>>(g.u, g.kx0, g.ky, r.mode_off, r.nxy, g.nz, qt, r.remap_thr, r.remap_shift);
     QL_HIP(hipGetLastError());
-    QL_HIP(hipStreamSynchronize(c->s));
+    QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
 
 int ql_diagnostics(ql_ctx *c, double t, double out[18])
 {
     if (!c || !out) return fail(QL_ERR_ARG, "ql_diagnostics: null argument");
-    qlk::DiagArgs d{};
-    d.u = c->u;
-    d.kx0 = c->kx0;
-    d.ky = c->ky;
-    d.kzv = c->kzv;
-    d.nxy = c->nxy;
-    d.nz = c->nz;
-    d.qt = c->p.q * t;
-    d.nu = c->p.nu;
-    d.eta = c->p.eta;
-    d.part = c->dpart;
-    const int grid = c->nxy < 256 ? c->nxy : 256;
-    qlk::k_diag_modes<<<grid, qlk::NT, 0, c->s>>>(d);
-    QL_HIP(hipGetLastError());
-    qlk::DiagFinalArgs f{};
-    f.part = c->dpart;
-    f.nparts = grid;
-    f.mf = c->mf;
-    f.drive = c->drive;
-    f.kzv = c->kzv;
-    f.nz = c->nz;
-    f.nMF = c->nMF;
-    f.model = c->p.model;
-    f.NZ = c->NZ;
-    f.Lz = c->p.L[2];
-    f.totalN2 = c->totalN2;
-    f.nu = c->p.nu;
-    f.eta = c->p.eta;
-    f.kzfac = c->kzfac;
-    f.out = c->dout;
-    qlk::k_diag_final<<<1, 64, 0, c->s>>>(f);
-    QL_HIP(hipGetLastError());
-    QL_HIP(hipMemcpyAsync(out, c->dout, sizeof(double) * 18, hipMemcpyDeviceToHost, c->s));
-    QL_HIP(hipStreamSynchronize(c->s));
-    return QL_OK;
+    return diagnostics(c->bt.b, 0, t, out);
 }
 
 int ql_get_drive(ql_ctx *c, double *drive)
 {
     if (!c || !drive) return fail(QL_ERR_ARG, "ql_get_drive: null argument");
-    QL_HIP(hipMemcpyAsync(drive, c->drive, sizeof(double2) * c->nMF * c->nz, hipMemcpyDeviceToHost, c->s));
-    QL_HIP(hipStreamSynchronize(c->s));
-    return QL_OK;
+    return ql_batch_get_drive(&c->bt, 0, drive);
 }
 
 int ql_mf_real(ql_ctx *c, double *out)
 {
     if (!c || !out) return fail(QL_ERR_ARG, "ql_mf_real: null argument");
-    QL_DISPATCH_NZ(c->NZ, launch_mfreal, c, c->mfr_tmp);
+    Batch &b = c->bt.b;
+    const RunInfo &r = b.runs[0];
+    Group &g = b.groups[r.group];
+    const double2 *mf = g.mf + (size_t)r.gi * b.nMF * g.nz;
+    const double fac = 1.0 / g.NZ;
+    switch (g.NZ) {
+    case 32: qlk::k_mf_real<32><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
+    case 64: qlk::k_mf_real<64><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
+    case 128: qlk::k_mf_real<128><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
+    default: qlk::k_mf_real<256><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
+    }
     QL_HIP(hipGetLastError());
-    QL_HIP(hipMemcpyAsync(out, c->mfr_tmp, sizeof(double) * c->nMF * c->NZ, hipMemcpyDeviceToHost, c->s));
-    QL_HIP(hipStreamSynchronize(c->s));
+    QL_HIP(hipMemcpyAsync(out, b.mfr_tmp, sizeof(double) * b.nMF * g.NZ, hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
 
 int ql_synchronize(ql_ctx *c)
 {
     if (!c) return fail(QL_ERR_ARG, "ql_synchronize: null context");
-    QL_HIP(hipStreamSynchronize(c->s));
+    QL_HIP(hipStreamSynchronize(c->bt.b.s));
     return QL_OK;
 }
 
 int ql_profile_kernels(ql_ctx *c, int nsteps, double *t_inout, double avg_ms[2], int counts[2])
 {
     if (!c || !t_inout || !avg_ms || !counts) return fail(QL_ERR_ARG, "ql_profile_kernels: null argument");
+    if (nsteps < 0) return fail(QL_ERR_ARG, "ql_profile_kernels: nsteps < 0");
+    Batch &b = c->bt.b;
     for (int k = 0; k < 2; ++k) {
-        c->t_ms[k] = 0.0;
-        c->t_cnt[k] = 0;
+        b.t_ms[k] = 0.0;
+        b.t_cnt[k] = 0;
     }
-    c->timing = true;
-    int rc = ql_steps(c, nsteps, t_inout);
-    c->timing = false;
+    b.timing = true;
+    int rc = run_steps(b, nsteps, t_inout);
+    b.timing = false;
     if (rc) return rc;
     for (int k = 0; k < 2; ++k) {
-        counts[k] = c->t_cnt[k];
-        avg_ms[k] = c->t_cnt[k] ? c->t_ms[k] / c->t_cnt[k] : 0.0;
+        counts[k] = b.t_cnt[k];
+        avg_ms[k] = b.t_cnt[k] ? b.t_ms[k] / b.t_cnt[k] : 0.0;
     }
     return QL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sets of runs
+// ---------------------------------------------------------------------------
+int ql_batch_create(const ql_params *runs, int nruns, ql_batch **out)
+{
+    if (!runs || !out) return fail(QL_ERR_ARG, "ql_batch_create: null argument");
+    *out = nullptr;
+    ql_batch *bt = new ql_batch();
+    const int rc = init_batch(bt->b, runs, nruns, "ql_batch_create");
+    if (rc) {
+        delete bt;
+        return rc;
+    }
+    *out = bt;
+    return QL_OK;
+}
+
+int ql_batch_destroy(ql_batch *bt)
+{
+    if (!bt) return QL_OK;
+    (void)sync_all(bt->b);
+    release(bt->b);
+    delete bt;
+    return QL_OK;
+}
+
+int ql_batch_size(const ql_batch *bt, int *nruns)
+{
+    if (!bt || !nruns) return fail(QL_ERR_ARG, "ql_batch_size: null argument");
+    *nruns = (int)bt->b.runs.size();
+    return QL_OK;
+}
+
+int ql_batch_dims(const ql_batch *bt, int run, int d[6])
+{
+    if (!bt || !d) return fail(QL_ERR_ARG, "ql_batch_dims: null argument");
+    QL_RC(check_run(bt->b, run, "ql_batch_dims"));
+    const RunInfo &r = bt->b.runs[run];
+    d[0] = r.nxy;
+    d[1] = r.nz;
+    d[2] = bt->b.nMF;
+    d[3] = 4;
+    d[4] = r.half;
+    d[5] = r.p.NZ;
+    return QL_OK;
+}
+
+int ql_batch_set_state(ql_batch *bt, int run, const double *lin, const double *mf, const double *kx0)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_set_state: null handle");
+    QL_RC(check_run(bt->b, run, "ql_batch_set_state"));
+    return set_state(bt->b, run, lin, mf, kx0);
+}
+
+int ql_batch_get_state(ql_batch *bt, int run, double *lin, double *mf, double *kx0)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_get_state: null handle");
+    QL_RC(check_run(bt->b, run, "ql_batch_get_state"));
+    return get_state(bt->b, run, lin, mf, kx0);
+}
+
+int ql_batch_linear_ops_init(ql_batch *bt, double t0)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_linear_ops_init: null handle");
+    bt->b.old_init = 1;
+    bt->b.t_init = t0;
+    bt->b.t_prev = t0;
+    return QL_OK;
+}
+
+int ql_batch_steps(ql_batch *bt, int nsteps, double *t_inout)
+{
+    if (!bt || !t_inout) return fail(QL_ERR_ARG, "ql_batch_steps: null argument");
+    if (nsteps < 0) return fail(QL_ERR_ARG, "ql_batch_steps: nsteps < 0");
+    return run_steps(bt->b, nsteps, t_inout);
+}
+
+int ql_batch_diagnostics(ql_batch *bt, int run, double t, double out[18])
+{
+    if (!bt || !out) return fail(QL_ERR_ARG, "ql_batch_diagnostics: null argument");
+    QL_RC(check_run(bt->b, run, "ql_batch_diagnostics"));
+    return diagnostics(bt->b, run, t, out);
+}
+
+int ql_batch_get_drive(ql_batch *bt, int run, double *drive)
+{
+    if (!bt || !drive) return fail(QL_ERR_ARG, "ql_batch_get_drive: null argument");
+    QL_RC(check_run(bt->b, run, "ql_batch_get_drive"));
+    Batch &b = bt->b;
+    const RunInfo &r = b.runs[run];
+    Group &g = b.groups[r.group];
+    QL_HIP(hipMemcpyAsync(drive, g.drive + (size_t)r.gi * b.nMF * g.nz, sizeof(double2) * b.nMF * g.nz,
+                          hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
+    return QL_OK;
+}
+
+int ql_batch_tune(ql_batch *bt, int trial_steps, double *step_ms)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_tune: null handle");
+    return tune(bt->b, trial_steps, step_ms);
+}
+
+int ql_batch_get_config(const ql_batch *bt, int max_groups, int *ngroups, int *cfg)
+{
+    if (!bt || !ngroups) return fail(QL_ERR_ARG, "ql_batch_get_config: null argument");
+    const Batch &b = bt->b;
+    *ngroups = (int)b.groups.size();
+    if (cfg)
+        for (int k = 0; k < max_groups && k < (int)b.groups.size(); ++k) {
+            const Group &g = b.groups[k];
+            cfg[5 * k + 0] = g.NZ;
+            cfg[5 * k + 1] = g.line;
+            cfg[5 * k + 2] = g.target;
+            cfg[5 * k + 3] = g.MB;
+            cfg[5 * k + 4] = g.W;
+        }
+    return QL_OK;
+}
+
+int ql_batch_set_config(ql_batch *bt, int NZ, int line, int target)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_set_config: null handle");
+    if (line != 128 && line != 256) return fail(QL_ERR_ARG, "ql_batch_set_config: line=%d not in {128, 256}", line);
+    if (target < 1) return fail(QL_ERR_ARG, "ql_batch_set_config: target=%d < 1", target);
+    Batch &b = bt->b;
+    for (Group &g : b.groups)
+        if (g.NZ == NZ) return configure(b, g, line, target);
+    return fail(QL_ERR_ARG, "ql_batch_set_config: no runs with NZ=%d", NZ);
 }
 
 }  // extern "C"

@@ -1,20 +1,20 @@
-// ql_kernels.hpp -- the per-substep kernels of the QL time step (FP64).
+// ql_kernels.hpp -- shared structures and the non-modal kernels of the QL time
+// step.
 //
-// One RK3 substep is three launches on one stream:
-//   k_mode_*      fused per-(kx,ky) RHS + substep update.  A workgroup takes MB
-//                 modes at a time: builds the padded spectral lines of u, b and
-//                 their z-derivatives, inverse-transforms them in LDS, forms the
-//                 mean-field x fluctuation products on the z grid, forward-
-//                 transforms them, assembles the RHS, applies the exp / CN
-//                 linear factor computed on the fly from kx(t), and writes the
-//                 new state (and the RHS when a later substep needs it).  The
-//                 EMF / Reynolds-stress z-profiles of its modes are summed in
-//                 registers in a fixed order and written as one partial per WG.
-//   k_reduce_cols fixed-order sum of the per-workgroup partials, column-parallel.
-//   k_meanfield   one workgroup: forward transform of the summed profiles,
-//                 x i kz, Hermitian fill (the reference's MPI Allreduce + fill,
-//                 MHD_FullQlin.cpp:386-399), mean-field RHS and update, and the
-//                 real-space mean-field profiles the next substep reads.
+// Runs are processed in groups that share NZ (one kernel instantiation per NZ).
+// A group's runs are stored one after the other in its arrays; every run has
+// its own physical parameters (RunParams), mean fields and reductions.  One RK3
+// substep of a group is three launches on one stream:
+//   k_mode<..>    fused per-(kx,ky) RHS + substep update (ql_mode.hpp); each
+//                 workgroup owns a contiguous range of one run's modes and
+//                 writes one EMF / stress partial row for that run;
+//   k_reduce_cols fixed-order, compensated sum of each run's partial rows;
+//   k_meanfield   one workgroup per run: forward transform of the summed
+//                 profiles, x i kz, Hermitian fill (the reference's MPI
+//                 Allreduce + fill, MHD_FullQlin.cpp:386-399), mean-field RHS
+//                 and update, and the real-space profiles the next substep reads.
+// Step times live on the device (Clock) and advance by one tiny kernel per
+// step, so a whole step can be captured once in a HIP graph and replayed.
 #pragma once
 #include "ql_fft.hpp"
 
@@ -22,27 +22,90 @@ namespace qlk {
 
 enum { MODE_RHS = 0, MODE_ELIN = 1, MODE_CN = 2, MODE_PROFILES = 3 };
 
+// physical parameters of one run
+struct Phys {
+    double q, omega, B0z, nu, eta;
+    int vo2;  // viscosity_order / 2 (integer division, MHD_FullQlin.cpp:372)
+    int quasilinear;
+};
 
+struct RunParams {
+    Phys ph;
+    double rsf;          // 1/(NX NY)^2 (MHD_FullQlin.cpp:67)
+    double remap_thr;    // (NX-1)/2 * 2 pi/Lx (MHD_FullQlin.cpp:472)
+    double remap_shift;  // (NX-1) * 2 pi/Lx (MHD_FullQlin.cpp:474)
+    int remap;           // Remap?
+    int nxy;             // modes of the run
+    int mode_off;        // first mode of the run in the group arrays
+    int wg_off, nwg;     // its mode-kernel workgroups = its partial rows
+};
+
+// time of a set of runs stepped together: t as main.cpp:116 accumulates it,
+// t_prev the previous step start; old_init: the RK3CN "old" operator is the
+// one of linearOPs_Init at t_init (RK3CN.cpp:57)
+struct Clock {
+    double t, t_prev, t_init;
+    int old_init, pad;
+};
 
 struct ModeArgs {
-    double2 *u;             // state [nxy][4][nz], read and (MODE_ELIN/CN) updated in place
+    double2 *u;             // state [modes][4][nz], updated in place (MODE_ELIN/CN)
     double2 *n_out;         // RHS out (nullable)
     const double2 *n_prev;  // previous substep RHS (nullable)
     double *lin_out;        // linear operator out (MODE_RHS, nullable)
-    double *kx0;            // [nxy] (remap updates it)
-    const double *ky;       // [nxy]
+    double *kx0;            // [modes] (remap updates it)
+    const double *ky;       // [modes]
     const double *kzv;      // [nz] physical kz
     const double2 *tw;      // [NZ] exp(-2 pi i k/NZ)
     const float2 *tw32;     // the same, rounded to FP32
-    const double *mfz;      // real-space mean-field profiles [.][NZ]
-    double *partials;       // [gridDim.x][nMF][NZ]
-    int nxy, vo2, mode, old_init, do_remap;
-    double qt, qdtl;        // q * t_s and q * dt_lin
-    double q, omega, B0z, nu, eta, fac, rsf;
-    double linC, nlc1, nlc2;
-    double qt_old, qdtl_old;  // where the CN "old" operator was evaluated
-    double remap_qt, remap_thr, remap_shift;
+    const double *mfz;      // real-space mean-field profiles [runs][8][NZ]
+    double *partials;       // [workgroups][nacc][NZ]
+    const RunParams *runs;  // [runs]
+    const int *wg_run, *wg_g0, *wg_g1;  // per workgroup: run, mode-group range
+    const Clock *clock;
+    int s;                  // substep 1..3, or 0 for the explicit times below
+    int mode, do_remap;
+    double ts_x, dtl_x;     // explicit rhs time and dt_lin (s == 0)
+    double dt, linC, nlc1, nlc2;
+    double dtP1, dtP2, dtl1, dtl2, dtl3;  // dt*8/15, dt*2/3, and the three dt_lin
+    double fac;             // 1/NZ
 };
+
+// substep times from the clock, with the host's roundings (unfused): ts,
+// dt_lin, where the RK3CN "old" operator was evaluated, and t + dt
+struct SubTimes {
+    double ts, dtl, tso, dtlo, tnew;
+    bool oinit;
+};
+__device__ __forceinline__ SubTimes sub_times(const ModeArgs &a)
+{
+    SubTimes r{};
+    if (a.s == 0) {
+        r.ts = a.ts_x;
+        r.dtl = a.dtl_x;
+        return r;
+    }
+    const double t = a.clock->t;
+    r.tnew = __dadd_rn(t, a.dt);
+    if (a.s == 1) {
+        r.ts = t;
+        r.dtl = a.dtl1;
+        r.oinit = a.clock->old_init != 0;
+        r.tso = r.oinit ? a.clock->t_init : __dadd_rn(a.clock->t_prev, a.dtP2);
+        r.dtlo = r.oinit ? 0.0 : a.dtl3;
+    } else if (a.s == 2) {
+        r.ts = __dadd_rn(t, a.dtP1);
+        r.dtl = a.dtl2;
+        r.tso = t;
+        r.dtlo = a.dtl1;
+    } else {
+        r.ts = __dadd_rn(t, a.dtP2);
+        r.dtl = a.dtl3;
+        r.tso = __dadd_rn(t, a.dtP1);
+        r.dtlo = a.dtl2;
+    }
+    return r;
+}
 
 // -|x|^(order/2): the dissipation exponent uses integer division
 // (MHD_FullQlin.cpp:372); order 2 and 4 avoid pow.
@@ -57,28 +120,46 @@ __device__ __forceinline__ double neg_abs_pow(double x, int vo2)
 
 // remap predicate of ShearingBox_Remap (MHD_FullQlin.cpp:470-472), evaluated
 // with unfused roundings so it agrees bit for bit with the host expression
-__device__ __forceinline__ bool remap_hit(const ModeArgs &a, double kx0, double ky)
+__device__ __forceinline__ bool remap_hit(double qt, double thr, double kx0, double ky)
 {
-    return __dadd_rn(kx0, __dmul_rn(a.remap_qt, ky)) > a.remap_thr;
+    return __dadd_rn(kx0, __dmul_rn(qt, ky)) > thr;
+}
+
+// the host's view of the clock, passed by value (no host buffer involved)
+__global__ void k_clock_set(Clock *c, Clock v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
+}
+
+// t_prev <- t, t <- t + dt (main.cpp:116), once per step
+__global__ void k_clock_advance(Clock *c, double dt)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double t = c->t;
+        c->t_prev = t;
+        c->t = __dadd_rn(t, dt);
+        c->old_init = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
-// stress / EMF reduction: column c of the mode-kernel partials summed over all
-// rows with Neumaier compensation.  A block owns 16 columns; its 16 row chunks
-// are summed in row order by separate threads and then added in chunk order,
-// so the result does not depend on scheduling.  No cross-block communication
-// (no fences, no atomics).
+// stress / EMF reduction: column c of run r's partial rows summed with
+// Neumaier compensation.  A block (blockIdx.y = run) owns 16 columns; its 16
+// row chunks are summed in row order by separate threads and then added in
+// chunk order, so the result does not depend on scheduling.  No cross-block
+// communication (no fences, no atomics).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, int nrows, int ncols,
-                                                    double *__restrict__ sums)
+__global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, const RunParams *__restrict__ runs,
+                                                    int ncols, double *__restrict__ sums)
 {
     __shared__ double red[16][17], red2[16][17];
+    const RunParams &rp = runs[blockIdx.y];
     const int cl = threadIdx.x % 16, chunk = threadIdx.x / 16;
     const int c = blockIdx.x * 16 + cl;
+    const int nrows = rp.nwg;
     const int per = (nrows + 15) / 16;
-    const int r0 = chunk * per, r1 = min(nrows, r0 + per);
+    const int r0 = rp.wg_off + chunk * per, r1 = rp.wg_off + min(nrows, chunk * per + per);
     constexpr int B = 16;
-    // Neumaier-compensated running sum (s, e)
     double s = 0.0, e = 0.0;
     if (c < ncols) {
         for (int r = r0; r < r1; r += B) {
@@ -105,27 +186,25 @@ __global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ p
             te += red2[k][cl];
             t = u;
         }
-        sums[c] = t + te;
+        sums[(size_t)blockIdx.y * ncols + c] = t + te;
     }
 }
 
 // ---------------------------------------------------------------------------
-// mean fields (one workgroup): forward transform of the reduced profiles,
-// x i kz, Hermitian fill (the reference's Allreduce + fill,
-// MHD_FullQlin.cpp:386-399), mean-field RHS and update, and the real-space
-// profiles the next substep reads
+// mean fields, one workgroup per run (blockIdx.x)
 // ---------------------------------------------------------------------------
 struct MFArgs {
-    const double *sums;   // reduced stress / EMF z-profiles [nMF][NZ]
-    double2 *mf;              // [nMF][nz]
-    double2 *mfn_out;         // mean-field RHS out [nMF][nz]
+    const double *sums;       // reduced stress / EMF z-profiles [runs][nMF][NZ]
+    double2 *mf;              // [runs][nMF][nz]
+    double2 *mfn_out;         // mean-field RHS out
     const double2 *mfn_prev;  // previous substep mean-field RHS (nullable)
-    double2 *drive;           // [nMF][nz]
-    double *mfz;              // profiles out
+    double2 *drive;           // [runs][nMF][nz]
+    double *mfz;              // profiles out [runs][8][NZ]
+    const RunParams *runs;
     const double *kzv;
     const double2 *tw;
-    int model, nMF, quasilinear, mode;
-    double q, omega, B0z, nu, eta, fac, linC, nlc1, nlc2;
+    int model, nMF, mode;
+    double fac, linC, nlc1, nlc2;
 };
 
 template <int NZ>
@@ -141,7 +220,10 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     __shared__ double2 sdrive[4 * NZD]; // drives
     __shared__ double2 smfn[4 * NZD];   // updated mean fields
     const int tid = threadIdx.x;
+    const int run = blockIdx.x;
+    const Phys ph = a.runs[run].ph;
     const int nMF = a.nMF, nslot = nMF * NZD;
+    const size_t mo = (size_t)run * nslot;
     for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
     for (int k = tid; k < NZD; k += NT) skz[k] = a.kzv[k];
 
@@ -150,10 +232,10 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
 #pragma unroll
         for (int w = 0; w < MW; ++w) {
             const int k = tid + w * NT;
-            pmf[w] = (k < nslot) ? a.mf[k] : make_double2(0.0, 0.0);
-            pprev[w] = (k < nslot && a.mfn_prev) ? a.mfn_prev[k] : make_double2(0.0, 0.0);
+            pmf[w] = (k < nslot) ? a.mf[mo + k] : make_double2(0.0, 0.0);
+            pprev[w] = (k < nslot && a.mfn_prev) ? a.mfn_prev[mo + k] : make_double2(0.0, 0.0);
         }
-        for (int c = tid; c < nMF * NZ; c += NT) buf[c] = make_double2(a.sums[c], 0.0);
+        for (int c = tid; c < nMF * NZ; c += NT) buf[c] = make_double2(a.sums[(size_t)run * nMF * NZ + c], 0.0);
 #pragma unroll
         for (int w = 0; w < MW; ++w) {
             const int k = tid + w * NT;
@@ -176,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
             if (f < 2) v = imul(skz[src], v);
             if (cj) v.y = -v.y;
             sdrive[k] = v;
-            a.drive[k] = v;
+            a.drive[mo + k] = v;
         }
         __syncthreads();
         // mean-field RHS (MHD_FullQlin.cpp:411-423, MHD_BQlin.cpp:274-280) and
@@ -189,22 +271,22 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
             const double kz = skz[m], kz2 = -(kz * kz);
             double2 N = make_double2(0.0, 0.0);
             double Lmf = 0.0;
-            if (a.quasilinear) {
+            if (ph.quasilinear) {
                 const double2 Dv = sdrive[k];
                 if (a.model == 0) {
                     const double2 M0 = smf[m], M1 = smf[NZD + m], M2 = smf[2 * NZD + m], M3 = smf[3 * NZD + m];
-                    if (f == 0) N = cadd(imul(a.B0z * kz, M2), Dv);
-                    else if (f == 1) N = cadd(cadd(cscale(-a.q, M0), imul(a.B0z * kz, M3)), Dv);
-                    else if (f == 2) N = cadd(cadd(cscale(2 * a.omega, M3), imul(a.B0z * kz, M0)), Dv);
-                    else N = cadd(cadd(cscale(a.q - 2.0 * a.omega, M2), imul(a.B0z * kz, M1)), Dv);
-                    Lmf = (f < 2 ? a.eta : a.nu) * kz2;
+                    if (f == 0) N = cadd(imul(ph.B0z * kz, M2), Dv);
+                    else if (f == 1) N = cadd(cadd(cscale(-ph.q, M0), imul(ph.B0z * kz, M3)), Dv);
+                    else if (f == 2) N = cadd(cadd(cscale(2 * ph.omega, M3), imul(ph.B0z * kz, M0)), Dv);
+                    else N = cadd(cadd(cscale(ph.q - 2.0 * ph.omega, M2), imul(ph.B0z * kz, M1)), Dv);
+                    Lmf = (f < 2 ? ph.eta : ph.nu) * kz2;
                 } else {
                     if (f == 0) N = Dv;
-                    else N = cadd(cscale(-a.q, smf[m]), Dv);
-                    Lmf = a.eta * kz2;
+                    else N = cadd(cscale(-ph.q, smf[m]), Dv);
+                    Lmf = ph.eta * kz2;
                 }
             }
-            a.mfn_out[k] = N;
+            a.mfn_out[mo + k] = N;
             if (a.mode == MODE_RHS) continue;
             const double2 M = smf[k];
             double2 Mn;
@@ -218,13 +300,13 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
                 if (a.mfn_prev) sacc = cadd(sacc, cscale(a.nlc2 * denom, pprev[w]));
                 Mn = cadd(sacc, cscale((1 + a.linC * Lmf) * denom, M));
             }
-            a.mf[k] = Mn;
+            a.mf[mo + k] = Mn;
             smfn[k] = Mn;
         }
         __syncthreads();
         if (a.mode == MODE_RHS) return;
     } else {
-        for (int k = tid; k < nslot; k += NT) smfn[k] = a.mf[k];
+        for (int k = tid; k < nslot; k += NT) smfn[k] = a.mf[mo + k];
         __syncthreads();
     }
     // real-space profiles for the next rhs (MHD_FullQlin.cpp:180-198,
@@ -250,18 +332,18 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     }
     __syncthreads();
     fft_lines<NZ, 8, true>(buf, tw);
-    for (int idx = tid; idx < nlines * NZ; idx += NT) a.mfz[idx] = buf[idx].x;
+    for (int idx = tid; idx < nlines * NZ; idx += NT) a.mfz[(size_t)run * 8 * NZ + idx] = buf[idx].x;
 }
 
 // ---------------------------------------------------------------------------
-// ShearingBox_Remap (MHD_FullQlin.cpp:464-481) as a standalone pass
+// ShearingBox_Remap (MHD_FullQlin.cpp:464-481) over modes [m0, m0 + nxy)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_remap(double2 *u, double *kx0, const double *ky, int nxy, int nz,
+__global__ __launch_bounds__(NT) void k_remap(double2 *u, double *kx0, const double *ky, int m0, int nxy, int nz,
                                               double qt, double thr, double shift)
 {
-    for (int i = blockIdx.x; i < nxy; i += gridDim.x) {
+    for (int i = m0 + blockIdx.x; i < m0 + nxy; i += gridDim.x) {
         const double k0 = kx0[i], kyv = ky[i];
-        const bool hit = __dadd_rn(k0, __dmul_rn(qt, kyv)) > thr;
+        const bool hit = remap_hit(qt, thr, k0, kyv);
         __syncthreads();
         if (hit) {
             for (int k = threadIdx.x; k < 4 * nz; k += NT) u[(size_t)i * 4 * nz + k] = make_double2(0.0, 0.0);
@@ -272,12 +354,12 @@ __global__ __launch_bounds__(NT) void k_remap(double2 *u, double *kx0, const dou
 
 // ---------------------------------------------------------------------------
 // Calc_Energy_AM_Diss (MHD_FullQlin.cpp:515-559): per-workgroup partial sums
-// of energy, angular momentum and dissipation over modes, in a fixed order
+// of energy, angular momentum and dissipation over a run's modes, fixed order
 // ---------------------------------------------------------------------------
 struct DiagArgs {
     const double2 *u;
     const double *kx0, *ky, *kzv;
-    int nxy, nz;
+    int m0, nxy, nz;
     double qt, nu, eta;
     double *part;  // [gridDim.x][6]
 };
@@ -287,7 +369,7 @@ __global__ __launch_bounds__(NT) void k_diag_modes(DiagArgs a)
     __shared__ double sred[6][NT];
     double s[6] = {0, 0, 0, 0, 0, 0};
     const int nz = a.nz;
-    for (int i = blockIdx.x; i < a.nxy; i += gridDim.x) {
+    for (int i = a.m0 + blockIdx.x; i < a.m0 + a.nxy; i += gridDim.x) {
         const double ky = a.ky[i];
         const double kx = __dadd_rn(a.kx0[i], __dmul_rn(a.qt, ky));
         const double mult = (ky == 0.0) ? 1.0 : 2.0;

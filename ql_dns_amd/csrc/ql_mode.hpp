@@ -6,10 +6,10 @@
 //   ModelBQ    MHD_BQlin (Models/MHD_BQlin.cpp:117-284): fluctuations see
 //              By(z), dBy/dz, d2By/dz2 only; the two EMF components reduced.
 //
-// A 256-thread workgroup grid-strides over groups of MB modes.  Per group:
+// A 256-thread workgroup takes a contiguous range of one run's groups of MB
+// modes.  Per group:
 //   A  the group's state rows -> LDS (prefetched into registers during the
-//      previous group), the previous substep's RHS for this group's update
-//      slots -> registers, the next group's rows -> registers;
+//      previous group), the next group's rows -> registers;
 //   B  per (mode, kz slot): 1/lap2, 1/lapF and the base spectral fields
 //      (fac1D folded in; y, z components rebuilt from (ux, zeta), (bx, eta));
 //   C  inverse transforms of the padded lines, the first pass reading the
@@ -23,22 +23,38 @@
 #pragma once
 #include "ql_kernels.hpp"
 
-// register prefetch of the next group's state rows / of the previous
-// substep's RHS (1 = on)
+// register prefetch of the next group's state rows (1 = on)
 #ifndef QL_PREFETCH_ROWS
 #define QL_PREFETCH_ROWS 1
 #endif
-#ifndef QL_PREFETCH_NPREV
-#define QL_PREFETCH_NPREV 0
-#endif
+// occupancy requests (waves per SIMD, which bound the VGPR budget) for
+// 256-point passes and for 128-point passes (half the modes per pass)
 #ifndef QL_WAVES64
 #define QL_WAVES64 2
 #endif
 #ifndef QL_WAVES32
 #define QL_WAVES32 2
 #endif
+#ifndef QL_WAVES64_HALF
+#define QL_WAVES64_HALF 2
+#endif
+#ifndef QL_WAVES32_HALF
+#define QL_WAVES32_HALF 2
+#endif
 
 namespace qlk {
+
+// A wave-uniform double moved to scalar registers (both halves read from the
+// first active lane).  FP64 arithmetic runs on the vector ALU only, so a
+// uniform value computed in the kernel would otherwise hold a VGPR pair for
+// the kernel's lifetime.
+__device__ __forceinline__ double to_sgpr(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // Re(a conj b) formed in FP64 (exact for FP32 inputs)
 template <class C>
@@ -122,7 +138,7 @@ struct ModelFull {
     // MHD_FullQlin.cpp:283-285 (pair 0: ux, zeta) and :303-305 (pair 1: bx, eta)
     template <class C>
     __device__ static void rhs_pair(int pr, const C *T, int ld, double2 ux, double2 ze, double2 bx, double2 et,
-                                    double kx, double ky, double kz, double ilapF, const ModeArgs &a, double2 &N0,
+                                    double kx, double ky, double kz, double ilapF, const Phys &a, double2 &N0,
                                     double2 &N1)
     {
         const double q = a.q, Om = a.omega, B0z = a.B0z;
@@ -144,11 +160,11 @@ struct ModelFull {
         }
     }
     // -coef |k^2|^(order/2) at the substep end (:371-376) and at t0 (:432-440)
-    __device__ static double lin(int pr, double kxn, double ky, double kz2, const ModeArgs &a)
+    __device__ static double lin(int pr, double kxn, double ky, double kz2, const Phys &a)
     {
         return (pr == 0 ? a.nu : a.eta) * neg_abs_pow((-kxn * kxn - ky * ky) + kz2, a.vo2);
     }
-    __device__ static double lin_init(int pr, double kxo, double lap2, const ModeArgs &a)
+    __device__ static double lin_init(int pr, double kxo, double lap2, const Phys &a)
     {
         return (pr == 0 ? a.nu : a.eta) * neg_abs_pow(-kxo * kxo + lap2, a.vo2);
     }
@@ -209,7 +225,7 @@ struct ModelBQ {
     // MHD_BQlin.cpp:160-162, :177-178 (pair 0), :185, :200-201 (pair 1)
     template <class C>
     __device__ static void rhs_pair(int pr, const C *T, int ld, double2 ux, double2 ze, double2 bx, double2,
-                                    double kx, double ky, double kz, double ilapF, const ModeArgs &a, double2 &N0,
+                                    double kx, double ky, double kz, double ilapF, const Phys &a, double2 &N0,
                                     double2 &N1)
     {
         const double q = a.q, Om = a.omega;
@@ -223,12 +239,12 @@ struct ModelBQ {
         }
     }
     // nu (-k^2) for u, zeta; (eta/nu) times that for b, eta (:248-252, :289-295)
-    __device__ static double lin(int pr, double kxn, double ky, double kz2, const ModeArgs &a)
+    __device__ static double lin(int pr, double kxn, double ky, double kz2, const Phys &a)
     {
         const double L0 = a.nu * ((-kxn * kxn - ky * ky) + kz2);
         return pr == 0 ? L0 : (a.eta / a.nu) * L0;
     }
-    __device__ static double lin_init(int pr, double kxo, double lap2, const ModeArgs &a)
+    __device__ static double lin_init(int pr, double kxo, double lap2, const Phys &a)
     {
         const double L0 = a.nu * (-kxo * kxo + lap2);
         return pr == 0 ? L0 : (a.eta / a.nu) * L0;
@@ -273,13 +289,27 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     __shared__ R skzr[NZD];
     __shared__ double silF[MB * NZD];
     __shared__ double skx0[MB], sky[MB], skx[MB];
-    __shared__ int sidx[MB];
+    __shared__ int sidx[MB];   // mode index in the group arrays (-1: empty slot)
+    __shared__ int sloc[MB];   // mode index within its run
     // mean-field profiles as [z][profile]: in LDS up to NZ = 128, in
     // registers (the thread's fixed z points) at NZ = 256 where LDS is short
     constexpr bool MF_LDS = (NZ <= 128) || sizeof(C) == 8;
     __shared__ R smfz[MF_LDS ? NZ * P::NMFZ : 1];
     R mfr[MF_LDS ? 1 : PW][P::NMFZ];
     const int tid = threadIdx.x;
+
+    // this workgroup's run, its mode groups [g0, g1), the substep times
+    const int run = a.wg_run[blockIdx.x];
+    const RunParams rp = a.runs[run];
+    const Phys ph = rp.ph;
+    const int g0 = a.wg_g0[blockIdx.x], g1 = a.wg_g1[blockIdx.x];
+    // q t products: wave-uniform, kept in scalar registers
+    const SubTimes st = sub_times(a);
+    const double qt = to_sgpr(__dmul_rn(ph.q, st.ts)), qdtl = to_sgpr(__dmul_rn(ph.q, st.dtl));
+    const double qt_old = to_sgpr(__dmul_rn(ph.q, st.tso)), qdtl_old = to_sgpr(__dmul_rn(ph.q, st.dtlo));
+    const double remap_qt = to_sgpr(__dmul_rn(ph.q, st.tnew));
+    const bool do_remap = a.do_remap && rp.remap;
+    const double *mfz = a.mfz + (size_t)run * 8 * NZ;
 
     const C *twg = reinterpret_cast<const C *>(sizeof(C) == 8 ? a.tw32 : (const void *)a.tw);
     for (int k = tid; k < NZ; k += NT) tw[k] = twg[k];
@@ -288,13 +318,13 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         skzr[k] = (R)a.kzv[k];
     }
     if constexpr (MF_LDS) {
-        for (int k = tid; k < NZ * P::NMFZ; k += NT) smfz[(k % NZ) * P::NMFZ + k / NZ] = (R)a.mfz[k];
+        for (int k = tid; k < NZ * P::NMFZ; k += NT) smfz[(k % NZ) * P::NMFZ + k / NZ] = (R)mfz[k];
     } else {
 #pragma unroll
         for (int p = 0; p < PW; ++p) {
             const int item = tid + p * NT;
 #pragma unroll
-            for (int c = 0; c < P::NMFZ; ++c) mfr[p][c] = (item < PI_) ? (R)a.mfz[c * NZ + item % NZ] : (R)0;
+            for (int c = 0; c < P::NMFZ; ++c) mfr[p][c] = (item < PI_) ? (R)mfz[c * NZ + item % NZ] : (R)0;
         }
     }
 
@@ -302,36 +332,36 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 #pragma unroll
     for (int p = 0; p < PW; ++p) acc[p][0] = acc[p][1] = 0.0;
 
-    const int ngroups = (a.nxy + MB - 1) / MB;
+    // run-local mode index of slot mb of group g, or -1
+    auto local_mode = [&](int g, int mb) { const int l = g * MB + mb; return l < rp.nxy ? l : -1; };
     double2 upre[UW];
     auto fetch_rows = [&](int g) {
-        if (!QL_PREFETCH_ROWS && g != blockIdx.x) return;
 #pragma unroll
         for (int w = 0; w < UW; ++w) {
             const int k = tid + w * NT;
             const int mb = k / (4 * NZD);
-            const int i = g * MB + mb;
-            upre[w] = (k < NU && g < ngroups && i < a.nxy) ? a.u[(size_t)i * 4 * NZD + (k - mb * 4 * NZD)]
-                                                          : make_double2(0.0, 0.0);
+            const int l = (k < NU && g < g1) ? local_mode(g, mb) : -1;
+            upre[w] = (l >= 0) ? a.u[(size_t)(rp.mode_off + l) * 4 * NZD + (k - mb * 4 * NZD)] : make_double2(0.0, 0.0);
         }
     };
-    fetch_rows(blockIdx.x);
+    fetch_rows(g0);
 
-    for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
-        // A: this group's rows to LDS, previous RHS and next rows to registers
+    for (int g = g0; g < g1; ++g) {
+        // A: this group's rows to LDS, the next group's rows to registers
         if (tid < MB) {
-            const int i = g * MB + tid;
-            sidx[tid] = i;
-            if (i < a.nxy) {
-                const double kx0 = a.kx0[i], ky = a.ky[i];
+            const int l = local_mode(g, tid);
+            sloc[tid] = l;
+            sidx[tid] = l >= 0 ? rp.mode_off + l : -1;
+            if (l >= 0) {
+                const double kx0 = a.kx0[rp.mode_off + l], ky = a.ky[rp.mode_off + l];
                 skx0[tid] = kx0;
                 sky[tid] = ky;
-                skx[tid] = __dadd_rn(kx0, __dmul_rn(a.qt, ky));  // kx(t), MHD_FullQlin.cpp:727
+                skx[tid] = __dadd_rn(kx0, __dmul_rn(qt, ky));  // kx(t), MHD_FullQlin.cpp:727
             } else {
                 skx0[tid] = sky[tid] = skx[tid] = 0.0;
             }
         }
-        if (QL_PREFETCH_ROWS || g == blockIdx.x) {
+        if (QL_PREFETCH_ROWS || g == g0) {
 #pragma unroll
             for (int w = 0; w < UW; ++w) {
                 const int k = tid + w * NT;
@@ -340,27 +370,11 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         } else {
             for (int k = tid; k < NU; k += NT) {
                 const int mb = k / (4 * NZD);
-                const int i = g * MB + mb;
-                uin[k] = (i < a.nxy) ? a.u[(size_t)i * 4 * NZD + (k - mb * 4 * NZD)] : make_double2(0.0, 0.0);
+                const int l = local_mode(g, mb);
+                uin[k] = (l >= 0) ? a.u[(size_t)(rp.mode_off + l) * 4 * NZD + (k - mb * 4 * NZD)] : make_double2(0.0, 0.0);
             }
         }
-        double2 npre[AW][2];
-#pragma unroll
-        for (int w = 0; w < AW; ++w) {
-            npre[w][0] = npre[w][1] = make_double2(0.0, 0.0);
-            const int k = tid + w * NT;
-            if (QL_PREFETCH_NPREV && a.n_prev && k < AI) {
-                const int mb = k / (2 * NZD), r2 = k % (2 * NZD);
-                const int pr = r2 / NZD, m = r2 % NZD;
-                const int i = g * MB + mb;
-                if (i < a.nxy) {
-                    const size_t off0 = (size_t)i * 4 * NZD + (2 * pr) * NZD + m;
-                    npre[w][0] = a.n_prev[off0];
-                    npre[w][1] = a.n_prev[off0 + NZD];
-                }
-            }
-        }
-        fetch_rows(g + gridDim.x);
+        if (QL_PREFETCH_ROWS) fetch_rows(g + 1);
         __syncthreads();
 
         // B: per-slot inverse Laplacians (assign_laplacians_, MHD_FullQlin.cpp:722-738,
@@ -371,7 +385,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             const bool zero = (ky == 0.0 && m == 0);
             const double lap2 = zero ? 0.0 : (-ky * ky + -(kz * kz));
             const double ilap2 = zero ? 1.0 : 1.0 / lap2;
-            silF[k] = (sidx[mb] == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
+            silF[k] = (sloc[mb] == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
             const double2 *U = uin + mb * 4 * NZD;
             P::bases(U[m], U[NZD + m], U[2 * NZD + m], U[3 * NZD + m], kx, ky, kz, ilap2, a.fac,
                      sbase + mb * P::NB * NZD + m, NZD);
@@ -395,8 +409,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             if constexpr (MF_LDS) mfp = smfz + z * P::NMFZ;
             else mfp = mfr[p];
             P::products(part, buf + mb * P::NLINE * NZ + z, NZ, mfp, (R)skx[mb], (R)ky, prod[p], racc);
-            if (sidx[mb] < a.nxy) {
-                const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * a.rsf;  // MHD_FullQlin.cpp:314-318
+            if (sidx[mb] >= 0) {
+                const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
                 acc[p][0] += ftfac * racc[0];
                 acc[p][1] += ftfac * racc[1];
             }
@@ -426,14 +440,14 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             const int mb = k / (2 * NZD), r2 = k % (2 * NZD);
             const int pr = r2 / NZD, m = r2 % NZD;
             const int i = sidx[mb];
-            if (i >= a.nxy) continue;
+            if (i < 0) continue;
             const double2 *U = uin + mb * 4 * NZD;
             const double kx = skx[mb], ky = sky[mb], kz = skz[m];
             const double kz2 = -(kz * kz);
             double2 N0, N1;
             P::rhs_pair(pr, buf + mb * P::NOUT * NZ + D::pad(m), NZ, U[m], U[NZD + m], U[2 * NZD + m],
-                        U[3 * NZD + m], kx, ky, kz, silF[mb * NZD + m], a, N0, N1);
-            const double Ln = P::lin(pr, kx + a.qdtl * ky, ky, kz2, a);
+                        U[3 * NZD + m], kx, ky, kz, silF[mb * NZD + m], ph, N0, N1);
+            const double Ln = P::lin(pr, kx + qdtl * ky, ky, kz2, ph);
             const size_t off0 = (size_t)i * 4 * NZD + (2 * pr) * NZD + m, off1 = off0 + NZD;
             if (a.mode == MODE_RHS) {
                 if (a.n_out) {
@@ -450,12 +464,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                 const double e = exp(a.linC * Ln);
                 double2 s0 = cscale(a.nlc1, N0), s1 = cscale(a.nlc1, N1);
                 if (a.n_prev) {
-                    if (!QL_PREFETCH_NPREV) {
-                        npre[w][0] = a.n_prev[off0];
-                        npre[w][1] = a.n_prev[off1];
-                    }
-                    s0 = cadd(s0, cscale(a.nlc2, npre[w][0]));
-                    s1 = cadd(s1, cscale(a.nlc2, npre[w][1]));
+                    s0 = cadd(s0, cscale(a.nlc2, a.n_prev[off0]));
+                    s1 = cadd(s1, cscale(a.nlc2, a.n_prev[off1]));
                 }
                 un0 = cscale(e, cadd(s0, u0));
                 un1 = cscale(e, cadd(s1, u1));
@@ -463,29 +473,25 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                 // RK3CN.cpp:105-107, :141-143, :177-179; the "old" operator is
                 // the one of the previous substep (or linearOPs_Init at t0)
                 double Lo;
-                const double kxo = __dadd_rn(skx0[mb], __dmul_rn(a.qt_old, ky));
-                if (a.old_init) {
+                const double kxo = __dadd_rn(skx0[mb], __dmul_rn(qt_old, ky));
+                if (st.oinit) {
                     const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
-                    Lo = P::lin_init(pr, kxo, lap2, a);
+                    Lo = P::lin_init(pr, kxo, lap2, ph);
                 } else {
-                    Lo = P::lin(pr, kxo + a.qdtl_old * ky, ky, kz2, a);
+                    Lo = P::lin(pr, kxo + qdtl_old * ky, ky, kz2, ph);
                 }
                 const double denom = 1.0 / (1.0 - a.linC * Ln);
                 const double c1 = a.nlc1 * denom, co = (1 + a.linC * Lo) * denom;
                 double2 s0 = cscale(c1, N0), s1 = cscale(c1, N1);
                 if (a.n_prev) {
-                    if (!QL_PREFETCH_NPREV) {
-                        npre[w][0] = a.n_prev[off0];
-                        npre[w][1] = a.n_prev[off1];
-                    }
                     const double c2 = a.nlc2 * denom;
-                    s0 = cadd(s0, cscale(c2, npre[w][0]));
-                    s1 = cadd(s1, cscale(c2, npre[w][1]));
+                    s0 = cadd(s0, cscale(c2, a.n_prev[off0]));
+                    s1 = cadd(s1, cscale(c2, a.n_prev[off1]));
                 }
                 un0 = cadd(s0, cscale(co, u0));
                 un1 = cadd(s1, cscale(co, u1));
             }
-            if (a.do_remap && remap_hit(a, skx0[mb], ky)) un0 = un1 = make_double2(0.0, 0.0);
+            if (do_remap && remap_hit(remap_qt, rp.remap_thr, skx0[mb], ky)) un0 = un1 = make_double2(0.0, 0.0);
             a.u[off0] = un0;
             a.u[off1] = un1;
             if (a.n_out) {
@@ -493,9 +499,10 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                 a.n_out[off1] = N1;
             }
         }
-        if (a.do_remap && tid < MB) {
+        if (do_remap && tid < MB) {
             const int i = sidx[tid];
-            if (i < a.nxy && remap_hit(a, skx0[tid], sky[tid])) a.kx0[i] = __dsub_rn(skx0[tid], a.remap_shift);
+            if (i >= 0 && remap_hit(remap_qt, rp.remap_thr, skx0[tid], sky[tid]))
+                a.kx0[i] = __dsub_rn(skx0[tid], rp.remap_shift);
         }
         __syncthreads();
     }
@@ -522,14 +529,17 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 }
 
 // FP64 path and mixed path (FP32 transforms); each with its own occupancy
-// request (waves per SIMD), which bounds its VGPR budget
+// request, by pass size
+constexpr bool half_pass(int NZ, int MB) { return NZ * MB < 256; }
 template <int NZ, int MB, class P>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QL_WAVES64))) void k_mode64(ModeArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES64_HALF : QL_WAVES64)))
+void k_mode64(ModeArgs a)
 {
     mode_body<NZ, MB, P, double2>(a);
 }
 template <int NZ, int MB, class P>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QL_WAVES32))) void k_mode32(ModeArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES32_HALF : QL_WAVES32)))
+void k_mode32(ModeArgs a)
 {
     mode_body<NZ, MB, P, float2>(a);
 }

@@ -11,7 +11,7 @@ from conftest import ROOT
 def _declared():
     with open(os.path.join(ROOT, "include", "ql_dns.h")) as f:
         src = f.read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ql_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|ql_batch \*)\s*(ql_\w+)\s*\(", src, re.M)))
 
 
 def test_header_matches_binding():
@@ -52,3 +52,46 @@ def test_null_arguments_are_errors():
     assert L.ql_destroy(None) == 0
     assert L.ql_steps(None, 1, None) == -1
     assert L.ql_dims(None, None) == -1
+
+
+def _create_set(params):
+    import ctypes
+    from ql_dns_amd import capi
+    L = capi.load()
+    arr = (capi.ql_params * len(params))(*[capi.to_c_params(p) for p in params])
+    h = ctypes.c_void_p()
+    rc = L.ql_batch_create(arr, len(params), ctypes.byref(h))
+    return rc, h, L
+
+
+@pytest.mark.parametrize("second,code", [
+    (dict(dt=0.02), -1),                 # runs of a set share dt
+    (dict(model=1), -1),                 # ... and model
+    (dict(integrator=0), -1),            # ... and integrator
+    (dict(precision=1), -1),             # ... and precision
+    (dict(Lz=2.0), -1),                  # runs of one NZ share Lz
+    (dict(NZ=96), -1),                   # each run is checked like ql_create
+    (dict(dt=-1.0), -3)])
+def test_set_create_rejects_inconsistent_runs(second, code):
+    from ql_dns_amd.params import QLParams
+    p = QLParams()
+    rc, h, L = _create_set([p, p.replace(**second)])
+    assert rc == code
+    assert h.value is None
+    assert len(L.ql_last_error()) > 0
+
+
+def test_set_null_arguments_are_errors():
+    import ctypes
+    from ql_dns_amd import capi
+    L = capi.load()
+    h = ctypes.c_void_p()
+    assert L.ql_batch_create(None, 1, ctypes.byref(h)) == -1
+    rc, h, L = _create_set([])
+    assert rc == -1
+    assert L.ql_batch_destroy(None) == 0
+    assert L.ql_batch_of(None) is None
+    assert L.ql_batch_steps(None, 1, None) == -1
+    assert L.ql_batch_tune(None, 1, None) == -1
+    n = ctypes.c_int(0)
+    assert L.ql_batch_size(None, ctypes.byref(n)) == -1
