@@ -12,7 +12,12 @@ Prints one JSON line.  Besides the throughput it reports
     bandwidth, from its algorithmic bytes per launch and its average launch
     duration measured with HIP events on the library's stream;
   * cpu_baseline: the CPU oracle (a C restatement of the reference algorithm)
-    timed on this host's cores on a bounded sample of the same workload.
+    timed on this host's cores on a bounded sample of the same workload;
+  * mixed: the FP32-transform path against FP64 at 128x128x256 (speed, and
+    errors in mean fields, EMF / stress drives and energies at steps 100, 500
+    and 2000);
+  * ensemble: 256 uneven runs stepped as one run set against the same runs
+    stepped one after another (aggregate mode-steps/s).
 """
 from __future__ import annotations
 
@@ -64,6 +69,99 @@ def cpu_baseline(p, lin, mf, budget_s: float = 10.0, max_steps: int = 20):
                       f"(oracle/ql_oracle.c, gcc -O3 -march=x86-64-v3, OpenMP mode blocks)"}
 
 
+def bench_ensemble(steps: int = 500, warmup: int = 3, tune_steps: int = 2):
+    """The 256-run parameter-study ensemble: one run set (all runs in the same
+    launches, one replayed graph per 16 steps, launch shapes tuned first) against
+    the same runs stepped one after another, each as its own single-run context.
+    Throughput is aggregate mode-steps/s (modes of every run x steps / time)."""
+    from ql_dns_amd.capi import QLBatch, QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import ensemble
+    runs = ensemble()
+    states = [initial_state(p) for p in runs]
+    mode_steps = sum(p.nxy for p in runs) * steps
+    b = QLBatch(runs)
+    for r, (lin, mf) in enumerate(states):
+        b.set_state(r, lin, mf)
+    t_tune = time.perf_counter()
+    b.tune(tune_steps)
+    t_tune = time.perf_counter() - t_tune
+    b.linear_ops_init(0.0)
+    t = b.steps(warmup, 0.0)
+    t0 = time.perf_counter()
+    b.steps(steps, t)
+    el_b = time.perf_counter() - t0
+    finite = all(abs(b.get_state(r)[1]).max() < 1e300 for r in range(len(runs)))
+    cfg = b.config()
+    b.close()
+    el_s = 0.0
+    for p, (lin, mf) in zip(runs, states):
+        c = QLContext(p)
+        c.set_state(lin, mf)
+        c.linear_ops_init(0.0)
+        t = c.steps(warmup, 0.0)
+        t0 = time.perf_counter()
+        c.steps(steps, t)
+        el_s += time.perf_counter() - t0
+        c.close()
+    return {"runs": len(runs), "modes": sum(p.nxy for p in runs), "steps": steps,
+            "mode_steps_per_s": mode_steps / el_b, "run_steps_per_s": len(runs) * steps / el_b,
+            "sequential_mode_steps_per_s": mode_steps / el_s, "speedup_vs_sequential": el_s / el_b,
+            "seconds": el_b, "sequential_seconds": el_s, "tune_seconds": t_tune,
+            "shapes": cfg, "state_finite": finite,
+            "workload": "256 runs: Rm log-uniform [500, 20000], q uniform [1, 1.8], "
+                        "NZ in {32, 64, 128}, NX, NY in {8, 16, 32}, Re 2000, dt 0.01"}
+
+
+def bench_mixed(name: str = "box128x128x256", marks=(100, 500, 2000)):
+    """FP32-transform path against the FP64 path on the same seeded state:
+    speed (graph replay, steps/s) and, at each mark, the error of the mixed run
+    against the FP64 run (norm-wise relative) in mean fields, EMF / stress
+    drives and fluctuation energy."""
+    import numpy as np
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import PREC_MIXED, workload
+
+    def rel(a, b):
+        d = float(np.abs(b).max())
+        return float(np.abs(a - b).max() / (d if d > 0 else 1.0))
+
+    p = workload(name)
+    lin, mf = initial_state(p)
+    ctx = {}
+    for prec in ("fp64", "mixed"):
+        c = QLContext(p.replace(precision=PREC_MIXED) if prec == "mixed" else p)
+        c.set_state(lin, mf)
+        c.linear_ops_init(0.0)
+        ctx[prec] = c
+    el = {"fp64": 0.0, "mixed": 0.0}
+    t = {"fp64": 0.0, "mixed": 0.0}
+    errs = []
+    done = 0
+    for m in marks:
+        for prec, c in ctx.items():
+            c.synchronize()
+            t0 = time.perf_counter()
+            t[prec] = c.steps(m - done, t[prec])
+            el[prec] += time.perf_counter() - t0
+        done = m
+        l64, m64, _ = ctx["fp64"].get_state()
+        l32, m32, _ = ctx["mixed"].get_state()
+        d64, d32 = ctx["fp64"].drive(), ctx["mixed"].drive()
+        e64, e32 = ctx["fp64"].diagnostics(t["fp64"]), ctx["mixed"].diagnostics(t["mixed"])
+        errs.append({"step": m, "mean_fields": rel(m32, m64), "emf_stress_drive": rel(d32, d64),
+                     "fluct_energy_u": abs(e32[2] - e64[2]) / abs(e64[2]),
+                     "fluct_energy_b": abs(e32[3] - e64[3]) / abs(e64[3]),
+                     "fluct_state": rel(l32, l64)})
+    for c in ctx.values():
+        c.close()
+    n = marks[-1]
+    return {"workload": name, "steps": n, "fp64_steps_per_s": n / el["fp64"],
+            "mixed_steps_per_s": n / el["mixed"], "speedup": el["fp64"] / el["mixed"],
+            "errors_vs_fp64": errs}
+
+
 def copy_calibration(nbytes: int = 1 << 30):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE / WRITE_SIZE) that rocprofv3 reports per dispatch."""
@@ -86,6 +184,8 @@ def main():
     ap.add_argument("--integrator", choices=["RK3CN_Elin", "RK3CN"], default="RK3CN_Elin")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-run ensemble section")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -108,6 +208,7 @@ def main():
     lin, mf = initial_state(p)
     ctx = QLContext(p)
     ctx.set_state(lin, mf)
+    ctx.tune(3)  # launch shape; restores the state
     ctx.linear_ops_init(0.0)
     t = ctx.steps(args.warmup, 0.0)
     ctx.synchronize()
@@ -123,6 +224,7 @@ def main():
     bytes_per_launch = 10.0 * S / 3.0  # u read+write, RHS write (s1), RHS read+write (s2), RHS read (s3)
     achieved = bytes_per_launch / (avg_ms[0] * 1e-3) / 1e9
     traffic = traffic_from_profiles(args.workload)
+    shape = ctx.config()[0]
     lin2, mf2, _ = ctx.get_state()
     finite = bool(abs(lin2).max() < 1e300 and abs(mf2).max() < 1e300)
     ctx.close()
@@ -151,10 +253,15 @@ def main():
         "kernels_ms": {"k_mode_full": avg_ms[0], "k_reduce_cols+k_meanfield": avg_ms[1],
                        "launches_profiled": counts},
         "state_finite": finite,
+        "launch_shape": shape,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, lin, mf)
         out["speedup_vs_cpu"] = steps_per_s / out["cpu_baseline"]["value"]
+    if not args.no_mixed:
+        out["mixed"] = bench_mixed()
+    if not args.no_ensemble:
+        out["ensemble"] = bench_ensemble()
     print(json.dumps(out), flush=True)
 
 

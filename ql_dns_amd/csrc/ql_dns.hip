@@ -101,6 +101,7 @@ struct Group {
     double *lin_tmp = nullptr;
     qlk::RunParams *rp = nullptr;
     int *wg_run = nullptr, *wg_g0 = nullptr, *wg_g1 = nullptr;
+    qlk::Clock *clock = nullptr;  // the group's step times
     std::vector<qlk::RunParams> h_rp;
 };
 
@@ -111,7 +112,6 @@ struct Batch {
     std::vector<Group> groups;
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr;
-    qlk::Clock *clock = nullptr;
     // host view of the clock: start of the previous step, linearOPs_Init time
     double t_prev = 0.0, t_init = 0.0;
     int old_init = 1;
@@ -170,12 +170,13 @@ void release(Batch &b)
         dfree(g.u), dfree(g.N1), dfree(g.N2), dfree(g.mf), dfree(g.mfN1), dfree(g.mfN2), dfree(g.drive);
         dfree(g.tw), dfree(g.tw32), dfree(g.kx0), dfree(g.ky), dfree(g.kzv), dfree(g.mfz), dfree(g.part);
         dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg_run), dfree(g.wg_g0), dfree(g.wg_g1);
+        dfree(g.clock);
         if (g.join) (void)hipEventDestroy(g.join);
         if (g.s) (void)hipStreamDestroy(g.s);
         g.join = nullptr;
         g.s = nullptr;
     }
-    dfree(b.clock), dfree(b.dpart), dfree(b.dout), dfree(b.mfr_tmp);
+    dfree(b.dpart), dfree(b.dout), dfree(b.mfr_tmp);
     for (auto &e : b.ev)
         if (e) (void)hipEventDestroy(e), e = nullptr;
     if (b.fork) (void)hipEventDestroy(b.fork);
@@ -352,7 +353,6 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
     QL_HIP(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
     QL_HIP(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming));
     for (auto &e : b.ev) QL_HIP(hipEventCreate(&e));
-    QL_RC(dalloc(&b.clock, 1));
     QL_RC(dalloc(&b.dpart, (size_t)256 * 6));
     QL_RC(dalloc(&b.dout, (size_t)18));
     QL_RC(dalloc(&b.mfr_tmp, (size_t)4 * 256));
@@ -377,6 +377,7 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_RC(dalloc(&g.mfz, (size_t)g.nruns * 8 * g.NZ));
         QL_RC(dalloc(&g.sums, (size_t)g.nruns * b.nMF * g.NZ));
         QL_RC(dalloc(&g.rp, g.nruns));
+        QL_RC(dalloc(&g.clock, 1));
         std::vector<double> kx0(g.modes), ky(g.modes);
         for (int id : g.run_ids) {
             const RunInfo &ri = b.runs[id];
@@ -453,7 +454,7 @@ qlk::ModeArgs mode_args(const Batch &b, const Group &g)
     a.wg_run = g.wg_run;
     a.wg_g0 = g.wg_g0;
     a.wg_g1 = g.wg_g1;
-    a.clock = b.clock;
+    a.clock = g.clock;
     const double dt = b.dt;
     a.dt = dt;
     a.dtP1 = dt * P1;
@@ -475,9 +476,11 @@ qlk::MFArgs mf_args(const Batch &b, const Group &g)
     m.runs = g.rp;
     m.kzv = g.kzv;
     m.tw = g.tw;
+    m.clock = g.clock;
     m.model = b.model;
     m.nMF = b.nMF;
     m.fac = 1.0 / g.NZ;
+    m.dt = b.dt;
     return m;
 }
 
@@ -518,8 +521,9 @@ int refresh_profiles(Batch &b, Group &g, hipStream_t st)
 }
 
 // One substep s (1..3) of RK3CN_Elin::Step (RK3CN_Elin.cpp:290-378) or
-// RK3CN::Step (RK3CN.cpp:92-193) for one group; times come from the clock.
-int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, hipStream_t st)
+// RK3CN::Step (RK3CN.cpp:92-193) for one group; times come from the group's
+// clock, which the last substep advances when `advance` is set.
+int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, hipStream_t st)
 {
     const bool cn = (b.integrator == QL_INT_RK3CN);
     const double dt = b.dt;
@@ -553,17 +557,18 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, hipStream_t st)
     m.nlc2 = nlc2;
     m.mfn_prev = (s == 1) ? nullptr : (s == 2 ? g.mfN1 : g.mfN2);
     m.mfn_out = (s == 2) ? g.mfN2 : g.mfN1;
+    m.advance = (advance && s == 3) ? 1 : 0;
     return enqueue_sequence(b, g, a, m, st);
 }
 
 // substep s of every group: groups fork from the handle's stream and join it
-int enqueue_substep_all(Batch &b, int s, bool remap_after)
+int enqueue_substep_all(Batch &b, int s, bool remap_after, bool advance)
 {
-    if (b.groups.size() == 1) return enqueue_substep(b, b.groups[0], s, remap_after, b.s);
+    if (b.groups.size() == 1) return enqueue_substep(b, b.groups[0], s, remap_after, advance, b.s);
     QL_HIP(hipEventRecord(b.fork, b.s));
     for (Group &g : b.groups) {
         QL_HIP(hipStreamWaitEvent(g.s, b.fork, 0));
-        QL_RC(enqueue_substep(b, g, s, remap_after, g.s));
+        QL_RC(enqueue_substep(b, g, s, remap_after, advance, g.s));
         QL_HIP(hipEventRecord(g.join, g.s));
     }
     for (Group &g : b.groups) QL_HIP(hipStreamWaitEvent(b.s, g.join, 0));
@@ -573,9 +578,7 @@ int enqueue_substep_all(Batch &b, int s, bool remap_after)
 // one step of main.cpp:108-123 (noise off): Step, t += dt, remap
 int enqueue_step(Batch &b, bool remap)
 {
-    for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, remap && s == 3));
-    qlk::k_clock_advance<<<1, 64, 0, b.s>>>(b.clock, b.dt);
-    QL_HIP(hipGetLastError());
+    for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, remap && s == 3, true));
     return QL_OK;
 }
 
@@ -586,7 +589,7 @@ int set_clock(Batch &b, double t)
     c.t_prev = b.t_prev;
     c.t_init = b.t_init;
     c.old_init = b.old_init;
-    qlk::k_clock_set<<<1, 64, 0, b.s>>>(b.clock, c);
+    for (Group &g : b.groups) qlk::k_clock_set<<<1, 64, 0, b.s>>>(g.clock, c);
     QL_HIP(hipGetLastError());
     return QL_OK;
 }
@@ -805,14 +808,14 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
                 for (auto &sw : seen) dup = dup || (sw.first == g.MB && sw.second == g.W);
                 if (dup) continue;
                 seen.emplace_back(g.MB, g.W);
-                for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, b.s);
+                for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, false, b.s);
                 if (rc) break;
                 if (hipEventRecord(b.ev[0], b.s) != hipSuccess) {
                     rc = fail(QL_ERR_HIP, "tune: event record failed");
                     break;
                 }
                 for (int n = 0; n < trial_steps && rc == QL_OK; ++n)
-                    for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, b.s);
+                    for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, false, b.s);
                 if (rc) break;
                 float ms = 0.f;
                 if (hipEventRecord(b.ev[1], b.s) != hipSuccess || hipEventSynchronize(b.ev[1]) != hipSuccess ||
@@ -972,7 +975,7 @@ int ql_step(ql_ctx *c, double t, double *dt_out)
     if (!c) return fail(QL_ERR_ARG, "ql_step: null context");
     Batch &b = c->bt.b;
     QL_RC(set_clock(b, t));
-    for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, false));
+    for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, false, false));
     b.t_prev = t;
     b.old_init = 0;
     QL_HIP(hipStreamSynchronize(b.s));

@@ -13,8 +13,9 @@
 //                 profiles, x i kz, Hermitian fill (the reference's MPI
 //                 Allreduce + fill, MHD_FullQlin.cpp:386-399), mean-field RHS
 //                 and update, and the real-space profiles the next substep reads.
-// Step times live on the device (Clock) and advance by one tiny kernel per
-// step, so a whole step can be captured once in a HIP graph and replayed.
+// Step times live on the device (one Clock per group), advanced by the
+// group's last mean-field kernel of each step, so a whole step can be captured
+// once in a HIP graph and replayed.
 #pragma once
 #include "ql_fft.hpp"
 
@@ -132,14 +133,12 @@ __global__ void k_clock_set(Clock *c, Clock v)
 }
 
 // t_prev <- t, t <- t + dt (main.cpp:116), once per step
-__global__ void k_clock_advance(Clock *c, double dt)
+__device__ __forceinline__ void clock_advance(Clock *c, double dt)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double t = c->t;
-        c->t_prev = t;
-        c->t = __dadd_rn(t, dt);
-        c->old_init = 0;
-    }
+    const double t = c->t;
+    c->t_prev = t;
+    c->t = __dadd_rn(t, dt);
+    c->old_init = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -203,8 +202,9 @@ struct MFArgs {
     const RunParams *runs;
     const double *kzv;
     const double2 *tw;
-    int model, nMF, mode;
-    double fac, linC, nlc1, nlc2;
+    Clock *clock;  // advanced by dt when advance != 0 (last substep of a step)
+    int model, nMF, mode, advance;
+    double fac, linC, nlc1, nlc2, dt;
 };
 
 template <int NZ>
@@ -221,6 +221,8 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     __shared__ double2 smfn[4 * NZD];   // updated mean fields
     const int tid = threadIdx.x;
     const int run = blockIdx.x;
+    // the substep's mode kernel, the only reader of the clock, has finished
+    if (a.advance && run == 0 && tid == 0) clock_advance(a.clock, a.dt);
     const Phys ph = a.runs[run].ph;
     const int nMF = a.nMF, nslot = nMF * NZD;
     const size_t mo = (size_t)run * nslot;

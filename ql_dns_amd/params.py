@@ -107,6 +107,25 @@ def workload(name: str, **overrides) -> QLParams:
     return WORKLOADS[name].replace(**overrides)
 
 
+def ensemble(n: int = 256, seed: int = 0x5EED0003, **overrides) -> list:
+    """A parameter-study ensemble of n independent runs of the 16x16x32 box's
+    physics (Lx, Ly, Lz = 1, 4, 1, Omega = 1, Re = 2000, dt = 0.01): Rm
+    log-uniform in [500, 20000], q uniform in [1.0, 1.8], NZ from {32, 64, 128},
+    NX and NY each from {8, 16, 32}; run r's initial state uses seed + 1 + r."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    base = WORKLOADS["box16x16x32"].replace(nsteps=500, **overrides)
+    runs = []
+    for r in range(n):
+        Rm = float(np.exp(rng.uniform(np.log(500.0), np.log(20000.0))))
+        q = float(rng.uniform(1.0, 1.8))
+        NZ = int(rng.choice([32, 64, 128]))
+        NX = int(rng.choice([8, 16, 32]))
+        NY = int(rng.choice([8, 16, 32]))
+        runs.append(base.replace(NX=NX, NY=NY, NZ=NZ, q=q, eta=1.0 / Rm, seed=seed + 1 + r))
+    return runs
+
+
 def kmax(p: QLParams) -> float:
     """MHD_FullQlin.cpp:92-97: largest dealiased wavenumber over x, y, z."""
     L = (p.Lx, p.Ly, p.Lz)
