@@ -159,6 +159,26 @@ int ql_batch_get_config(const ql_batch *b, int max_groups, int *ngroups, int *cf
 /* sets the shape of the NZ group: line in {128, 256}, target >= 1 */
 int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
+/* ------------------------------------------------------------------------
+ * Adjoint (reverse-mode gradient) of nsteps of ql_steps, for a single-run
+ * MHD_FullQlin context in FP64 (either integrator, remap on or off).  The
+ * reference has no adjoint; these entry points differentiate the reference's
+ * step (Models/MHD_FullQlin.cpp:171-427, Integrators/RK3CN.cpp:77-201,
+ * Integrators/RK3CN_Elin.cpp:74-188, main.cpp:108-123) as this library
+ * computes it.  Remap decisions are held fixed (piecewise constant in q).
+ *
+ * ql_adjoint_forward: nsteps of ql_steps from the current state, storing a
+ * checkpoint every `every` steps (device memory ~ (nsteps/every + 3 every + 8)
+ * states).  ql_adjoint_backward: given dL/d(final state) (g_lin [nxy][4][nz]
+ * complex, g_mf [nMF][nz] complex, either may be NULL for zero; complex
+ * gradients as dL/dRe + i dL/dIm), returns dL/d(state before the forward) in
+ * g_lin0 / g_mf0 (NULL to skip) and g_par[0..4] = dL/d(q, omega, nu, eta,
+ * B0z).  Segments are recomputed from their checkpoints; the context is left
+ * at the forward's final state. */
+int ql_adjoint_forward(ql_ctx *ctx, int nsteps, int every, double *t_inout);
+int ql_adjoint_backward(ql_ctx *ctx, const double *g_lin, const double *g_mf, double *g_lin0, double *g_mf0,
+                        double g_par[5]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,7 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_create", "ql_batch_destroy", "ql_batch_of", "ql_batch_size", "ql_batch_dims",
            "ql_batch_set_state", "ql_batch_get_state", "ql_batch_diagnostics", "ql_batch_get_drive",
            "ql_batch_linear_ops_init", "ql_batch_steps", "ql_batch_tune", "ql_batch_get_config",
-           "ql_batch_set_config")
+           "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward")
 
 
 class QLError(RuntimeError):
@@ -85,6 +85,8 @@ def load(path: str = LIB_PATH):
         "ql_batch_tune": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_batch_get_config": ([P, ctypes.c_int, I, I], ctypes.c_int),
         "ql_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "ql_adjoint_forward": ([P, ctypes.c_int, ctypes.c_int, D], ctypes.c_int),
+        "ql_adjoint_backward": ([P, D, D, D, D, D], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -230,6 +232,23 @@ class QLContext:
         cnt = (ctypes.c_int * 2)()
         self._check(self._L.ql_profile_kernels(self._h, n, ctypes.byref(tt), _dp(avg), cnt))
         return tt.value, avg, list(cnt)
+
+    # adjoint of `steps` (checkpointed)
+    def adjoint_forward(self, n: int, every: int, t: float = 0.0) -> float:
+        tt = ctypes.c_double(t)
+        self._check(self._L.ql_adjoint_forward(self._h, n, every, ctypes.byref(tt)))
+        return tt.value
+
+    def adjoint_backward(self, g_lin=None, g_mf=None):
+        """dL/d(initial lin, mf) and dL/d(q, omega, nu, eta, B0z) for the given
+        dL/d(final lin, mf) (complex arrays; None = zero)."""
+        g_lin = _checked(g_lin, np.complex128, self.nxy * 4 * self.nz)
+        g_mf = _checked(g_mf, np.complex128, self.nMF * self.nz)
+        gl0 = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        gm0 = np.zeros((self.nMF, self.nz), dtype=np.complex128)
+        gp = np.zeros(5)
+        self._check(self._L.ql_adjoint_backward(self._h, _dp(g_lin), _dp(g_mf), _dp(gl0), _dp(gm0), _dp(gp)))
+        return gl0, gm0, gp
 
     # launch shape (the context's run set of one)
     def tune(self, trial_steps: int = 5) -> float:

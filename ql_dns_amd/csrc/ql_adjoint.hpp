@@ -1,0 +1,637 @@
+// ql_adjoint.hpp -- reverse-mode (adjoint) kernels of one RK3 substep of the
+// MHD_FullQlin quasilinear step in FP64 (Models/MHD_FullQlin.cpp:171-427 with
+// the substep updates of Integrators/RK3CN_Elin.cpp:290-378 and
+// Integrators/RK3CN.cpp:92-193).
+//
+// Gradients of a real loss L with respect to a complex value z are carried as
+// G = dL/dRe z + i dL/dIm z, so dL = Re(conj(G) dz).  For z' = a z the
+// reverse rule is G_z += conj(a) G_z'; for r = Re(a conj b) it is
+// G_a += dL/dr b and G_b += dL/dr a.
+//
+// The transposes of the dealiased transforms are the other transform: the
+// adjoint of "zero-pad nz -> NZ, unnormalised exp(+i) transform" is the
+// unnormalised exp(-i) transform followed by truncation NZ -> nz, and vice
+// versa, so the reverse pass reuses the forward transform code unchanged.
+//
+// One reverse substep (host sequence, ql_adjoint.hip):
+//   k_adj_mf_a    one workgroup: mean-field update and mean-field RHS in
+//                 reverse, drive (EMF / stress) reverse down to dL/d(stress
+//                 z-profiles) by an inverse transform;
+//   k_adj_mode    per mode: recompute the forward lines and products, then
+//                 reverse the update, the RHS assembly, the forward transforms,
+//                 the mean-field x fluctuation products and the EMF / stress
+//                 products, the inverse transforms and the base fields; per
+//                 workgroup partial rows of dL/d(mean-field z-profiles) and of
+//                 the parameter gradients;
+//   k_reduce_cols column sum of those rows;
+//   k_adj_mf_b    one workgroup: z-profile adjoint into dL/dM, parameter
+//                 gradient accumulation.
+// The remap decision (MHD_FullQlin.cpp:470-472) is piecewise constant in q and
+// enters as a fixed mask.
+#pragma once
+#include "ql_mode.hpp"
+
+namespace qlk {
+
+// parameter-gradient slots (columns after the 8 z-profiles of a partial row)
+enum { GP_Q = 0, GP_OMEGA = 1, GP_NU = 2, GP_ETA = 3, GP_B0Z = 4, NGP = 8 };
+
+// d/dx of -|x|^p (neg_abs_pow)
+__device__ __forceinline__ double neg_abs_pow_d(double x, int p)
+{
+    if (p == 0) return 0.0;
+    const double a = fabs(x), sg = x < 0 ? -1.0 : (x > 0 ? 1.0 : 0.0);
+    if (p == 1) return -sg;
+    if (p == 2) return -2.0 * x;
+    return -(double)p * pow(a, (double)(p - 1)) * sg;
+}
+
+struct AdjModeArgs {
+    const double2 *u_in;   // state at the substep start [nxy][4][nz]
+    const double2 *u_out;  // state after the substep (after the remap at s = 3)
+    double2 *gu;           // in: dL/du_out; out: dL/du_in
+    double2 *gnp;          // in (s < 3): dL/dN_s via substep s+1; out (s > 1): dL/dN_{s-1}
+    const double *kx0, *ky, *kzv;
+    const double2 *tw;
+    const double *mfz;     // z-profiles of the substep's mean fields [8][NZ]
+    const double *gsums;   // dL/d(EMF / stress z-profiles) [4][NZ]
+    double *gpart;         // [gridDim.x][8 NZ + NGP]
+    Phys ph;
+    int nxy, s, mode, do_remap, oinit;
+    double rsf, remap_thr, fac;
+    double linC, nlc1, nlc2;
+    double ts, dtl, tso, dtlo, tnew;  // substep times (sub_times of the forward)
+};
+
+// first-pass source of the reverse transforms: compact kz-slot lines
+template <int NZ>
+struct CompactSrc {
+    const double2 *v;  // [line][nz]
+    __device__ double2 operator()(int line, int g) const
+    {
+        using D = Dealias<NZ>;
+        return D::kept(g) ? v[line * D::nz + D::slot(g)] : make_double2(0.0, 0.0);
+    }
+};
+
+template <int NZ, int MB>
+__global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
+{
+    using P = ModelFull;
+    using D = Dealias<NZ>;
+    constexpr int NZD = D::nz;
+    constexpr int NLX = P::NLINE * MB;  // 10 lines per mode
+    constexpr int NLT = P::NOUT * MB;   // 6 lines per mode
+    constexpr int HI = MB * 2 * NZ;     // product items (mode, half, z)
+    constexpr int HW = (HI + NT - 1) / NT;
+    __shared__ double2 bufX[NLX * NZ];  // X lines, then dL/dX lines
+    __shared__ double2 bufT[NLT * NZ];  // P -> T, then dL/dP
+    __shared__ double2 uin[MB * 4 * NZD];
+    __shared__ double2 sgu[MB * 4 * NZD];    // dL/du_out rows, then dL/du_in (direct terms)
+    __shared__ double2 sbase[MB * 6 * NZD];  // bases, then dL/dT (compact)
+    __shared__ double2 tw[NZ];
+    __shared__ double skz[NZD];
+    __shared__ double silF[MB * NZD];
+    __shared__ double skx0[MB], sky[MB], skx[MB];
+    __shared__ int sidx[MB];
+    __shared__ double smfz[NZ * 8];  // [z][profile]
+    __shared__ double sgs[4 * NZ];   // dL/d(stress sums) [slot][z]
+    const int tid = threadIdx.x;
+    const Phys ph = a.ph;
+    const double q = ph.q, Om = ph.omega, B0z = ph.B0z;
+    const double qt = to_sgpr(__dmul_rn(q, a.ts)), qdtl = to_sgpr(__dmul_rn(q, a.dtl));
+    const double qt_old = to_sgpr(__dmul_rn(q, a.tso)), qdtl_old = to_sgpr(__dmul_rn(q, a.dtlo));
+    const double remap_qt = to_sgpr(__dmul_rn(q, a.tnew));
+    const bool remap_now = a.s == 3 && a.do_remap;
+    for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
+    for (int k = tid; k < NZD; k += NT) skz[k] = a.kzv[k];
+    for (int k = tid; k < NZ * 8; k += NT) smfz[(k % NZ) * 8 + k / NZ] = a.mfz[k];
+    for (int k = tid; k < 4 * NZ; k += NT) sgs[k] = a.gsums[k];
+
+    // per-thread gradient accumulators: parameters and z-profile rows
+    double gq = 0, gom = 0, gnu = 0, geta = 0, gb0 = 0;
+    double accm[HW][8];
+#pragma unroll
+    for (int p = 0; p < HW; ++p)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) accm[p][c] = 0.0;
+
+    const int ngroups = (a.nxy + MB - 1) / MB;
+    for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        // A: rows of this group
+        if (tid < MB) {
+            const int i = g * MB + tid;
+            sidx[tid] = i < a.nxy ? i : -1;
+            if (i < a.nxy) {
+                const double kx0 = a.kx0[i], ky = a.ky[i];
+                skx0[tid] = kx0;
+                sky[tid] = ky;
+                skx[tid] = __dadd_rn(kx0, __dmul_rn(qt, ky));
+            } else {
+                skx0[tid] = sky[tid] = skx[tid] = 0.0;
+            }
+        }
+        for (int k = tid; k < MB * 4 * NZD; k += NT) {
+            const int mb = k / (4 * NZD), i = g * MB + mb;
+            const bool ok = i < a.nxy;
+            const size_t off = (size_t)i * 4 * NZD + (k - mb * 4 * NZD);
+            uin[k] = ok ? a.u_in[off] : make_double2(0.0, 0.0);
+            sgu[k] = ok ? a.gu[off] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+
+        // B, C: bases and inverse transforms (forward recomputation)
+        for (int k = tid; k < MB * NZD; k += NT) {
+            const int mb = k / NZD, m = k % NZD;
+            const double kx = skx[mb], ky = sky[mb], kz = skz[m];
+            const bool zero = (ky == 0.0 && m == 0);
+            const double lap2 = zero ? 0.0 : (-ky * ky + -(kz * kz));
+            const double ilap2 = zero ? 1.0 : 1.0 / lap2;
+            silF[k] = (sidx[mb] == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
+            const double2 *U = uin + mb * 4 * NZD;
+            P::bases(U[m], U[NZD + m], U[2 * NZD + m], U[3 * NZD + m], kx, ky, kz, ilap2, a.fac,
+                     sbase + mb * P::NB * NZD + m, NZD);
+        }
+        __syncthreads();
+        fft_lines_from<NZ, NLX, true>(bufX, tw, LineSrc<NZ, P, double2>{sbase, skz});
+
+        // D, E: products and forward transforms (forward recomputation)
+        for (int item = tid; item < HI; item += NT) {
+            const int mb = item / (2 * NZ), rem = item % (2 * NZ);
+            const int part = rem / NZ, z = rem % NZ;
+            double2 prod[3];
+            double racc[2];
+            P::products(part, bufX + mb * P::NLINE * NZ + z, NZ, smfz + z * 8, skx[mb], sky[mb], prod, racc);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bufT[(mb * P::NOUT + part * 3 + c) * NZ + z] = prod[c];
+        }
+        __syncthreads();
+        fft_lines<NZ, NLT, false>(bufT, tw);
+
+        // F: substep update and RHS assembly in reverse, per (mode, kz slot)
+        for (int k = tid; k < MB * NZD; k += NT) {
+            const int mb = k / NZD, m = k % NZD;
+            const int i = sidx[mb];
+            double2 *GT = sbase + mb * 6 * NZD + m;  // dL/dT, compact [line][nz]
+            if (i < 0) {
+                for (int o = 0; o < 6; ++o) GT[o * NZD] = make_double2(0.0, 0.0);
+                continue;
+            }
+            const double kx = skx[mb], ky = sky[mb], kz = skz[m];
+            const double kz2 = -(kz * kz);
+            double2 *Gu = sgu + mb * 4 * NZD + m;
+            const double2 *U = uin + mb * 4 * NZD + m;
+            const double2 ux = U[0], ze = U[NZD], bx = U[2 * NZD], et = U[3 * NZD];
+            double2 gout[4], GN[4], gdir[4];
+            const bool hit = remap_now && remap_hit(remap_qt, a.remap_thr, skx0[mb], ky);
+            for (int f = 0; f < 4; ++f) gout[f] = hit ? make_double2(0.0, 0.0) : Gu[f * NZD];
+            const size_t off = (size_t)i * 4 * NZD + m;
+            for (int pr = 0; pr < 2; ++pr) {
+                const double coef = pr == 0 ? ph.nu : ph.eta;
+                const double kxn = kx + qdtl * ky;
+                const double xn = (-kxn * kxn - ky * ky) + kz2;
+                const double Ln = coef * neg_abs_pow(xn, ph.vo2);
+                double gLn = 0.0, gLo = 0.0, xo = 0.0, dxo_dq = 0.0, kxo_e = 0.0;
+                for (int h = 0; h < 2; ++h) {
+                    const int f = 2 * pr + h;
+                    const double2 go = gout[f];
+                    const double2 gin = (a.s < 3) ? a.gnp[off + f * NZD] : make_double2(0.0, 0.0);
+                    const double2 uo = a.u_out[off + f * NZD];
+                    if (a.mode == MODE_ELIN) {
+                        const double e = exp(a.linC * Ln);
+                        const double2 mu = cscale(e, go);
+                        GN[f] = cadd(cscale(a.nlc1, mu), gin);
+                        if (a.s > 1) a.gnp[off + f * NZD] = cscale(a.nlc2, mu);
+                        gdir[f] = mu;
+                        gLn += a.linC * redot(uo, go);
+                    } else {
+                        double Lo;
+                        const double kxo = __dadd_rn(skx0[mb], __dmul_rn(qt_old, ky));
+                        if (a.oinit) {
+                            const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
+                            xo = -kxo * kxo + lap2;
+                            kxo_e = kxo;
+                            dxo_dq = a.tso * ky;
+                        } else {
+                            kxo_e = kxo + qdtl_old * ky;
+                            xo = (-kxo_e * kxo_e - ky * ky) + kz2;
+                            dxo_dq = (a.tso + a.dtlo) * ky;
+                        }
+                        Lo = coef * neg_abs_pow(xo, ph.vo2);
+                        const double d = 1.0 / (1.0 - a.linC * Ln);
+                        GN[f] = cadd(cscale(a.nlc1 * d, go), gin);
+                        if (a.s > 1) a.gnp[off + f * NZD] = cscale(a.nlc2 * d, go);
+                        gdir[f] = cscale((1 + a.linC * Lo) * d, go);
+                        gLn += a.linC * d * redot(uo, go);
+                        gLo += a.linC * d * redot(U[f * NZD], go);
+                    }
+                }
+                // L = coef * (-|x|^p): coefficient and kx(t) = kx0 + q t ky dependence
+                const double gc = gLn * neg_abs_pow(xn, ph.vo2) + (a.mode == MODE_CN ? gLo * neg_abs_pow(xo, ph.vo2) : 0.0);
+                if (pr == 0) gnu += gc;
+                else geta += gc;
+                gq += gLn * coef * neg_abs_pow_d(xn, ph.vo2) * (-2.0 * kxn) * ((a.ts + a.dtl) * ky);
+                if (a.mode == MODE_CN) gq += gLo * coef * neg_abs_pow_d(xo, ph.vo2) * (-2.0 * kxo_e) * dxo_dq;
+            }
+            // RHS assembly in reverse (MHD_FullQlin.cpp:283-285, :303-305)
+            const double2 *Tm = bufT + mb * P::NOUT * NZ + D::pad(m);
+            const double2 T1 = Tm[0], T2 = Tm[NZ], T3 = Tm[2 * NZ];
+            const double ilapF = silF[k];
+            double gkx = 0.0;
+            // N_ux = ilapF s + i B0z kz bx - T1,  s = 2 kx ky q ux + i 2 Om kz ze - kx^2 T1 - kx ky T2 - kx kz T3
+            double2 s = cscale(2.0 * kx * ky * q, ux);
+            s = cadd(s, imul(2.0 * Om * kz, ze));
+            s = csub(s, cscale(kx * kx, T1));
+            s = csub(s, cscale(kx * ky, T2));
+            s = csub(s, cscale(kx * kz, T3));
+            const double2 Gs = cscale(ilapF, GN[0]);
+            double2 GT0 = csub(cscale(-kx * kx, Gs), GN[0]);
+            double2 GT1 = cscale(-kx * ky, Gs);
+            double2 GT2 = cscale(-kx * kz, Gs);
+            gdir[0] = cadd(gdir[0], cscale(2.0 * kx * ky * q, Gs));
+            gdir[1] = cadd(gdir[1], imul(-2.0 * Om * kz, Gs));
+            gdir[2] = cadd(gdir[2], imul(-B0z * kz, GN[0]));
+            {
+                double2 dsk = cscale(2.0 * ky * q, ux);
+                dsk = csub(dsk, cscale(2.0 * kx, T1));
+                dsk = csub(dsk, cscale(ky, T2));
+                dsk = csub(dsk, cscale(kz, T3));
+                gkx += redot(dsk, Gs);
+            }
+            if (!(i == 0 && m == 0)) gkx += redot(s, GN[0]) * 2.0 * kx * ilapF * ilapF;
+            gq += redot(cscale(2.0 * kx * ky, ux), Gs);
+            gom += redot(imul(2.0 * kz, ze), Gs);
+            gb0 += redot(imul(kz, bx), GN[0]);
+            // N_ze = i (q - 2 Om) kz ux + i B0z kz et - i kz T2 + i ky T3
+            gdir[0] = cadd(gdir[0], imul(-(q - 2.0 * Om) * kz, GN[1]));
+            gdir[3] = cadd(gdir[3], imul(-B0z * kz, GN[1]));
+            GT1 = cadd(GT1, imul(kz, GN[1]));
+            GT2 = cadd(GT2, imul(-ky, GN[1]));
+            gq += redot(imul(kz, ux), GN[1]);
+            gom += redot(imul(-2.0 * kz, ux), GN[1]);
+            gb0 += redot(imul(kz, et), GN[1]);
+            // N_bx = i B0z kz ux - T4
+            gdir[0] = cadd(gdir[0], imul(-B0z * kz, GN[2]));
+            const double2 GT3 = cscale(-1.0, GN[2]);
+            gb0 += redot(imul(kz, ux), GN[2]);
+            // N_et = -i q kz bx + i B0z kz ze - i kz T5 + i ky T6
+            gdir[2] = cadd(gdir[2], imul(q * kz, GN[3]));
+            gdir[1] = cadd(gdir[1], imul(-B0z * kz, GN[3]));
+            const double2 GT4 = imul(kz, GN[3]);
+            const double2 GT5 = imul(-ky, GN[3]);
+            gq += redot(imul(-kz, bx), GN[3]);
+            gb0 += redot(imul(kz, ze), GN[3]);
+            gq += gkx * (a.ts * ky);
+            GT[0] = GT0;
+            GT[NZD] = GT1;
+            GT[2 * NZD] = GT2;
+            GT[3 * NZD] = GT3;
+            GT[4 * NZD] = GT4;
+            GT[5 * NZD] = GT5;
+            for (int f = 0; f < 4; ++f) Gu[f * NZD] = gdir[f];
+        }
+        __syncthreads();
+
+        // G: adjoint of the forward transforms = padded inverse transforms
+        fft_lines_from<NZ, NLT, true>(bufT, tw, CompactSrc<NZ>{sbase});
+
+        // H: products and EMF / stress terms in reverse, per (mode, half, z):
+        // half 0 owns the u lines (ux uy uz dz ux dz uy), half 1 the b lines
+        double2 gX[HW][5];
+#pragma unroll
+        for (int p = 0; p < HW; ++p) {
+            const int item = tid + p * NT;
+            if (HI % NT != 0 && item >= HI) continue;
+            const int mb = item / (2 * NZ), rem = item % (2 * NZ);
+            const int h = rem / NZ, z = rem % NZ;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) gX[p][c] = make_double2(0.0, 0.0);
+            if (sidx[mb] < 0) continue;
+            const double2 *X = bufX + mb * P::NLINE * NZ + z;
+            const double2 u0 = X[0], u1 = X[NZ], u2 = X[2 * NZ], b0 = X[3 * NZ], b1 = X[4 * NZ], b2 = X[5 * NZ];
+            const double2 dzux = X[6 * NZ], dzbx = X[7 * NZ], dzuy = X[8 * NZ], dzby = X[9 * NZ];
+            const double2 *GPp = bufT + mb * P::NOUT * NZ + z;
+            const double2 G0 = GPp[0], G1 = GPp[NZ], G2 = GPp[2 * NZ], G3 = GPp[3 * NZ], G4 = GPp[4 * NZ],
+                          G5 = GPp[5 * NZ];
+            const double *mf = smfz + z * 8;
+            const double MBx = mf[0], dzMBx = mf[1], MBy = mf[2], dzMBy = mf[3];
+            const double MUx = mf[4], dzMUx = mf[5], MUy = mf[6], dzMUy = mf[7];
+            const double kx = skx[mb], ky = sky[mb];
+            const double aU = kx * MUx + ky * MUy, aB = kx * MBx + ky * MBy;
+            const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * a.rsf;
+            const double g0 = ftfac * sgs[z], g1 = ftfac * sgs[NZ + z], g2 = ftfac * sgs[2 * NZ + z],
+                         g3 = ftfac * sgs[3 * NZ + z];
+            double gkx = 0.0;
+            if (h == 0) {
+                // u0 (0), u1 (1), u2 (2), dz ux (3), dz uy (4)
+                double2 Gu0 = cadd(imul(-aU, G0), imul(aB, G3));
+                double2 Gu1 = cadd(imul(-aU, G1), imul(aB, G4));
+                double2 Gu2 = cadd(cadd(cscale(dzMUx, G0), cscale(dzMUy, G1)), imul(-aU, G2));
+                Gu2 = cadd(Gu2, cadd(cadd(cscale(dzMBx, G3), cscale(dzMBy, G4)), imul(aB, G5)));
+                // EMF x, y (slots 0, 1) and stresses (slots 2, 3)
+                Gu0 = cadd(Gu0, cscale(g0, b2));
+                Gu2 = csub(Gu2, cadd(cscale(g0, b0), cscale(g1, b1)));
+                Gu1 = cadd(Gu1, cscale(g1, b2));
+                Gu1 = cadd(Gu1, imul(-g2 * ky, u0));
+                Gu0 = cadd(Gu0, imul(g2 * ky, u1));
+                Gu2 = csub(Gu2, cadd(cscale(g2, dzux), cscale(g3, dzuy)));
+                Gu0 = cadd(Gu0, imul(-g3 * kx, u1));
+                Gu1 = cadd(Gu1, imul(g3 * kx, u0));
+                gX[p][0] = Gu0;
+                gX[p][1] = Gu1;
+                gX[p][2] = Gu2;
+                gX[p][3] = cscale(-g2, u2);
+                gX[p][4] = cscale(-g3, u2);
+                // mean-field profile and kx gradients from the first three products
+                const double gaU = redot(imul(1.0, u0), G0) + redot(imul(1.0, u1), G1) + redot(imul(1.0, u2), G2);
+                const double gaB = redot(imul(-1.0, b0), G0) + redot(imul(-1.0, b1), G1) + redot(imul(-1.0, b2), G2);
+                accm[p][5] += redot(u2, G0);   // dz MUx
+                accm[p][1] -= redot(b2, G0);   // dz MBx
+                accm[p][7] += redot(u2, G1);   // dz MUy
+                accm[p][3] -= redot(b2, G1);   // dz MBy
+                accm[p][4] += kx * gaU;
+                accm[p][6] += ky * gaU;
+                accm[p][0] += kx * gaB;
+                accm[p][2] += ky * gaB;
+                gkx += MUx * gaU + MBx * gaB;
+            } else {
+                // b0 (0), b1 (1), b2 (2), dz bx (3), dz by (4)
+                double2 Gb0 = cadd(imul(aB, G0), imul(-aU, G3));
+                double2 Gb1 = cadd(imul(aB, G1), imul(-aU, G4));
+                double2 Gb2 = cadd(cadd(cscale(-dzMBx, G0), cscale(-dzMBy, G1)), imul(aB, G2));
+                Gb2 = cadd(Gb2, cadd(cadd(cscale(-dzMUx, G3), cscale(-dzMUy, G4)), imul(-aU, G5)));
+                Gb2 = cadd(Gb2, cadd(cscale(g0, u0), cscale(g1, u1)));
+                Gb0 = csub(Gb0, cscale(g0, u2));
+                Gb1 = csub(Gb1, cscale(g1, u2));
+                Gb1 = cadd(Gb1, imul(g2 * ky, b0));
+                Gb0 = cadd(Gb0, imul(-g2 * ky, b1));
+                Gb2 = cadd(Gb2, cadd(cscale(g2, dzbx), cscale(g3, dzby)));
+                Gb0 = cadd(Gb0, imul(g3 * kx, b1));
+                Gb1 = cadd(Gb1, imul(-g3 * kx, b0));
+                gX[p][0] = Gb0;
+                gX[p][1] = Gb1;
+                gX[p][2] = Gb2;
+                gX[p][3] = cscale(g2, b2);
+                gX[p][4] = cscale(g3, b2);
+                const double gaU = redot(imul(1.0, b0), G3) + redot(imul(1.0, b1), G4) + redot(imul(1.0, b2), G5);
+                const double gaB = redot(imul(-1.0, u0), G3) + redot(imul(-1.0, u1), G4) + redot(imul(-1.0, u2), G5);
+                accm[p][1] += redot(u2, G3);   // dz MBx
+                accm[p][5] -= redot(b2, G3);   // dz MUx
+                accm[p][3] += redot(u2, G4);   // dz MBy
+                accm[p][7] -= redot(b2, G4);   // dz MUy
+                accm[p][4] += kx * gaU;
+                accm[p][6] += ky * gaU;
+                accm[p][0] += kx * gaB;
+                accm[p][2] += ky * gaB;
+                gkx += MUx * gaU + MBx * gaB;
+                // d(stress y)/dkx: -Re(u0 conj(i u1)) + Re(b0 conj(i b1))
+                gkx += g3 * (-redot(u0, imul(1.0, u1)) + redot(b0, imul(1.0, b1)));
+            }
+            gq += gkx * (a.ts * ky);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < HW; ++p) {
+            const int item = tid + p * NT;
+            if (HI % NT != 0 && item >= HI) continue;
+            const int mb = item / (2 * NZ), rem = item % (2 * NZ);
+            const int h = rem / NZ, z = rem % NZ;
+            double2 *Y = bufX + mb * P::NLINE * NZ + z;
+            // lines: u0 u1 u2 b0 b1 b2 dzux dzbx dzuy dzby
+            const int l0 = h ? 3 : 0, l3 = h ? 7 : 6, l4 = h ? 9 : 8;
+            Y[l0 * NZ] = gX[p][0];
+            Y[(l0 + 1) * NZ] = gX[p][1];
+            Y[(l0 + 2) * NZ] = gX[p][2];
+            Y[l3 * NZ] = gX[p][3];
+            Y[l4 * NZ] = gX[p][4];
+        }
+        __syncthreads();
+
+        // I: adjoint of the padded inverse transforms = truncated forward transforms
+        fft_lines<NZ, NLX, false>(bufX, tw);
+
+        // J: base fields in reverse; total dL/du_in
+        for (int k = tid; k < MB * NZD; k += NT) {
+            const int mb = k / NZD, m = k % NZD;
+            const int i = sidx[mb];
+            if (i < 0) continue;
+            const double kx = skx[mb], ky = sky[mb], kz = skz[m];
+            const bool zero = (ky == 0.0 && m == 0);
+            const double ilap2 = zero ? 1.0 : 1.0 / (-ky * ky + -(kz * kz));
+            const double2 *GL = bufX + mb * P::NLINE * NZ + D::pad(m);
+            const double2 GB0 = cadd(GL[0], imul(-kz, GL[6 * NZ]));
+            const double2 GB1 = cadd(GL[NZ], imul(-kz, GL[8 * NZ]));
+            const double2 GB2 = GL[2 * NZ];
+            const double2 GB3 = cadd(GL[3 * NZ], imul(-kz, GL[7 * NZ]));
+            const double2 GB4 = cadd(GL[4 * NZ], imul(-kz, GL[9 * NZ]));
+            const double2 GB5 = GL[5 * NZ];
+            const double2 *U = uin + mb * 4 * NZD + m;
+            const double2 ux = U[0], bx = U[2 * NZD];
+            const double f = a.fac;
+            double2 *Gu = sgu + mb * 4 * NZD + m;
+            // uy = (ky kx ux + i kz ze) ilap2, uz = (kx kz ux - i ky ze) ilap2 (and b likewise)
+            const double2 Gux = cscale(f, cadd(GB0, cadd(cscale(ky * kx * ilap2, GB1), cscale(kx * kz * ilap2, GB2))));
+            const double2 Gze = cscale(f, cadd(imul(-kz * ilap2, GB1), imul(ky * ilap2, GB2)));
+            const double2 Gbx = cscale(f, cadd(GB3, cadd(cscale(ky * kx * ilap2, GB4), cscale(kx * kz * ilap2, GB5))));
+            const double2 Get = cscale(f, cadd(imul(-kz * ilap2, GB4), imul(ky * ilap2, GB5)));
+            const double gkx = f * (redot(cscale(ky * ilap2, ux), GB1) + redot(cscale(kz * ilap2, ux), GB2) +
+                                    redot(cscale(ky * ilap2, bx), GB4) + redot(cscale(kz * ilap2, bx), GB5));
+            gq += gkx * (a.ts * ky);
+            const size_t off = (size_t)i * 4 * NZD + m;
+            a.gu[off] = cadd(Gu[0], Gux);
+            a.gu[off + NZD] = cadd(Gu[NZD], Gze);
+            a.gu[off + 2 * NZD] = cadd(Gu[2 * NZD], Gbx);
+            a.gu[off + 3 * NZD] = cadd(Gu[3 * NZD], Get);
+        }
+        __syncthreads();
+    }
+
+    // partial row of this workgroup: z-profile gradients summed over mode slots
+    // and halves in a fixed order, then the parameter gradients
+    double *red = reinterpret_cast<double *>(bufX);
+#pragma unroll
+    for (int p = 0; p < HW; ++p) {
+        const int item = tid + p * NT;
+        if (HI % NT != 0 && item >= HI) continue;
+        const int mbh = item / NZ, z = item % NZ;  // (mode slot, half) pair, z
+#pragma unroll
+        for (int c = 0; c < 8; ++c) red[(mbh * 8 + c) * NZ + z] = accm[p][c];
+    }
+    __syncthreads();
+    double *row = a.gpart + (size_t)blockIdx.x * (8 * NZ + NGP);
+    for (int k = tid; k < 8 * NZ; k += NT) {
+        double s = 0.0;
+        for (int mbh = 0; mbh < 2 * MB; ++mbh) s += red[mbh * 8 * NZ + k];
+        row[k] = s;
+    }
+    __syncthreads();
+    red[0 * NT + tid] = gq;
+    red[1 * NT + tid] = gom;
+    red[2 * NT + tid] = gnu;
+    red[3 * NT + tid] = geta;
+    red[4 * NT + tid] = gb0;
+    __syncthreads();
+    if (tid < NGP) {
+        double s = 0.0;
+        if (tid < 5)
+            for (int k = 0; k < NT; ++k) s += red[tid * NT + k];
+        row[8 * NZ + tid] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mean fields in reverse, part A (one workgroup)
+// ---------------------------------------------------------------------------
+struct AdjMFArgs {
+    double2 *gM;           // in: dL/dM_out; out: dL/dM_in without the z-profile path
+    double2 *gnmp;         // in (s < 3): dL/dNM_s; out (s > 1): dL/dNM_{s-1}
+    const double2 *M_in, *M_out;
+    const double *kzv;
+    const double2 *tw;
+    double *gsums;         // out: dL/d(EMF / stress z-profiles) [4][NZ]
+    double *gpar;          // out: parameter-gradient partials [NGP]
+    Phys ph;
+    int mode, s;
+    double linC, nlc1, nlc2;
+};
+
+template <int NZ>
+__global__ __launch_bounds__(NT) void k_adj_mf_a(AdjMFArgs a)
+{
+    using D = Dealias<NZ>;
+    constexpr int NZD = D::nz, HALF = D::half;
+    __shared__ double2 buf[4 * NZ];
+    __shared__ double2 tw[NZ];
+    __shared__ double skz[NZD];
+    __shared__ double2 sGN[4 * NZD], sdir[4 * NZD];
+    __shared__ double red[5][NT];
+    const int tid = threadIdx.x;
+    const Phys ph = a.ph;
+    for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
+    for (int k = tid; k < NZD; k += NT) skz[k] = a.kzv[k];
+    __syncthreads();
+    double gq = 0, gom = 0, gnu = 0, geta = 0, gb0 = 0;
+    // update in reverse (RK3CN_Elin.cpp:109-112 / RK3CN.cpp:112-115)
+    for (int k = tid; k < 4 * NZD; k += NT) {
+        const int f = k / NZD, m = k % NZD;
+        const double kz2 = -(skz[m] * skz[m]);
+        const double coef = f < 2 ? ph.eta : ph.nu;
+        const double L = ph.quasilinear ? coef * kz2 : 0.0;
+        const double2 g = a.gM[k];
+        const double2 gin = (a.s < 3) ? a.gnmp[k] : make_double2(0.0, 0.0);
+        double2 GN, dir;
+        double gL;
+        if (a.mode == MODE_ELIN) {
+            const double2 mu = cscale(exp(a.linC * L), g);
+            GN = cadd(cscale(a.nlc1, mu), gin);
+            if (a.s > 1) a.gnmp[k] = cscale(a.nlc2, mu);
+            dir = mu;
+            gL = a.linC * redot(a.M_out[k], g);
+        } else {
+            const double d = 1.0 / (1.0 - a.linC * L);
+            GN = cadd(cscale(a.nlc1 * d, g), gin);
+            if (a.s > 1) a.gnmp[k] = cscale(a.nlc2 * d, g);
+            dir = cscale((1 + a.linC * L) * d, g);
+            gL = a.linC * d * (redot(a.M_in[k], g) + redot(a.M_out[k], g));
+        }
+        if (ph.quasilinear) {
+            if (f < 2) geta += gL * kz2;
+            else gnu += gL * kz2;
+        }
+        sGN[k] = ph.quasilinear ? GN : make_double2(0.0, 0.0);
+        sdir[k] = dir;
+    }
+    __syncthreads();
+    // mean-field RHS in reverse (MHD_FullQlin.cpp:411-423)
+    for (int k = tid; k < 4 * NZD; k += NT) {
+        const int f = k / NZD, m = k % NZD;
+        const double kz = skz[m];
+        const double2 N0 = sGN[m], N1 = sGN[NZD + m], N2 = sGN[2 * NZD + m], N3 = sGN[3 * NZD + m];
+        double2 G = sdir[k];
+        if (f == 0) G = cadd(G, cadd(cscale(-ph.q, N1), imul(-ph.B0z * kz, N2)));
+        else if (f == 1) G = cadd(G, imul(-ph.B0z * kz, N3));
+        else if (f == 2) G = cadd(G, cadd(imul(-ph.B0z * kz, N0), cscale(ph.q - 2.0 * ph.omega, N3)));
+        else G = cadd(G, cadd(imul(-ph.B0z * kz, N1), cscale(2.0 * ph.omega, N2)));
+        a.gM[k] = G;
+        if (f == 0) {
+            const double2 M0 = a.M_in[m], M1 = a.M_in[NZD + m], M2 = a.M_in[2 * NZD + m], M3 = a.M_in[3 * NZD + m];
+            gq += redot(cscale(-1.0, M0), N1) + redot(M2, N3);
+            gom += redot(cscale(2.0, M3), N2) + redot(cscale(-2.0, M2), N3);
+            gb0 += redot(imul(kz, M2), N0) + redot(imul(kz, M3), N1) + redot(imul(kz, M0), N2) +
+                   redot(imul(kz, M1), N3);
+        }
+    }
+    // drive in reverse: Hermitian fill and x i kz (MHD_FullQlin.cpp:386-399),
+    // then the forward transform's adjoint (an inverse transform) of the
+    // real stress profiles
+    for (int k = tid; k < 4 * NZ; k += NT) {
+        const int f = k / NZ, n = k % NZ;
+        double2 v = make_double2(0.0, 0.0);
+        if (n < HALF) {
+            const double c = skz[n];
+            const double2 d0 = sGN[f * NZD + n];
+            v = (f < 2) ? imul(-c, d0) : d0;
+            const int mm = 2 * HALF - 1 - n;  // slot filled with conj of this one
+            if (n >= 1 && mm < NZD) {
+                double2 d1 = sGN[f * NZD + mm];
+                d1.y = -d1.y;
+                v = cadd(v, (f < 2) ? imul(-c, d1) : d1);
+            }
+        }
+        buf[k] = v;
+    }
+    __syncthreads();
+    fft_lines<NZ, 4, true>(buf, tw);
+    for (int k = tid; k < 4 * NZ; k += NT) a.gsums[k] = buf[k].x;
+    red[0][tid] = gq;
+    red[1][tid] = gom;
+    red[2][tid] = gnu;
+    red[3][tid] = geta;
+    red[4][tid] = gb0;
+    __syncthreads();
+    if (tid < NGP) {
+        double s = 0.0;
+        if (tid < 5)
+            for (int k = 0; k < NT; ++k) s += red[tid][k];
+        a.gpar[tid] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mean fields in reverse, part B (one workgroup): the z-profiles
+// mfz_l = Re(inverse(pad(c_l M))) with c = fac (l even) or i fac kz (l odd)
+// (MHD_FullQlin.cpp:180-198), and the parameter-gradient sum of the substep
+// ---------------------------------------------------------------------------
+struct AdjMF2Args {
+    double2 *gM;
+    const double *rows;     // reduced partial rows [8 NZ + NGP]
+    const double *gpar_mf;  // [NGP] from k_adj_mf_a
+    double *gpar_total;     // [NGP], accumulated
+    const double *kzv;
+    const double2 *tw;
+    double fac;
+};
+
+template <int NZ>
+__global__ __launch_bounds__(NT) void k_adj_mf_b(AdjMF2Args a)
+{
+    using D = Dealias<NZ>;
+    constexpr int NZD = D::nz;
+    __shared__ double2 buf[8 * NZ];
+    __shared__ double2 tw[NZ];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
+    for (int k = tid; k < 8 * NZ; k += NT) buf[k] = make_double2(a.rows[k], 0.0);
+    __syncthreads();
+    fft_lines<NZ, 8, false>(buf, tw);
+    for (int k = tid; k < 4 * NZD; k += NT) {
+        const int f = k / NZD, m = k % NZD;
+        const double kz = a.kzv[m];
+        const int j = D::pad(m);
+        const double2 G = cadd(cscale(a.fac, buf[(2 * f) * NZ + j]), imul(-a.fac * kz, buf[(2 * f + 1) * NZ + j]));
+        a.gM[k] = cadd(a.gM[k], G);
+    }
+    if (tid < NGP) a.gpar_total[tid] += a.gpar_mf[tid] + a.rows[8 * NZ + tid];
+}
+
+}  // namespace qlk

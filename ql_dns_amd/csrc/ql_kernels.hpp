@@ -127,7 +127,7 @@ __device__ __forceinline__ bool remap_hit(double qt, double thr, double kx0, dou
 }
 
 // the host's view of the clock, passed by value (no host buffer involved)
-__global__ void k_clock_set(Clock *c, Clock v)
+[[maybe_unused]] static __global__ void k_clock_set(Clock *c, Clock v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
 }
@@ -148,7 +148,7 @@ __device__ __forceinline__ void clock_advance(Clock *c, double dt)
 // chunk order, so the result does not depend on scheduling.  No cross-block
 // communication (no fences, no atomics).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, const RunParams *__restrict__ runs,
+[[maybe_unused]] static __global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, const RunParams *__restrict__ runs,
                                                     int ncols, double *__restrict__ sums)
 {
     __shared__ double red[16][17], red2[16][17];
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
 // ---------------------------------------------------------------------------
 // ShearingBox_Remap (MHD_FullQlin.cpp:464-481) over modes [m0, m0 + nxy)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_remap(double2 *u, double *kx0, const double *ky, int m0, int nxy, int nz,
+[[maybe_unused]] static __global__ __launch_bounds__(NT) void k_remap(double2 *u, double *kx0, const double *ky, int m0, int nxy, int nz,
                                               double qt, double thr, double shift)
 {
     for (int i = m0 + blockIdx.x; i < m0 + nxy; i += gridDim.x) {
@@ -366,7 +366,7 @@ struct DiagArgs {
     double *part;  // [gridDim.x][6]
 };
 
-__global__ __launch_bounds__(NT) void k_diag_modes(DiagArgs a)
+[[maybe_unused]] static __global__ __launch_bounds__(NT) void k_diag_modes(DiagArgs a)
 {
     __shared__ double sred[6][NT];
     double s[6] = {0, 0, 0, 0, 0, 0};
@@ -417,7 +417,7 @@ struct DiagFinalArgs {
 };
 
 // MHD_FullQlin.cpp:560-676 (single thread: a few hundred terms)
-__global__ void k_diag_final(DiagFinalArgs a)
+[[maybe_unused]] static __global__ void k_diag_final(DiagFinalArgs a)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double rb[6] = {0, 0, 0, 0, 0, 0};

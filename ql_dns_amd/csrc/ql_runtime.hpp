@@ -1,0 +1,139 @@
+// ql_runtime.hpp -- host-side structures of the QL library shared by its
+// translation units (ql_dns.hip: run sets and the forward step; ql_adjoint.hip:
+// the adjoint).  Not part of the public interface (include/ql_dns.h).
+//
+// Every handle holds a set of runs (a context is a set of one).  Runs are
+// grouped by NZ; a group's runs share one kernel instantiation and are laid
+// out one after another in the group's arrays, each with its own parameters,
+// mean fields and reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ql_dns.h"
+#include "ql_kernels.hpp"
+
+namespace qlh {
+
+// message of the last failure on this thread (ql_last_error)
+inline thread_local std::string g_err;
+int fail(int code, const char *fmt, ...);
+
+#define QL_HIP(call)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) return qlh::fail(QL_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define QL_RC(call)                   \
+    do {                              \
+        int rc_ = (call);             \
+        if (rc_ != QL_OK) return rc_; \
+    } while (0)
+
+// Integrator coefficients (Integrators/RK3CN.cpp:19-20, RK3CN_Elin.cpp:19-20)
+constexpr double A0 = 8.0 / 15.0, A1 = 5.0 / 12.0, A2 = 3.0 / 4.0;
+constexpr double B1 = -17.0 / 60.0, B2 = -5.0 / 12.0;
+constexpr double P1 = 8.0 / 15.0, P2 = 2.0 / 3.0;
+// General_Definitions.h:236: PI = atan(1)*4, which rounds to this double
+constexpr double QL_PI = 3.14159265358979323846;
+
+// steps captured in the long replay graph
+constexpr int GRAPH_STEPS = 16;
+// launch-shape candidates of the tuner: z points per workgroup pass
+// (modes per workgroup MB = line / NZ) and the target workgroup count
+constexpr int LINES[] = {128, 256};
+constexpr int TARGETS[] = {256, 512, 1024, 2048, 4096, 1 << 30};
+constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
+
+inline bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
+inline int mb_for(int NZ, int line) { return NZ >= line ? 1 : line / NZ; }
+
+// one run: parameters, sizes and its place in its group
+struct RunInfo {
+    ql_params p{};
+    int group = 0, gi = 0, mode_off = 0;
+    int nxy = 0, nz = 0, nzl = 0, half = 0;
+    double omega_eff = 0, rsf = 0, remap_thr = 0, remap_shift = 0, totalN2 = 0, kzfac = 0;
+    std::vector<double> h_kx0, h_ky, h_kz;
+};
+
+// runs of one NZ
+struct Group {
+    int NZ = 0, nz = 0, nruns = 0, modes = 0;
+    std::vector<int> run_ids;
+    int line = DEFAULT_LINE, target = DEFAULT_TARGET;
+    int MB = 1, W = 0, part_cap = 0, wg_cap = 0;
+    hipStream_t s = nullptr;  // own stream (sets with several groups)
+    hipEvent_t join = nullptr;
+    double2 *u = nullptr, *N1 = nullptr, *N2 = nullptr;
+    double2 *mf = nullptr, *mfN1 = nullptr, *mfN2 = nullptr, *drive = nullptr, *tw = nullptr;
+    float2 *tw32 = nullptr;
+    double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr, *part = nullptr, *sums = nullptr;
+    double *lin_tmp = nullptr;
+    qlk::RunParams *rp = nullptr;
+    int *wg_run = nullptr, *wg_g0 = nullptr, *wg_g1 = nullptr;
+    qlk::Clock *clock = nullptr;  // the group's step times
+    std::vector<qlk::RunParams> h_rp;
+};
+
+struct AdjState;  // adjoint checkpoints and buffers (ql_adjoint.hip)
+
+struct Batch {
+    int model = 0, integrator = 0, precision = 0, device = 0, nMF = 4;
+    double dt = 0;
+    std::vector<RunInfo> runs;
+    std::vector<Group> groups;
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr;
+    // host view of the clock: start of the previous step, linearOPs_Init time
+    double t_prev = 0.0, t_init = 0.0;
+    int old_init = 1;
+    // diagnostics scratch
+    double *dpart = nullptr, *dout = nullptr, *mfr_tmp = nullptr;
+    // replay graphs of 1 and GRAPH_STEPS steps (null until first use)
+    hipGraphExec_t g1 = nullptr, gK = nullptr;
+    // per-kernel timing (ql_profile_kernels)
+    bool timing = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    double t_ms[2] = {0, 0};
+    int t_cnt[2] = {0, 0};
+    AdjState *adj = nullptr;
+};
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+    QL_HIP(hipMalloc((void **)p, sizeof(T) * (n ? n : 1)));
+    QL_HIP(hipMemset(*p, 0, sizeof(T) * (n ? n : 1)));
+    return QL_OK;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// shared by the translation units (defined in ql_dns.hip)
+int sync_all(Batch &b);
+int set_clock(Batch &b, double t);
+int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, hipStream_t st);
+void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st);
+qlk::MFArgs mf_args(const Batch &b, const Group &g);
+int refresh_profiles(Batch &b, Group &g, hipStream_t st);
+int run_steps(Batch &b, int nsteps, double *t_inout);
+// defined in ql_adjoint.hip
+void adj_release(Batch &b);
+
+}  // namespace qlh
+
+struct ql_batch {
+    qlh::Batch b;
+};
+struct ql_ctx {
+    ql_batch bt;
+};

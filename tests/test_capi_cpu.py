@@ -95,3 +95,5 @@ def test_set_null_arguments_are_errors():
     assert L.ql_batch_tune(None, 1, None) == -1
     n = ctypes.c_int(0)
     assert L.ql_batch_size(None, ctypes.byref(n)) == -1
+    assert L.ql_adjoint_forward(None, 1, 1, None) == -1
+    assert L.ql_adjoint_backward(None, None, None, None, None, None) == -1

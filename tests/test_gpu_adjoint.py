@@ -1,0 +1,173 @@
+"""The hand-written adjoint of the QL step against central finite differences
+of the FP64 GPU forward.
+
+Bars: directional derivatives from the adjoint within 1e-6 (relative) of
+central differences; gradients bitwise independent of the checkpoint spacing
+(the recomputation reproduces the forward's bits); the harness's forward equal
+to plain ql_steps.  Differences use eps = 1e-6 along directions scaled to each
+input's magnitude (state: its rms; q: 1; Rm: Rm), so every input moves by a
+relative 1e-6.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+EPS = 1e-6
+
+
+def _p(**kw):
+    from ql_dns_amd.params import QLParams
+    return QLParams(**kw)
+
+
+def _loss_parts(p, lin, mf, alpha):
+    # L = 0.5/NZ^2 sum_{Bx,By}|M|^2 + alpha sum |u|^2
+    from ql_dns_amd.adjoint import mf_energy
+    L = mf_energy(p, mf) + alpha * float((np.abs(lin) ** 2).sum())
+    g_mf = np.zeros_like(mf)
+    g_mf[:2] = mf[:2] / p.NZ ** 2
+    return L, 2.0 * alpha * lin, g_mf
+
+
+def _run(p, lin, mf, n):
+    from ql_dns_amd.capi import QLContext
+    c = QLContext(p)
+    c.set_state(lin, mf)
+    c.linear_ops_init(0.0)
+    c.steps(n, 0.0)
+    out = c.get_state()
+    c.close()
+    return out[0], out[1]
+
+
+PARAMS = ("q", "omega", "nu", "eta", "B0z")
+
+
+@pytest.mark.parametrize("integrator", [1, 0])
+def test_adjoint_matches_finite_differences(integrator):
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state, mode_tables
+    # remaps happen within these steps (Ly = 1); B0z != 0 exercises its terms
+    p = _p(NX=8, NY=8, NZ=32, Ly=1.0, B0z=0.03, integrator=integrator, nu=1 / 1500, eta=1 / 1200)
+    n, every = 14, 5
+    lin0, mf0 = initial_state(p)
+    alpha = 0.01 / float((np.abs(lin0) ** 2).sum())
+    c = QLContext(p)
+    c.set_state(lin0, mf0)
+    c.linear_ops_init(0.0)
+    c.adjoint_forward(n, every, 0.0)
+    linT, mfT, kxT = c.get_state()
+    assert not np.array_equal(kxT, mode_tables(p)[0]), "no remap happened"
+    _, g_lin, g_mf = _loss_parts(p, linT, mfT, alpha)
+    gl0, gm0, gpar = c.adjoint_backward(g_lin, g_mf)
+    c.close()
+
+    rng = np.random.default_rng(7)
+    rms_l = np.sqrt(np.mean(np.abs(lin0) ** 2))
+    rms_m = np.sqrt(np.mean(np.abs(mf0) ** 2))
+    for d in range(4):
+        vl = (rng.standard_normal(lin0.shape) + 1j * rng.standard_normal(lin0.shape)) * rms_l
+        vm = (rng.standard_normal(mf0.shape) + 1j * rng.standard_normal(mf0.shape)) * rms_m
+        ad = float(np.sum(gl0.real * vl.real + gl0.imag * vl.imag) + np.sum(gm0.real * vm.real + gm0.imag * vm.imag))
+        Lp = _loss_parts(p, *_run(p, lin0 + EPS * vl, mf0 + EPS * vm, n), alpha)[0]
+        Lm = _loss_parts(p, *_run(p, lin0 - EPS * vl, mf0 - EPS * vm, n), alpha)[0]
+        fd = (Lp - Lm) / (2 * EPS)
+        assert abs(ad - fd) <= TOL * abs(fd), f"state direction {d}: adjoint {ad!r} fd {fd!r}"
+    for k, name in enumerate(PARAMS):
+        base = getattr(p, name)
+        h = EPS * abs(base)
+        Lp = _loss_parts(p, *_run(p.replace(**{name: base + h}), lin0, mf0, n), alpha)[0]
+        Lm = _loss_parts(p, *_run(p.replace(**{name: base - h}), lin0, mf0, n), alpha)[0]
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
+
+
+def test_adjoint_independent_of_checkpoint_spacing():
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    p = _p(NX=8, NY=6, NZ=64, Ly=1.0)
+    n = 9
+    lin0, mf0 = initial_state(p)
+    res = []
+    for every in (1, 4, 9, 20):
+        c = QLContext(p)
+        c.set_state(lin0, mf0)
+        c.linear_ops_init(0.0)
+        c.adjoint_forward(n, every, 0.0)
+        linT, mfT, _ = c.get_state()
+        _, g_lin, g_mf = _loss_parts(p, linT, mfT, 1e-9)
+        res.append(c.adjoint_backward(g_lin, g_mf))
+        # the context is left at the forward's final state
+        l2, m2, _ = c.get_state()
+        assert np.array_equal(l2, linT) and np.array_equal(m2, mfT)
+        c.close()
+    for r in res[1:]:
+        for x, y in zip(res[0], r):
+            assert np.array_equal(x, y)
+
+
+def test_adjoint_rejects_unsupported():
+    from ql_dns_amd.capi import QLContext, QLError
+    for kw in (dict(model=1), dict(precision=1)):
+        c = QLContext(_p(NX=4, NY=4, NZ=32, **kw))
+        with pytest.raises(QLError):
+            c.adjoint_forward(2, 1, 0.0)
+        c.close()
+    c = QLContext(_p(NX=4, NY=4, NZ=32))
+    with pytest.raises(QLError):
+        c.adjoint_backward(None, None)  # no forward yet
+    with pytest.raises(QLError):
+        c.adjoint_forward(2, 0, 0.0)
+    c.close()
+
+
+def test_sensitivity_box_harness_matches_finite_differences():
+    """32x32x64, Rm = 4000, 200 steps, checkpoint every 20; loss = final
+    0.5 <Bx^2 + By^2>_z; gradients w.r.t. the initial Bx(z), By(z), q and Rm
+    through the torch.autograd.Function harness, checked on 16 seeded
+    directions."""
+    import torch
+    from ql_dns_amd.adjoint import final_mf_energy, kept_index, mf_energy
+    from ql_dns_amd.ic import initial_state, mf_profile_real
+    from ql_dns_amd.params import workload
+    p = workload("box32x32x64")
+    n, every = 200, 20
+    lin0, mf0 = initial_state(p)
+    prof = mf_profile_real(p, mf0)[:2]  # real profiles whose stored spectrum is mf0
+    bx = torch.tensor(prof[0], dtype=torch.float64, requires_grad=True)
+    by = torch.tensor(prof[1], dtype=torch.float64, requires_grad=True)
+    q = torch.tensor(p.q, dtype=torch.float64, requires_grad=True)
+    Rm = torch.tensor(1.0 / p.eta, dtype=torch.float64, requires_grad=True)
+    L = final_mf_energy(bx, by, q, Rm, p, lin0, n, every)
+    L.backward()
+    # the harness's forward is the plain forward from the same coefficients
+    kept = torch.from_numpy(kept_index(p))
+    spec = torch.fft.fft(torch.stack([bx, by]).detach().to(torch.complex128), dim=-1)[:, kept].numpy()
+    assert np.abs(spec - mf0[:2]).max() < 1e-12 * np.abs(mf0).max()
+    mf_h = np.concatenate([spec, np.zeros((p.nMF - 2, p.nz), dtype=np.complex128)])
+    _, mfT = _run(p, lin0, mf_h, n)
+    assert float(L.detach()) == mf_energy(p, mfT)
+
+    def loss(bx_, by_, q_, Rm_):
+        with torch.no_grad():
+            return float(final_mf_energy(bx_, by_, torch.tensor(q_, dtype=torch.float64),
+                                         torch.tensor(Rm_, dtype=torch.float64), p, lin0, n, every))
+
+    rng = np.random.default_rng(16)
+    scale_prof = float(np.sqrt(np.mean(prof ** 2)))
+    for d in range(16):
+        vx = rng.standard_normal(p.NZ) * scale_prof
+        vy = rng.standard_normal(p.NZ) * scale_prof
+        vq = rng.standard_normal() * 1.0
+        vR = rng.standard_normal() * float(Rm)
+        ad = float(bx.grad @ torch.from_numpy(vx) + by.grad @ torch.from_numpy(vy)) + float(q.grad) * vq + \
+            float(Rm.grad) * vR
+        b0, b1 = bx.detach(), by.detach()
+        Lp = loss(b0 + EPS * torch.from_numpy(vx), b1 + EPS * torch.from_numpy(vy), p.q + EPS * vq,
+                  float(Rm) + EPS * vR)
+        Lm = loss(b0 - EPS * torch.from_numpy(vx), b1 - EPS * torch.from_numpy(vy), p.q - EPS * vq,
+                  float(Rm) - EPS * vR)
+        fd = (Lp - Lm) / (2 * EPS)
+        assert abs(ad - fd) <= TOL * abs(fd), f"direction {d}: adjoint {ad!r} fd {fd!r}"
