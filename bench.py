@@ -17,7 +17,10 @@ Prints one JSON line.  Besides the throughput it reports
     errors in mean fields, EMF / stress drives and energies at steps 100, 500
     and 2000);
   * ensemble: 256 uneven runs stepped as one run set against the same runs
-    stepped one after another (aggregate mode-steps/s).
+    stepped one after another (aggregate mode-steps/s);
+  * adjoint: gradient of the final mean-field energy at 32x32x64 (200 steps,
+    checkpoint every 20) -- forward / backward time and the largest relative
+    difference from central finite differences on 16 directions.
 """
 from __future__ import annotations
 
@@ -162,6 +165,64 @@ def bench_mixed(name: str = "box128x128x256", marks=(100, 500, 2000)):
             "errors_vs_fp64": errs}
 
 
+def bench_adjoint(name: str = "box32x32x64", nsteps: int = 200, every: int = 20, ndir: int = 16,
+                  eps: float = 1e-6):
+    """Gradient of the final 0.5 <Bx^2 + By^2>_z w.r.t. the initial Bx(z),
+    By(z), q and Rm through the adjoint (checkpoint every `every` steps):
+    forward and backward wall time, and the largest relative difference from
+    central finite differences of the FP64 forward on `ndir` seeded
+    directions (each input scaled to its magnitude)."""
+    import numpy as np
+    from ql_dns_amd.adjoint import mf_energy
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import workload
+    p = workload(name)
+    lin0, mf0 = initial_state(p)
+
+    def forward(mf, q, Rm):
+        c = QLContext(p.replace(q=q, eta=1.0 / Rm))
+        c.set_state(lin0, mf)
+        c.linear_ops_init(0.0)
+        c.steps(nsteps, 0.0)
+        L = mf_energy(p, c.get_state()[1])
+        c.close()
+        return L
+
+    Rm0 = 1.0 / p.eta
+    c = QLContext(p)
+    c.set_state(lin0, mf0)
+    c.linear_ops_init(0.0)
+    c.steps(2, 0.0)  # warm-up of the launch path
+    c.set_state(lin0, mf0)
+    c.linear_ops_init(0.0)
+    t0 = time.perf_counter()
+    c.adjoint_forward(nsteps, every, 0.0)
+    t_fwd = time.perf_counter() - t0
+    mfT = c.get_state()[1]
+    g = np.zeros_like(mfT)
+    g[:2] = mfT[:2] / p.NZ ** 2
+    t0 = time.perf_counter()
+    _, gm0, gpar = c.adjoint_backward(None, g)
+    t_bwd = time.perf_counter() - t0
+    c.close()
+    rng = np.random.default_rng(16)
+    sm = float(np.sqrt(np.mean(np.abs(mf0[:2]) ** 2)))
+    worst = 0.0
+    for _ in range(ndir):
+        vm = np.zeros_like(mf0)
+        vm[:2] = (rng.standard_normal((2, p.nz)) + 1j * rng.standard_normal((2, p.nz))) * sm
+        vq, vR = rng.standard_normal(), rng.standard_normal() * Rm0
+        ad = float(np.sum(gm0.real * vm.real + gm0.imag * vm.imag)) + gpar[0] * vq - gpar[3] / Rm0 ** 2 * vR
+        fd = (forward(mf0 + eps * vm, p.q + eps * vq, Rm0 + eps * vR) -
+              forward(mf0 - eps * vm, p.q - eps * vq, Rm0 - eps * vR)) / (2 * eps)
+        worst = max(worst, abs(ad - fd) / abs(fd))
+    return {"workload": name, "steps": nsteps, "checkpoint_every": every,
+            "forward_seconds": t_fwd, "backward_seconds": t_bwd, "backward_over_forward": t_bwd / t_fwd,
+            "grad_q": float(gpar[0]), "grad_Rm": float(-gpar[3] / Rm0 ** 2),
+            "fd_directions": ndir, "fd_eps": eps, "max_rel_err_vs_fd": worst}
+
+
 def copy_calibration(nbytes: int = 1 << 30):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE / WRITE_SIZE) that rocprofv3 reports per dispatch."""
@@ -186,6 +247,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-run ensemble section")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint section")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shape (profiling runs: only steady-state launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -208,7 +272,8 @@ def main():
     lin, mf = initial_state(p)
     ctx = QLContext(p)
     ctx.set_state(lin, mf)
-    ctx.tune(3)  # launch shape; restores the state
+    if not args.no_tune:
+        ctx.tune(3)  # launch shape; restores the state
     ctx.linear_ops_init(0.0)
     t = ctx.steps(args.warmup, 0.0)
     ctx.synchronize()
@@ -262,6 +327,8 @@ def main():
         out["mixed"] = bench_mixed()
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble()
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint()
     print(json.dumps(out), flush=True)
 
 
