@@ -248,6 +248,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-run ensemble section")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint section")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="arithmetic of the main workload (mixed: FP32 transforms; for profiling)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape (profiling runs: only steady-state launches)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -266,9 +268,10 @@ def main():
 
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state
-    from ql_dns_amd.params import INTEGRATOR_NAMES, workload
+    from ql_dns_amd.params import INTEGRATOR_NAMES, PREC_FP64, PREC_MIXED, workload
 
-    p = workload(args.workload, integrator=INTEGRATOR_NAMES[args.integrator])
+    p = workload(args.workload, integrator=INTEGRATOR_NAMES[args.integrator],
+                 precision=PREC_MIXED if args.precision == "mixed" else PREC_FP64)
     lin, mf = initial_state(p)
     ctx = QLContext(p)
     ctx.set_state(lin, mf)
@@ -305,7 +308,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32 transforms, f64 state",
         "data": "synthetic: seeded divergence-free fluctuations and 4-harmonic mean fields",
         "config": {"workload": args.workload, "NX": p.NX, "NY": p.NY, "NZ": p.NZ,
                    "modes": p.nxy, "nz_dealiased": p.nz, "Re": 1.0 / p.nu, "Rm": 1.0 / p.eta,

@@ -287,17 +287,25 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
 // ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
-template <int NZ, int MB>
-void launch_mode_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+template <int NZ, int MB, int MODE>
+void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
     const bool full = (b.model == QL_MODEL_MHD_FULLQLIN);
     if (b.precision == QL_PREC_MIXED) {
-        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull><<<g.W, qlk::NT, 0, st>>>(a);
-        else qlk::k_mode32<NZ, MB, qlk::ModelBQ><<<g.W, qlk::NT, 0, st>>>(a);
+        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE><<<g.W, qlk::NT, 0, st>>>(a);
+        else qlk::k_mode32<NZ, MB, qlk::ModelBQ, MODE><<<g.W, qlk::NT, 0, st>>>(a);
     } else {
-        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull><<<g.W, qlk::NT, 0, st>>>(a);
-        else qlk::k_mode64<NZ, MB, qlk::ModelBQ><<<g.W, qlk::NT, 0, st>>>(a);
+        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE><<<g.W, qlk::NT, 0, st>>>(a);
+        else qlk::k_mode64<NZ, MB, qlk::ModelBQ, MODE><<<g.W, qlk::NT, 0, st>>>(a);
     }
+}
+
+template <int NZ, int MB>
+void launch_mode_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    if (a.mode == qlk::MODE_ELIN) launch_mode_m<NZ, MB, qlk::MODE_ELIN>(b, g, a, st);
+    else if (a.mode == qlk::MODE_CN) launch_mode_m<NZ, MB, qlk::MODE_CN>(b, g, a, st);
+    else launch_mode_m<NZ, MB, qlk::MODE_RHS>(b, g, a, st);
 }
 
 void launch_mode(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)

@@ -96,6 +96,38 @@ __device__ __forceinline__ void dft8(C *v)
     }
 }
 
+// 16-point DFT as two 8-point DFTs (radix-2 split): even outputs from
+// v[k] + v[k+8], odd outputs from (v[k] - v[k+8]) * w16^k.
+template <bool INV, class C>
+__device__ __forceinline__ void dft16(C *v)
+{
+    typedef RealOf<C> R;
+    const R c1 = (R)0.92387953251128675613;  // cos(pi/8)
+    const R s1 = (R)0.38268343236508977173;  // sin(pi/8)
+    const R H = (R)0.70710678118654752440;   // cos(pi/4)
+    const R sg = INV ? (R)1 : (R)-1;         // sign of Im w16
+    C a[8], b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        a[k] = cadd(v[k], v[k + 8]);
+        b[k] = csub(v[k], v[k + 8]);
+    }
+    b[1] = cmul(b[1], Cx<C>::make(c1, sg * s1));
+    b[2] = cmul(b[2], Cx<C>::make(H, sg * H));
+    b[3] = cmul(b[3], Cx<C>::make(s1, sg * c1));
+    b[4] = Cx<C>::make(-sg * b[4].y, sg * b[4].x);
+    b[5] = cmul(b[5], Cx<C>::make(-s1, sg * c1));
+    b[6] = cmul(b[6], Cx<C>::make(-H, sg * H));
+    b[7] = cmul(b[7], Cx<C>::make(-c1, sg * s1));
+    dft8<INV>(a);
+    dft8<INV>(b);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        v[2 * k] = a[k];
+        v[2 * k + 1] = b[k];
+    }
+}
+
 // One Stockham pass of radix R at stride NS over NL lines held in buf.
 template <int NZ, int NL, bool INV, int R, int NS, class C>
 __device__ __forceinline__ void fft_pass(C *buf, const C *tw)
@@ -129,7 +161,9 @@ __device__ __forceinline__ void fft_pass(C *buf, const C *tw)
                     v[it][r] = cmul(v[it][r], w);
                 }
             }
-            if constexpr (R == 8) {
+            if constexpr (R == 16) {
+                dft16<INV>(v[it]);
+            } else if constexpr (R == 8) {
                 dft8<INV>(v[it]);
             } else if constexpr (R == 4) {
                 dft4<INV>(v[it][0], v[it][1], v[it][2], v[it][3]);
@@ -192,6 +226,122 @@ __device__ __forceinline__ void fft_lines_from(C *buf, const C *tw, const Src &s
     }
     __syncthreads();
     fft_lines<NZ, NL, INV, R>(buf, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass transforms (radix 8x4 at NZ = 32, 8x8 at 64, 16x8 at 128, 16x16 at
+// 256).  A pass reads its inputs through ld(line, index) and writes through
+// st(line, index, value); SYNC_MID separates the reads from the writes with a
+// barrier when both address the same buffer.
+// ---------------------------------------------------------------------------
+template <int NZ>
+struct Plan2 {
+    static constexpr int R1 = NZ >= 128 ? 16 : 8;
+    static constexpr int R2 = NZ / R1;
+};
+
+template <int NZ, int NL, bool INV, int R, int NS, bool SYNC_MID, class C, class Ld, class St>
+__device__ __forceinline__ void fft_pass_io(const C *tw, const Ld &ld, const St &st)
+{
+    constexpr int M = NZ / R;
+    constexpr int TOT = NL * M;
+    constexpr int IT = (TOT + NT - 1) / NT;
+    C v[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int item = threadIdx.x + it * NT;
+        if (TOT % NT == 0 || item < TOT) {
+            const int line = item / M, j = item % M;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = ld(line, j + r * M);
+        }
+    }
+    if constexpr (SYNC_MID) __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int item = threadIdx.x + it * NT;
+        if (TOT % NT == 0 || item < TOT) {
+            const int line = item / M, j = item % M;
+            const int k = j % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    C w = tw[k * r * (NZ / (NS * R))];
+                    if (INV) w.y = -w.y;
+                    v[it][r] = cmul(v[it][r], w);
+                }
+            }
+            if constexpr (R == 16) {
+                dft16<INV>(v[it]);
+            } else if constexpr (R == 8) {
+                dft8<INV>(v[it]);
+            } else if constexpr (R == 4) {
+                dft4<INV>(v[it][0], v[it][1], v[it][2], v[it][3]);
+            } else {
+                const C a = v[it][0], b = v[it][1];
+                v[it][0] = cadd(a, b);
+                v[it][1] = csub(a, b);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) st(line, (j - k) * R + k + r * NS, v[it][r]);
+        }
+    }
+    __syncthreads();
+}
+
+template <int NZ, class C>
+struct BufIO {
+    C *buf;
+    __device__ C operator()(int line, int i) const { return buf[line * NZ + i]; }
+    __device__ void operator()(int line, int i, C v) const { buf[line * NZ + i] = v; }
+};
+
+// The intermediate between the two passes.  The first pass writes index
+// j*R1 + s from lane j, a stride of R1 complex values across lanes, which
+// would put every lane of a group on the same LDS banks; one padding slot per
+// R1 values (index i -> i + i/R1, lines of LP slots) spreads them over all
+// banks, and the second pass still reads consecutive slots across lanes.
+template <int NZ, class C>
+struct PadIO {
+    static constexpr int R1 = Plan2<NZ>::R1;
+    static constexpr int LP = NZ + NZ / R1;
+    C *buf;
+    __device__ C operator()(int line, int i) const { return buf[line * LP + i + i / R1]; }
+    __device__ void operator()(int line, int i, C v) const { buf[line * LP + i + i / R1] = v; }
+};
+
+// slots of buf used by a two-pass transform of NL lines
+template <int NZ, int NL>
+struct Plan2Size {
+    static constexpr int LP = NZ + NZ / Plan2<NZ>::R1;
+    static constexpr int SLOTS = NL * (LP > NZ ? LP : NZ);
+};
+
+// Inverse (exp(+i)) transform of NL lines whose first pass reads src(line, g),
+// result in buf (lines of NZ).  The caller must have synchronised before buf
+// may be overwritten and before src may be read.  Three barriers.
+template <int NZ, int NL, class C, class Src>
+__device__ __forceinline__ void fft2_inv_from(C *buf, const C *tw, const Src &src)
+{
+    using PL = Plan2<NZ>;
+    const BufIO<NZ, C> io{buf};
+    const PadIO<NZ, C> pad{buf};
+    fft_pass_io<NZ, NL, true, PL::R1, 1, false>(tw, src, pad);
+    fft_pass_io<NZ, NL, true, PL::R2, PL::R1, true>(tw, pad, io);
+}
+
+// Forward (exp(-i)) transform of NL lines held in buf (lines of NZ); the last
+// pass writes through dst(line, index, value), which must not alias the first
+// Plan2Size<NZ, NL>::SLOTS slots of buf.  The caller must have synchronised
+// after filling buf.  Three barriers.
+template <int NZ, int NL, class C, class Dst>
+__device__ __forceinline__ void fft2_fwd_to(C *buf, const C *tw, const Dst &dst)
+{
+    using PL = Plan2<NZ>;
+    const BufIO<NZ, C> io{buf};
+    const PadIO<NZ, C> pad{buf};
+    fft_pass_io<NZ, NL, false, PL::R1, 1, true>(tw, io, pad);
+    fft_pass_io<NZ, NL, false, PL::R2, PL::R1, false>(tw, pad, dst);
 }
 
 // dealiased sizes fixed by NZ (Models/Model.h:34: nz = 1 + 2*int(NZ/3))

@@ -268,34 +268,69 @@ struct LineSrc {
     }
 };
 
+// LDS layout of a mode group: line l of mode slot mb at line l*MB + mb, so the
+// product lines (l < NOUT) are the first NOUT*MB lines; the transformed
+// products are stored compactly ([line][kz slot]) behind the slots their
+// transform uses, in the space of the remaining input lines.
+template <int NZ, int MB, class P>
+struct GroupLayout {
+    static constexpr int NZD = Dealias<NZ>::nz;
+    static constexpr int NL = P::NLINE * MB;
+    static constexpr int NLO = P::NOUT * MB;
+    static constexpr int CT_OFF = Plan2Size<NZ, NLO>::SLOTS;
+    static constexpr int CT = NLO * NZD;
+    static constexpr int A = Plan2Size<NZ, NL>::SLOTS, B = CT_OFF + CT;
+    static constexpr int BUF = A > B ? A : B;
+};
+
+// first-pass source of the inverse transforms in the interleaved layout
 template <int NZ, int MB, class P, class C>
+struct LineSrcI {
+    const C *sbase;  // [(base * MB + mb)][nz]
+    const RealOf<C> *skz;
+    __device__ C operator()(int line, int g) const
+    {
+        using D = Dealias<NZ>;
+        if (!D::kept(g)) return Cx<C>::make(0, 0);
+        const int m = D::slot(g);
+        const int l = line / MB, mb = line - l * MB;
+        const C v = sbase[(P::base_of(l) * MB + mb) * D::nz + m];
+        return P::dz_of(l) ? imul(skz[m], v) : v;
+    }
+};
+
+// last-pass sink of the forward transforms: keep the dealiased band, compact
+template <int NZ, class C>
+struct CompactDst {
+    C *ct;  // [line][nz]
+    __device__ void operator()(int line, int j, C v) const
+    {
+        using D = Dealias<NZ>;
+        if (D::kept(j)) ct[line * D::nz + D::slot(j)] = v;
+    }
+};
+
+template <int NZ, int MB, class P, class C, int MODE>
 __device__ __forceinline__ void mode_body(const ModeArgs &a)
 {
     typedef RealOf<C> R;
     using D = Dealias<NZ>;
+    using L = GroupLayout<NZ, MB, P>;
     constexpr int NZD = D::nz;
-    constexpr int NL = P::NLINE * MB;
-    constexpr int PI_ = MB * 2 * NZ;  // pointwise items: (mode slot, part, z)
-    constexpr int PW = (PI_ + NT - 1) / NT;
-    constexpr int NU = MB * 4 * NZD;  // state values of a group
-    constexpr int UW = (NU + NT - 1) / NT;
-    constexpr int AI = MB * 2 * NZD;  // update items: (mode slot, field pair, kz slot)
-    constexpr int AW = (AI + NT - 1) / NT;
-    __shared__ C buf[NL * NZ];
-    __shared__ double2 uin[NU];
+    constexpr int NIT = MB * NZD;  // (mode slot, kz slot) items: bases, RHS, update
+    constexpr int NZI = MB * NZ;   // (mode slot, z) items: products
+    static_assert(NIT <= NT && NZI <= NT, "one item per thread");
+    __shared__ C buf[L::BUF];
     __shared__ C sbase[MB * P::NB * NZD];
     __shared__ C tw[NZ];
-    __shared__ double skz[NZD];
     __shared__ R skzr[NZD];
-    __shared__ double silF[MB * NZD];
-    __shared__ double skx0[MB], sky[MB], skx[MB];
-    __shared__ int sidx[MB];   // mode index in the group arrays (-1: empty slot)
-    __shared__ int sloc[MB];   // mode index within its run
-    // mean-field profiles as [z][profile]: in LDS up to NZ = 128, in
-    // registers (the thread's fixed z points) at NZ = 256 where LDS is short
-    constexpr bool MF_LDS = (NZ <= 128) || sizeof(C) == 8;
-    __shared__ R smfz[MF_LDS ? NZ * P::NMFZ : 1];
-    R mfr[MF_LDS ? 1 : PW][P::NMFZ];
+    __shared__ double skx[MB], sky[MB];
+    __shared__ int sok[MB];
+    // mean-field z-profiles [profile][z]: in registers of the product threads
+    // for the FP32 path and at NZ = 256 (where LDS is short), in LDS otherwise
+    // (FP64 registers are the scarcer resource)
+    constexpr bool MF_REG = sizeof(C) == 8 || NZ >= 256;
+    __shared__ R smfz[MF_REG ? 1 : NZ * P::NMFZ];
     const int tid = threadIdx.x;
 
     // this workgroup's run, its mode groups [g0, g1), the substep times
@@ -309,216 +344,190 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     const double qt_old = to_sgpr(__dmul_rn(ph.q, st.tso)), qdtl_old = to_sgpr(__dmul_rn(ph.q, st.dtlo));
     const double remap_qt = to_sgpr(__dmul_rn(ph.q, st.tnew));
     const bool do_remap = a.do_remap && rp.remap;
-    const double *mfz = a.mfz + (size_t)run * 8 * NZ;
 
     const C *twg = reinterpret_cast<const C *>(sizeof(C) == 8 ? a.tw32 : (const void *)a.tw);
     for (int k = tid; k < NZ; k += NT) tw[k] = twg[k];
-    for (int k = tid; k < NZD; k += NT) {
-        skz[k] = a.kzv[k];
-        skzr[k] = (R)a.kzv[k];
-    }
-    if constexpr (MF_LDS) {
-        for (int k = tid; k < NZ * P::NMFZ; k += NT) smfz[(k % NZ) * P::NMFZ + k / NZ] = (R)mfz[k];
-    } else {
+    for (int k = tid; k < NZD; k += NT) skzr[k] = (R)a.kzv[k];
+
+    // the thread's items: (imb, im) for the bases and the update, (zmb, zz)
+    // for the products; both fixed for the whole kernel
+    const bool has_it = tid < NIT, has_z = tid < NZI;
+    const int imb = tid / NZD, im = tid - imb * NZD;
+    const int zmb = tid / NZ, zz = tid - zmb * NZ;
+    const double kz = has_it ? a.kzv[im] : 0.0, kz2 = -(kz * kz);
+    R mfr[MF_REG ? P::NMFZ : 1];
+    {
+        const double *mfz = a.mfz + (size_t)run * 8 * NZ;
+        if constexpr (MF_REG) {
 #pragma unroll
-        for (int p = 0; p < PW; ++p) {
-            const int item = tid + p * NT;
-#pragma unroll
-            for (int c = 0; c < P::NMFZ; ++c) mfr[p][c] = (item < PI_) ? (R)mfz[c * NZ + item % NZ] : (R)0;
+            for (int c = 0; c < P::NMFZ; ++c) mfr[c] = has_z ? (R)mfz[c * NZ + zz] : (R)0;
+        } else {
+            for (int k = tid; k < NZ * P::NMFZ; k += NT) smfz[k] = (R)mfz[k];
         }
     }
-
-    double acc[PW][2];
+    double acc[P::NACC];
 #pragma unroll
-    for (int p = 0; p < PW; ++p) acc[p][0] = acc[p][1] = 0.0;
+    for (int c = 0; c < P::NACC; ++c) acc[c] = 0.0;
 
-    // run-local mode index of slot mb of group g, or -1
-    auto local_mode = [&](int g, int mb) { const int l = g * MB + mb; return l < rp.nxy ? l : -1; };
-    double2 upre[UW];
-    auto fetch_rows = [&](int g) {
-#pragma unroll
-        for (int w = 0; w < UW; ++w) {
-            const int k = tid + w * NT;
-            const int mb = k / (4 * NZD);
-            const int l = (k < NU && g < g1) ? local_mode(g, mb) : -1;
-            upre[w] = (l >= 0) ? a.u[(size_t)(rp.mode_off + l) * 4 * NZD + (k - mb * 4 * NZD)] : make_double2(0.0, 0.0);
-        }
+    // run-local mode index of this thread's item in group g, or -1
+    auto local_mode = [&](int g) {
+        const int l = g * MB + imb;
+        return (has_it && g < g1 && l < rp.nxy) ? l : -1;
     };
-    fetch_rows(g0);
+    double2 upre[4];
+    auto fetch = [&](int g) {
+        const int l = local_mode(g);
+        const size_t off = (size_t)(rp.mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) upre[f] = (l >= 0) ? a.u[off + f * NZD] : make_double2(0.0, 0.0);
+    };
+    fetch(g0);
 
     for (int g = g0; g < g1; ++g) {
-        // A: this group's rows to LDS, the next group's rows to registers
-        if (tid < MB) {
-            const int l = local_mode(g, tid);
-            sloc[tid] = l;
-            sidx[tid] = l >= 0 ? rp.mode_off + l : -1;
-            if (l >= 0) {
-                const double kx0 = a.kx0[rp.mode_off + l], ky = a.ky[rp.mode_off + l];
-                skx0[tid] = kx0;
-                sky[tid] = ky;
-                skx[tid] = __dadd_rn(kx0, __dmul_rn(qt, ky));  // kx(t), MHD_FullQlin.cpp:727
-            } else {
-                skx0[tid] = sky[tid] = skx[tid] = 0.0;
-            }
-        }
-        if (QL_PREFETCH_ROWS || g == g0) {
-#pragma unroll
-            for (int w = 0; w < UW; ++w) {
-                const int k = tid + w * NT;
-                if (k < NU) uin[k] = upre[w];
-            }
-        } else {
-            for (int k = tid; k < NU; k += NT) {
-                const int mb = k / (4 * NZD);
-                const int l = local_mode(g, mb);
-                uin[k] = (l >= 0) ? a.u[(size_t)(rp.mode_off + l) * 4 * NZD + (k - mb * 4 * NZD)] : make_double2(0.0, 0.0);
-            }
-        }
-        if (QL_PREFETCH_ROWS) fetch_rows(g + 1);
-        __syncthreads();
-
-        // B: per-slot inverse Laplacians (assign_laplacians_, MHD_FullQlin.cpp:722-738,
+        // B: state of the item (prefetched), next group's state, previous RHS;
+        // kx(t), 1/lap2, 1/lapF (assign_laplacians_, MHD_FullQlin.cpp:722-738,
         // Kdata.cpp:100-120) and the base spectral fields
-        for (int k = tid; k < MB * NZD; k += NT) {
-            const int mb = k / NZD, m = k % NZD;
-            const double kx = skx[mb], ky = sky[mb], kz = skz[m];
-            const bool zero = (ky == 0.0 && m == 0);
-            const double lap2 = zero ? 0.0 : (-ky * ky + -(kz * kz));
+        const int l = local_mode(g);
+        const double2 U0 = upre[0], U1 = upre[1], U2 = upre[2], U3 = upre[3];
+        if (QL_PREFETCH_ROWS) fetch(g + 1);
+        const size_t off = (size_t)(rp.mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
+        double kx0 = 0.0, ky = 0.0, kx = 0.0, ilapF = 1.0;
+        if (has_it) {
+            if (l >= 0) {
+                kx0 = a.kx0[rp.mode_off + l];
+                ky = a.ky[rp.mode_off + l];
+                kx = __dadd_rn(kx0, __dmul_rn(qt, ky));  // kx(t), MHD_FullQlin.cpp:727
+            }
+            const bool zero = (ky == 0.0 && im == 0);
+            const double lap2 = zero ? 0.0 : (-ky * ky + kz2);
             const double ilap2 = zero ? 1.0 : 1.0 / lap2;
-            silF[k] = (sloc[mb] == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
-            const double2 *U = uin + mb * 4 * NZD;
-            P::bases(U[m], U[NZD + m], U[2 * NZD + m], U[3 * NZD + m], kx, ky, kz, ilap2, a.fac,
-                     sbase + mb * P::NB * NZD + m, NZD);
-        }
-        __syncthreads();
-
-        // C: inverse transforms
-        fft_lines_from<NZ, NL, true>(buf, tw, LineSrc<NZ, P, C>{sbase, skzr});
-
-        // D: products on the z grid and EMF / stress sums
-        C prod[PW][3];
-#pragma unroll
-        for (int p = 0; p < PW; ++p) {
-            const int item = tid + p * NT;
-            if (PI_ % NT != 0 && item >= PI_) continue;
-            const int mb = item / (2 * NZ), rem = item % (2 * NZ);
-            const int part = rem / NZ, z = rem % NZ;
-            const double ky = sky[mb];
-            double racc[2];
-            const R *mfp;
-            if constexpr (MF_LDS) mfp = smfz + z * P::NMFZ;
-            else mfp = mfr[p];
-            P::products(part, buf + mb * P::NLINE * NZ + z, NZ, mfp, (R)skx[mb], (R)ky, prod[p], racc);
-            if (sidx[mb] >= 0) {
-                const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
-                acc[p][0] += ftfac * racc[0];
-                acc[p][1] += ftfac * racc[1];
+            if (l >= 0) ilapF = (l == 0 && im == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
+            P::bases(U0, U1, U2, U3, kx, ky, kz, ilap2, a.fac, sbase + imb * NZD + im, MB * NZD);
+            if (im == 0) {
+                skx[imb] = kx;
+                sky[imb] = ky;
+                sok[imb] = l >= 0;
             }
         }
         __syncthreads();
+
+        // C: inverse transforms of the padded lines
+        fft2_inv_from<NZ, L::NL>(buf, tw, LineSrcI<NZ, MB, P, C>{sbase, skzr});
+
+        // D: products on the z grid, in place, and the EMF / stress sums
+        if (has_z) {
+            C *X = buf + zmb * NZ + zz;  // line l at X[l * MB * NZ]
+            const R kxr = (R)skx[zmb], kyr = (R)sky[zmb];
+            R mfl[P::NMFZ];
+            if constexpr (!MF_REG) {
 #pragma unroll
-        for (int p = 0; p < PW; ++p) {
-            const int item = tid + p * NT;
-            if (PI_ % NT != 0 && item >= PI_) continue;
-            const int mb = item / (2 * NZ), rem = item % (2 * NZ);
-            const int part = rem / NZ, z = rem % NZ;
-            C *Y = buf + mb * P::NOUT * NZ + z;
+                for (int c = 0; c < P::NMFZ; ++c) mfl[c] = smfz[c * NZ + zz];
+            }
+            const R *mfp = MF_REG ? mfr : mfl;
+            C prod[2][3];
+            double racc[2][2];
+            P::products(0, X, MB * NZ, mfp, kxr, kyr, prod[0], racc[0]);
+            P::products(1, X, MB * NZ, mfp, kxr, kyr, prod[1], racc[1]);
+            if (sok[zmb]) {
+                const double ftfac = (sky[zmb] == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (c < P::nprod(part)) Y[P::out_line(part, c) * NZ] = prod[p][c];
+                for (int part = 0; part < 2; ++part) {
+                    acc[P::acc_slot(part, 0)] += ftfac * racc[part][0];
+                    if (P::NACC == 4) acc[P::acc_slot(part, 1)] += ftfac * racc[part][1];
+                }
+            }
+#pragma unroll
+            for (int part = 0; part < 2; ++part)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (c < P::nprod(part)) X[P::out_line(part, c) * MB * NZ] = prod[part][c];
         }
         __syncthreads();
-
-        // E: forward transforms
-        fft_lines<NZ, P::NOUT * MB, false>(buf, tw);
-
-        // F: RHS, dissipation operator, update
+        // previous substep's RHS of the item, needed in F
+        double2 np[4];
 #pragma unroll
-        for (int w = 0; w < AW; ++w) {
-            const int k = tid + w * NT;
-            if (AI % NT != 0 && k >= AI) continue;
-            const int mb = k / (2 * NZD), r2 = k % (2 * NZD);
-            const int pr = r2 / NZD, m = r2 % NZD;
-            const int i = sidx[mb];
-            if (i < 0) continue;
-            const double2 *U = uin + mb * 4 * NZD;
-            const double kx = skx[mb], ky = sky[mb], kz = skz[m];
-            const double kz2 = -(kz * kz);
-            double2 N0, N1;
-            P::rhs_pair(pr, buf + mb * P::NOUT * NZ + D::pad(m), NZ, U[m], U[NZD + m], U[2 * NZD + m],
-                        U[3 * NZD + m], kx, ky, kz, silF[mb * NZD + m], ph, N0, N1);
-            const double Ln = P::lin(pr, kx + qdtl * ky, ky, kz2, ph);
-            const size_t off0 = (size_t)i * 4 * NZD + (2 * pr) * NZD + m, off1 = off0 + NZD;
-            if (a.mode == MODE_RHS) {
+        for (int f = 0; f < 4; ++f)
+            np[f] = (l >= 0 && a.n_prev) ? a.n_prev[off + f * NZD] : make_double2(0.0, 0.0);
+
+        // E: forward transforms of the product lines, dealiased band kept
+        C *ct = buf + L::CT_OFF;  // [line][nz]
+        fft2_fwd_to<NZ, L::NLO>(buf, tw, CompactDst<NZ, C>{ct});
+
+        // F: RHS, dissipation operator at the substep end, update
+        if (l >= 0) {
+            const C *T = ct + imb * NZD + im;
+            bool hit = false;
+            if (do_remap) hit = remap_hit(remap_qt, rp.remap_thr, kx0, ky);
+#pragma unroll
+            for (int pr = 0; pr < 2; ++pr) {
+                double2 N0, N1;
+                P::rhs_pair(pr, T, MB * NZD, U0, U1, U2, U3, kx, ky, kz, ilapF, ph, N0, N1);
+                const double Ln = P::lin(pr, kx + qdtl * ky, ky, kz2, ph);
+                const size_t off0 = off + (2 * pr) * NZD, off1 = off0 + NZD;
+                if constexpr (MODE == MODE_RHS) {
+                    if (a.n_out) {
+                        a.n_out[off0] = N0;
+                        a.n_out[off1] = N1;
+                    }
+                    if (a.lin_out) a.lin_out[off0] = a.lin_out[off1] = Ln;
+                    continue;
+                }
+                const double2 u0 = pr ? U2 : U0, u1 = pr ? U3 : U1;
+                const double2 p0 = np[2 * pr], p1 = np[2 * pr + 1];
+                double2 un0, un1;
+                if constexpr (MODE == MODE_ELIN) {
+                    // RK3CN_Elin.cpp:303-305, :335-337, :368-370
+                    const double e = exp(a.linC * Ln);
+                    double2 s0 = cscale(a.nlc1, N0), s1 = cscale(a.nlc1, N1);
+                    if (a.n_prev) {
+                        s0 = cadd(s0, cscale(a.nlc2, p0));
+                        s1 = cadd(s1, cscale(a.nlc2, p1));
+                    }
+                    un0 = cscale(e, cadd(s0, u0));
+                    un1 = cscale(e, cadd(s1, u1));
+                } else {
+                    // RK3CN.cpp:105-107, :141-143, :177-179; the "old" operator is
+                    // the one of the previous substep (or linearOPs_Init at t0)
+                    double Lo;
+                    const double kxo = __dadd_rn(kx0, __dmul_rn(qt_old, ky));
+                    if (st.oinit) {
+                        const double lap2 = (ky == 0.0 && im == 0) ? 0.0 : (-ky * ky + kz2);
+                        Lo = P::lin_init(pr, kxo, lap2, ph);
+                    } else {
+                        Lo = P::lin(pr, kxo + qdtl_old * ky, ky, kz2, ph);
+                    }
+                    const double denom = 1.0 / (1.0 - a.linC * Ln);
+                    const double c1 = a.nlc1 * denom, co = (1 + a.linC * Lo) * denom;
+                    double2 s0 = cscale(c1, N0), s1 = cscale(c1, N1);
+                    if (a.n_prev) {
+                        const double c2 = a.nlc2 * denom;
+                        s0 = cadd(s0, cscale(c2, p0));
+                        s1 = cadd(s1, cscale(c2, p1));
+                    }
+                    un0 = cadd(s0, cscale(co, u0));
+                    un1 = cadd(s1, cscale(co, u1));
+                }
+                if (hit) un0 = un1 = make_double2(0.0, 0.0);
+                a.u[off0] = un0;
+                a.u[off1] = un1;
                 if (a.n_out) {
                     a.n_out[off0] = N0;
                     a.n_out[off1] = N1;
                 }
-                if (a.lin_out) a.lin_out[off0] = a.lin_out[off1] = Ln;
-                continue;
             }
-            const double2 u0 = U[2 * pr * NZD + m], u1 = U[(2 * pr + 1) * NZD + m];
-            double2 un0, un1;
-            if (a.mode == MODE_ELIN) {
-                // RK3CN_Elin.cpp:303-305, :335-337, :368-370
-                const double e = exp(a.linC * Ln);
-                double2 s0 = cscale(a.nlc1, N0), s1 = cscale(a.nlc1, N1);
-                if (a.n_prev) {
-                    s0 = cadd(s0, cscale(a.nlc2, a.n_prev[off0]));
-                    s1 = cadd(s1, cscale(a.nlc2, a.n_prev[off1]));
-                }
-                un0 = cscale(e, cadd(s0, u0));
-                un1 = cscale(e, cadd(s1, u1));
-            } else {
-                // RK3CN.cpp:105-107, :141-143, :177-179; the "old" operator is
-                // the one of the previous substep (or linearOPs_Init at t0)
-                double Lo;
-                const double kxo = __dadd_rn(skx0[mb], __dmul_rn(qt_old, ky));
-                if (st.oinit) {
-                    const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
-                    Lo = P::lin_init(pr, kxo, lap2, ph);
-                } else {
-                    Lo = P::lin(pr, kxo + qdtl_old * ky, ky, kz2, ph);
-                }
-                const double denom = 1.0 / (1.0 - a.linC * Ln);
-                const double c1 = a.nlc1 * denom, co = (1 + a.linC * Lo) * denom;
-                double2 s0 = cscale(c1, N0), s1 = cscale(c1, N1);
-                if (a.n_prev) {
-                    const double c2 = a.nlc2 * denom;
-                    s0 = cadd(s0, cscale(c2, a.n_prev[off0]));
-                    s1 = cadd(s1, cscale(c2, a.n_prev[off1]));
-                }
-                un0 = cadd(s0, cscale(co, u0));
-                un1 = cadd(s1, cscale(co, u1));
-            }
-            if (do_remap && remap_hit(remap_qt, rp.remap_thr, skx0[mb], ky)) un0 = un1 = make_double2(0.0, 0.0);
-            a.u[off0] = un0;
-            a.u[off1] = un1;
-            if (a.n_out) {
-                a.n_out[off0] = N0;
-                a.n_out[off1] = N1;
-            }
+            if (hit && im == 0) a.kx0[rp.mode_off + l] = __dsub_rn(kx0, rp.remap_shift);
         }
-        if (do_remap && tid < MB) {
-            const int i = sidx[tid];
-            if (i >= 0 && remap_hit(remap_qt, rp.remap_thr, skx0[tid], sky[tid]))
-                a.kx0[i] = __dsub_rn(skx0[tid], rp.remap_shift);
-        }
-        __syncthreads();
+        // no barrier here: the next group's B touches only sbase and the mode
+        // tables, which nothing in F reads; its C starts after B's barrier
     }
 
     // fixed-order reduction of the per-thread sums over the MB mode slots:
     // one partial [NACC][NZ] per workgroup
+    __syncthreads();
     double *red = reinterpret_cast<double *>(buf);
+    if (has_z)
 #pragma unroll
-    for (int p = 0; p < PW; ++p) {
-        const int item = tid + p * NT;
-        if (PI_ % NT != 0 && item >= PI_) continue;
-        const int mb = item / (2 * NZ), rem = item % (2 * NZ);
-        const int part = rem / NZ, z = rem % NZ;
-        red[(mb * P::NACC + P::acc_slot(part, 0)) * NZ + z] = acc[p][0];
-        if (P::NACC == 4) red[(mb * P::NACC + P::acc_slot(part, 1)) * NZ + z] = acc[p][1];
-    }
+        for (int c = 0; c < P::NACC; ++c) red[(zmb * P::NACC + c) * NZ + zz] = acc[c];
     __syncthreads();
     for (int k = tid; k < P::NACC * NZ; k += NT) {
         double s = 0.0;
@@ -530,18 +539,20 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 
 // FP64 path and mixed path (FP32 transforms); each with its own occupancy
 // request, by pass size
+// request, by pass size; the substep mode (MODE_RHS, MODE_ELIN, MODE_CN) is a
+// template parameter, so each instantiation carries only its own update
 constexpr bool half_pass(int NZ, int MB) { return NZ * MB < 256; }
-template <int NZ, int MB, class P>
+template <int NZ, int MB, class P, int MODE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES64_HALF : QL_WAVES64)))
 void k_mode64(ModeArgs a)
 {
-    mode_body<NZ, MB, P, double2>(a);
+    mode_body<NZ, MB, P, double2, MODE>(a);
 }
-template <int NZ, int MB, class P>
+template <int NZ, int MB, class P, int MODE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES32_HALF : QL_WAVES32)))
 void k_mode32(ModeArgs a)
 {
-    mode_body<NZ, MB, P, float2>(a);
+    mode_body<NZ, MB, P, float2, MODE>(a);
 }
 
 }  // namespace qlk
