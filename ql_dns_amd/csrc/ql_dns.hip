@@ -50,7 +50,7 @@ void release(Batch &b)
     for (Group &g : b.groups) {
         dfree(g.u), dfree(g.N1), dfree(g.N2), dfree(g.mf), dfree(g.mfN1), dfree(g.mfN2), dfree(g.drive);
         dfree(g.tw), dfree(g.tw32), dfree(g.kx0), dfree(g.ky), dfree(g.kzv), dfree(g.mfz), dfree(g.part);
-        dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg_run), dfree(g.wg_g0), dfree(g.wg_g1);
+        dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg);
         dfree(g.clock);
         if (g.join) (void)hipEventDestroy(g.join);
         if (g.s) (void)hipStreamDestroy(g.s);
@@ -163,32 +163,32 @@ int configure(Batch &b, Group &g, int line, int target)
     long total = 0;
     for (int id : g.run_ids) total += (b.runs[id].nxy + MB - 1) / MB;
     const long gpw = std::max<long>(1, (total + target - 1) / target);
-    std::vector<int> wr, w0, w1;
+    std::vector<qlk::WgInfo> wi;
     for (int k = 0; k < g.nruns; ++k) {
         qlk::RunParams &rp = g.h_rp[k];
         const int ng = (rp.nxy + MB - 1) / MB;
         const int nw = (int)((ng + gpw - 1) / gpw);
-        rp.wg_off = (int)wr.size();
+        rp.wg_off = (int)wi.size();
         rp.nwg = nw;
         for (int j = 0; j < nw; ++j) {
-            wr.push_back(k);
-            w0.push_back((int)((long)j * ng / nw));
-            w1.push_back((int)((long)(j + 1) * ng / nw));
+            qlk::WgInfo w{};
+            w.run = k;
+            w.g0 = (int)((long)j * ng / nw);
+            w.g1 = (int)((long)(j + 1) * ng / nw);
+            w.mode_off = rp.mode_off;
+            w.nxy = rp.nxy;
+            wi.push_back(w);
         }
     }
-    const int W = (int)wr.size();
+    const int W = (int)wi.size();
     const int ncols = (b.nMF) * g.NZ;
     if (W > g.wg_cap) {
-        dfree(g.wg_run), dfree(g.wg_g0), dfree(g.wg_g1), dfree(g.part);
-        QL_RC(dalloc(&g.wg_run, W));
-        QL_RC(dalloc(&g.wg_g0, W));
-        QL_RC(dalloc(&g.wg_g1, W));
+        dfree(g.wg), dfree(g.part);
+        QL_RC(dalloc(&g.wg, W));
         QL_RC(dalloc(&g.part, (size_t)W * ncols));
         g.wg_cap = W;
     }
-    QL_HIP(hipMemcpy(g.wg_run, wr.data(), sizeof(int) * W, hipMemcpyHostToDevice));
-    QL_HIP(hipMemcpy(g.wg_g0, w0.data(), sizeof(int) * W, hipMemcpyHostToDevice));
-    QL_HIP(hipMemcpy(g.wg_g1, w1.data(), sizeof(int) * W, hipMemcpyHostToDevice));
+    QL_HIP(hipMemcpy(g.wg, wi.data(), sizeof(qlk::WgInfo) * W, hipMemcpyHostToDevice));
     QL_HIP(hipMemcpy(g.rp, g.h_rp.data(), sizeof(qlk::RunParams) * g.nruns, hipMemcpyHostToDevice));
     g.line = line;
     g.target = target;
@@ -340,9 +340,7 @@ qlk::ModeArgs mode_args(const Batch &b, const Group &g)
     a.mfz = g.mfz;
     a.partials = g.part;
     a.runs = g.rp;
-    a.wg_run = g.wg_run;
-    a.wg_g0 = g.wg_g0;
-    a.wg_g1 = g.wg_g1;
+    a.wg = g.wg;
     a.clock = g.clock;
     const double dt = b.dt;
     a.dt = dt;

@@ -289,11 +289,12 @@ __device__ __forceinline__ void fft_pass_io(const C *tw, const Ld &ld, const St 
     __syncthreads();
 }
 
-template <int NZ, class C>
+// lines of LS slots (LS >= NZ)
+template <int LS, class C>
 struct BufIO {
     C *buf;
-    __device__ C operator()(int line, int i) const { return buf[line * NZ + i]; }
-    __device__ void operator()(int line, int i, C v) const { buf[line * NZ + i] = v; }
+    __device__ C operator()(int line, int i) const { return buf[line * LS + i]; }
+    __device__ void operator()(int line, int i, C v) const { buf[line * LS + i] = v; }
 };
 
 // The intermediate between the two passes.  The first pass writes index
@@ -310,35 +311,40 @@ struct PadIO {
     __device__ void operator()(int line, int i, C v) const { buf[line * LP + i + i / R1] = v; }
 };
 
-// slots of buf used by a two-pass transform of NL lines
-template <int NZ, int NL>
+// Line strides that keep 8-lane groups on different LDS banks: consecutive
+// lines start 8 slots apart modulo 16 (half the banks apart for 16-byte
+// values, a quarter for 8-byte values).
+constexpr int bank_stride(int n) { return n + ((8 - n % 16) + 16) % 16; }
+
+// slots of buf used by a two-pass transform of NL lines of stride LS
+template <int NZ, int NL, int LS>
 struct Plan2Size {
     static constexpr int LP = NZ + NZ / Plan2<NZ>::R1;
-    static constexpr int SLOTS = NL * (LP > NZ ? LP : NZ);
+    static constexpr int SLOTS = NL * (LP > LS ? LP : LS);
 };
 
 // Inverse (exp(+i)) transform of NL lines whose first pass reads src(line, g),
-// result in buf (lines of NZ).  The caller must have synchronised before buf
-// may be overwritten and before src may be read.  Three barriers.
-template <int NZ, int NL, class C, class Src>
+// result in buf (lines of LS slots).  The caller must have synchronised before
+// buf may be overwritten and before src may be read.  Three barriers.
+template <int NZ, int NL, int LS, class C, class Src>
 __device__ __forceinline__ void fft2_inv_from(C *buf, const C *tw, const Src &src)
 {
     using PL = Plan2<NZ>;
-    const BufIO<NZ, C> io{buf};
+    const BufIO<LS, C> io{buf};
     const PadIO<NZ, C> pad{buf};
     fft_pass_io<NZ, NL, true, PL::R1, 1, false>(tw, src, pad);
     fft_pass_io<NZ, NL, true, PL::R2, PL::R1, true>(tw, pad, io);
 }
 
-// Forward (exp(-i)) transform of NL lines held in buf (lines of NZ); the last
-// pass writes through dst(line, index, value), which must not alias the first
-// Plan2Size<NZ, NL>::SLOTS slots of buf.  The caller must have synchronised
-// after filling buf.  Three barriers.
-template <int NZ, int NL, class C, class Dst>
+// Forward (exp(-i)) transform of NL lines held in buf (lines of LS slots); the
+// last pass writes through dst(line, index, value), which must not alias the
+// first Plan2Size<NZ, NL, LS>::SLOTS slots of buf.  The caller must have
+// synchronised after filling buf.  Three barriers.
+template <int NZ, int NL, int LS, class C, class Dst>
 __device__ __forceinline__ void fft2_fwd_to(C *buf, const C *tw, const Dst &dst)
 {
     using PL = Plan2<NZ>;
-    const BufIO<NZ, C> io{buf};
+    const BufIO<LS, C> io{buf};
     const PadIO<NZ, C> pad{buf};
     fft_pass_io<NZ, NL, false, PL::R1, 1, true>(tw, io, pad);
     fft_pass_io<NZ, NL, false, PL::R2, PL::R1, false>(tw, pad, dst);

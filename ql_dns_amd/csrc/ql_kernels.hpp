@@ -41,6 +41,12 @@ struct RunParams {
     int wg_off, nwg;     // its mode-kernel workgroups = its partial rows
 };
 
+// one mode-kernel workgroup: its run, its mode-group range [g0, g1), and the
+// run's first mode and mode count (one record, one scalar load)
+struct WgInfo {
+    int run, g0, g1, mode_off, nxy, pad0, pad1, pad2;
+};
+
 // time of a set of runs stepped together: t as main.cpp:116 accumulates it,
 // t_prev the previous step start; old_init: the RK3CN "old" operator is the
 // one of linearOPs_Init at t_init (RK3CN.cpp:57)
@@ -62,7 +68,7 @@ struct ModeArgs {
     const double *mfz;      // real-space mean-field profiles [runs][8][NZ]
     double *partials;       // [workgroups][nacc][NZ]
     const RunParams *runs;  // [runs]
-    const int *wg_run, *wg_g0, *wg_g1;  // per workgroup: run, mode-group range
+    const WgInfo *wg;       // per workgroup
     const Clock *clock;
     int s;                  // substep 1..3, or 0 for the explicit times below
     int mode, do_remap;

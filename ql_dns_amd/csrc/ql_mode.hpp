@@ -23,14 +23,15 @@
 #pragma once
 #include "ql_kernels.hpp"
 
-// register prefetch of the next group's state rows (1 = on)
-#ifndef QL_PREFETCH_ROWS
-#define QL_PREFETCH_ROWS 1
-#endif
 // occupancy requests (waves per SIMD, which bound the VGPR budget) for
 // 256-point passes and for 128-point passes (half the modes per pass)
 #ifndef QL_WAVES64
 #define QL_WAVES64 2
+#endif
+// FP64 at NZ = 256: fetch the state at the start of each group (1) or one
+// group ahead (0)
+#ifndef QL_NZ256_FETCH_AT_USE
+#define QL_NZ256_FETCH_AT_USE 1
 #endif
 #ifndef QL_WAVES32
 #define QL_WAVES32 2
@@ -275,18 +276,20 @@ struct LineSrc {
 template <int NZ, int MB, class P>
 struct GroupLayout {
     static constexpr int NZD = Dealias<NZ>::nz;
+    static constexpr int NZDP = bank_stride(NZD);  // base-table line stride
+    static constexpr int LS = bank_stride(NZ);     // z-grid line stride
     static constexpr int NL = P::NLINE * MB;
     static constexpr int NLO = P::NOUT * MB;
-    static constexpr int CT_OFF = Plan2Size<NZ, NLO>::SLOTS;
+    static constexpr int CT_OFF = Plan2Size<NZ, NLO, LS>::SLOTS;
     static constexpr int CT = NLO * NZD;
-    static constexpr int A = Plan2Size<NZ, NL>::SLOTS, B = CT_OFF + CT;
+    static constexpr int A = Plan2Size<NZ, NL, LS>::SLOTS, B = CT_OFF + CT;
     static constexpr int BUF = A > B ? A : B;
 };
 
 // first-pass source of the inverse transforms in the interleaved layout
 template <int NZ, int MB, class P, class C>
 struct LineSrcI {
-    const C *sbase;  // [(base * MB + mb)][nz]
+    const C *sbase;  // [(base * MB + mb)][NZDP]
     const RealOf<C> *skz;
     __device__ C operator()(int line, int g) const
     {
@@ -294,7 +297,7 @@ struct LineSrcI {
         if (!D::kept(g)) return Cx<C>::make(0, 0);
         const int m = D::slot(g);
         const int l = line / MB, mb = line - l * MB;
-        const C v = sbase[(P::base_of(l) * MB + mb) * D::nz + m];
+        const C v = sbase[(P::base_of(l) * MB + mb) * GroupLayout<NZ, MB, P>::NZDP + m];
         return P::dz_of(l) ? imul(skz[m], v) : v;
     }
 };
@@ -321,7 +324,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     constexpr int NZI = MB * NZ;   // (mode slot, z) items: products
     static_assert(NIT <= NT && NZI <= NT, "one item per thread");
     __shared__ C buf[L::BUF];
-    __shared__ C sbase[MB * P::NB * NZD];
+    __shared__ C sbase[MB * P::NB * L::NZDP];
     __shared__ C tw[NZ];
     __shared__ R skzr[NZD];
     __shared__ double skx[MB], sky[MB];
@@ -334,10 +337,10 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     const int tid = threadIdx.x;
 
     // this workgroup's run, its mode groups [g0, g1), the substep times
-    const int run = a.wg_run[blockIdx.x];
+    const WgInfo wgi = a.wg[blockIdx.x];
+    const int run = wgi.run, g0 = wgi.g0, g1 = wgi.g1, mode_off = wgi.mode_off, nxy = wgi.nxy;
     const RunParams rp = a.runs[run];
     const Phys ph = rp.ph;
-    const int g0 = a.wg_g0[blockIdx.x], g1 = a.wg_g1[blockIdx.x];
     // q t products: wave-uniform, kept in scalar registers
     const SubTimes st = sub_times(a);
     const double qt = to_sgpr(__dmul_rn(ph.q, st.ts)), qdtl = to_sgpr(__dmul_rn(ph.q, st.dtl));
@@ -372,37 +375,42 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     // run-local mode index of this thread's item in group g, or -1
     auto local_mode = [&](int g) {
         const int l = g * MB + imb;
-        return (has_it && g < g1 && l < rp.nxy) ? l : -1;
+        return (has_it && g < g1 && l < nxy) ? l : -1;
     };
+    // the item's state row and wavenumbers, fetched one group ahead (except
+    // for FP64 at NZ = 256, where the registers are needed by the transforms)
+    constexpr bool AHEAD = !(sizeof(C) == 16 && NZ >= 256 && QL_NZ256_FETCH_AT_USE);
     double2 upre[4];
+    double kpre[2];
     auto fetch = [&](int g) {
         const int l = local_mode(g);
-        const size_t off = (size_t)(rp.mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
+        const int i = mode_off + (l < 0 ? 0 : l);
+        const size_t off = (size_t)i * 4 * NZD + im;
 #pragma unroll
         for (int f = 0; f < 4; ++f) upre[f] = (l >= 0) ? a.u[off + f * NZD] : make_double2(0.0, 0.0);
+        kpre[0] = (l >= 0) ? a.kx0[i] : 0.0;
+        kpre[1] = (l >= 0) ? a.ky[i] : 0.0;
     };
-    fetch(g0);
+    if (AHEAD) fetch(g0);
 
     for (int g = g0; g < g1; ++g) {
         // B: state of the item (prefetched), next group's state, previous RHS;
         // kx(t), 1/lap2, 1/lapF (assign_laplacians_, MHD_FullQlin.cpp:722-738,
         // Kdata.cpp:100-120) and the base spectral fields
         const int l = local_mode(g);
+        if (!AHEAD) fetch(g);
         const double2 U0 = upre[0], U1 = upre[1], U2 = upre[2], U3 = upre[3];
-        if (QL_PREFETCH_ROWS) fetch(g + 1);
-        const size_t off = (size_t)(rp.mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
-        double kx0 = 0.0, ky = 0.0, kx = 0.0, ilapF = 1.0;
+        const double kx0 = kpre[0], ky = kpre[1];
+        if (AHEAD) fetch(g + 1);
+        const size_t off = (size_t)(mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
+        double kx = 0.0, ilapF = 1.0;
         if (has_it) {
-            if (l >= 0) {
-                kx0 = a.kx0[rp.mode_off + l];
-                ky = a.ky[rp.mode_off + l];
-                kx = __dadd_rn(kx0, __dmul_rn(qt, ky));  // kx(t), MHD_FullQlin.cpp:727
-            }
+            if (l >= 0) kx = __dadd_rn(kx0, __dmul_rn(qt, ky));  // kx(t), MHD_FullQlin.cpp:727
             const bool zero = (ky == 0.0 && im == 0);
             const double lap2 = zero ? 0.0 : (-ky * ky + kz2);
             const double ilap2 = zero ? 1.0 : 1.0 / lap2;
             if (l >= 0) ilapF = (l == 0 && im == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
-            P::bases(U0, U1, U2, U3, kx, ky, kz, ilap2, a.fac, sbase + imb * NZD + im, MB * NZD);
+            P::bases(U0, U1, U2, U3, kx, ky, kz, ilap2, a.fac, sbase + imb * L::NZDP + im, MB * L::NZDP);
             if (im == 0) {
                 skx[imb] = kx;
                 sky[imb] = ky;
@@ -412,11 +420,11 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         __syncthreads();
 
         // C: inverse transforms of the padded lines
-        fft2_inv_from<NZ, L::NL>(buf, tw, LineSrcI<NZ, MB, P, C>{sbase, skzr});
+        fft2_inv_from<NZ, L::NL, L::LS>(buf, tw, LineSrcI<NZ, MB, P, C>{sbase, skzr});
 
         // D: products on the z grid, in place, and the EMF / stress sums
         if (has_z) {
-            C *X = buf + zmb * NZ + zz;  // line l at X[l * MB * NZ]
+            C *X = buf + zmb * L::LS + zz;  // line l at X[l * MB * LS]
             const R kxr = (R)skx[zmb], kyr = (R)sky[zmb];
             R mfl[P::NMFZ];
             if constexpr (!MF_REG) {
@@ -426,8 +434,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             const R *mfp = MF_REG ? mfr : mfl;
             C prod[2][3];
             double racc[2][2];
-            P::products(0, X, MB * NZ, mfp, kxr, kyr, prod[0], racc[0]);
-            P::products(1, X, MB * NZ, mfp, kxr, kyr, prod[1], racc[1]);
+            P::products(0, X, MB * L::LS, mfp, kxr, kyr, prod[0], racc[0]);
+            P::products(1, X, MB * L::LS, mfp, kxr, kyr, prod[1], racc[1]);
             if (sok[zmb]) {
                 const double ftfac = (sky[zmb] == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
 #pragma unroll
@@ -440,7 +448,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             for (int part = 0; part < 2; ++part)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    if (c < P::nprod(part)) X[P::out_line(part, c) * MB * NZ] = prod[part][c];
+                    if (c < P::nprod(part)) X[P::out_line(part, c) * MB * L::LS] = prod[part][c];
         }
         __syncthreads();
         // previous substep's RHS of the item, needed in F
@@ -451,7 +459,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 
         // E: forward transforms of the product lines, dealiased band kept
         C *ct = buf + L::CT_OFF;  // [line][nz]
-        fft2_fwd_to<NZ, L::NLO>(buf, tw, CompactDst<NZ, C>{ct});
+        fft2_fwd_to<NZ, L::NLO, L::LS>(buf, tw, CompactDst<NZ, C>{ct});
 
         // F: RHS, dissipation operator at the substep end, update
         if (l >= 0) {
@@ -515,7 +523,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                     a.n_out[off1] = N1;
                 }
             }
-            if (hit && im == 0) a.kx0[rp.mode_off + l] = __dsub_rn(kx0, rp.remap_shift);
+            if (hit && im == 0) a.kx0[mode_off + l] = __dsub_rn(kx0, rp.remap_shift);
         }
         // no barrier here: the next group's B touches only sbase and the mode
         // tables, which nothing in F reads; its C starts after B's barrier

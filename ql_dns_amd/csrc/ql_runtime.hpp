@@ -74,7 +74,7 @@ struct Group {
     double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr, *part = nullptr, *sums = nullptr;
     double *lin_tmp = nullptr;
     qlk::RunParams *rp = nullptr;
-    int *wg_run = nullptr, *wg_g0 = nullptr, *wg_g1 = nullptr;
+    qlk::WgInfo *wg = nullptr;
     qlk::Clock *clock = nullptr;  // the group's step times
     std::vector<qlk::RunParams> h_rp;
 };
