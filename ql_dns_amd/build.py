@@ -43,8 +43,75 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libql_dns.so")
     os.replace(tmp, LIB)
+    io = build_io(verbose)
     build_host_test(verbose)
+    if io:
+        build_io_test(verbose)
     return LIB
+
+
+IO_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_io_restart.cpp")
+IO_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_io_restart")
+
+
+def build_io_test(verbose: bool = False) -> str:
+    """g++ build of the C++ dump / restart driver over FullSave_Load."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), IO_TEST_SRC, "-L", HERE, "-lql_io",
+           "-lql_dns", "-Wl,-rpath,$ORIGIN/../../ql_dns_amd", "-o", IO_TEST_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building tests/cpp/test_io_restart")
+    return IO_TEST_BIN
+
+
+IO_LIB = os.path.join(HERE, "libql_io.so")
+IO_SRC = os.path.join(CSRC, "io", "ql_io.cpp")
+HDF5_PREFIXES = ("/opt/conda", "/usr", "/usr/local")
+
+
+def hdf5_prefix():
+    for pre in HDF5_PREFIXES:
+        if os.path.exists(os.path.join(pre, "include", "hdf5.h")):
+            return pre
+    return None
+
+
+def build_io(verbose: bool = False):
+    """g++ build of libql_io.so (HDF5 dump/restart and the .dat time series),
+    linked against libql_dns.so and the HDF5 C library.  Skipped (with a
+    message) when no HDF5 installation is found."""
+    pre = hdf5_prefix()
+    if pre is None:
+        sys.stderr.write("ql_dns_amd.build: HDF5 not found; libql_io.so not built\n")
+        return None
+    # only the HDF5 library itself is reached through a private link
+    # directory; putting the HDF5 prefix's lib/ on the run path would also
+    # pull its (older) C++ runtime into the process
+    libdir = os.path.join(pre, "lib")
+    real = os.path.join(libdir, "libhdf5.so")
+    soname = next((f for f in sorted(os.listdir(libdir)) if f.startswith("libhdf5.so.")), None)
+    linkdir = os.path.join(HERE, "_hdf5")
+    os.makedirs(linkdir, exist_ok=True)
+    if soname:
+        dst = os.path.join(linkdir, soname)
+        if os.path.lexists(dst):
+            os.remove(dst)
+        os.symlink(os.path.join(libdir, soname), dst)
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(pre, "include"), IO_SRC, "-L", HERE, "-lql_dns", real,
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/_hdf5", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building libql_io.so")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
 
 
 def build_host_test(verbose: bool = False) -> str:
