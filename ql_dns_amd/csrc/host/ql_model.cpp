@@ -176,6 +176,7 @@ solution::solution(Model *eqs)
 
 void solution::pack(std::vector<dcmplx> &lin, std::vector<dcmplx> &mf) const
 {
+    pull();  // the device copy may be the newer one
     lin.resize((size_t)nxy_ * nLin_ * nz_);
     mf.resize((size_t)nMF_ * nz_);
     for (size_t k = 0; k < lin_.size(); ++k) std::copy(lin_[k].data(), lin_[k].data() + nz_, lin.data() + k * nz_);

@@ -54,11 +54,13 @@ def test_cpp_dump_and_restart(tmp_path, fp64):
                        timeout=120)
     assert r.returncode == 0, r.stderr
     (la, ma, ka, ta), (lb, mb, kb, tb) = _read(out, p.nxy, p.nz, p.nMF)
-    assert tb == ta
-    assert np.array_equal(kb, ka)
     if fp64:
+        assert tb == ta and np.array_equal(kb, ka)
         assert np.array_equal(lb, la) and np.array_equal(mb, ma)
     else:
+        # the group name keeps t to 15 significant digits and kx is stored
+        # as float32 (FullSave_Load.cpp:183-188, 231-240)
+        assert abs(tb - ta) < 1e-12 and rel_err(kb, ka) < 1e-6
         assert rel_err(mb, ma) < 1e-5 and rel_err(lb, la) < 1e-5
     n = n1 + n2 + 1
     rd = lambda name: np.fromfile(os.path.join(d, name), dtype="<f8")
