@@ -30,7 +30,7 @@ extern "C" {
 #define QL_OK 0
 #define QL_ERR_ARG (-1)         /* invalid argument or unsupported size      */
 #define QL_ERR_HIP (-2)         /* HIP runtime failure / no device           */
-#define QL_ERR_UNSUPPORTED (-3) /* feature outside this build (e.g. dt < 0)  */
+#define QL_ERR_UNSUPPORTED (-3) /* feature outside this build               */
 #define QL_ERR_STATE (-4)       /* non-finite or runaway state detected      */
 
 enum { QL_MODEL_MHD_FULLQLIN = 0, QL_MODEL_MHD_BQLIN = 1 };
@@ -52,8 +52,11 @@ typedef struct ql_params {
     int viscosity_order; /* viscosity_order_                                 */
     int quasilinear;     /* QuasiLinear?                                     */
     int remap;           /* Remap? (applied by ql_steps after each step)      */
-    double dt;           /* dt_ (> 0; variable-dt CFL stepping unsupported)   */
+    double dt;           /* dt_: > 0 fixed steps; < 0 variable steps of at most
+                            -dt, CFL / Calculate_CFL (RK3CN.cpp:20-30, 79-83)  */
     int device;          /* HIP device ordinal                               */
+    double CFL;          /* CFL number of variable steps (Input_parameters.cpp:77,
+                            default 1.0); unused when dt > 0                  */
 } ql_params;
 
 typedef struct ql_ctx ql_ctx;
@@ -83,12 +86,17 @@ int ql_linear_ops_init(ql_ctx *ctx, double t0, double *linop, double *linop_mf);
 int ql_rhs(ql_ctx *ctx, double t, double dt_lin, double *rhs_lin, double *rhs_mf, double *linop);
 
 /* Integrator::Step(t, sol) (Integrators/Integrator.h:23; RK3CN.cpp:77-201,
- * RK3CN_Elin.cpp:275-389).  Writes the step size to *dt_out (may be NULL). */
+ * RK3CN_Elin.cpp:275-389).  Writes the step size to *dt_out (may be NULL):
+ * dt, or with dt < 0 the variable step CFL / Calculate_CFL capped at -dt
+ * (RK3CN_Elin.cpp:76-80), computed on the device. */
 int ql_step(ql_ctx *ctx, double t, double *dt_out);
 
 /* nsteps of the reference main loop (main.cpp:108-123, noise off, no output):
  * Step, t += dt (accumulated in double, as main.cpp:116), then
- * ShearingBox_Remap(q t) if remap is on.  *t_inout is read and updated. */
+ * ShearingBox_Remap(q t) if remap is on.  *t_inout is read and updated.
+ * Variable steps (dt < 0) are chosen on the device step by step, so they
+ * replay from the same captured graph; a run set with variable steps holds
+ * one run. */
 int ql_steps(ql_ctx *ctx, int nsteps, double *t_inout);
 
 /* Model::ShearingBox_Remap(qt, sol) (MHD_FullQlin.cpp:464-481). */
@@ -107,6 +115,21 @@ int ql_get_drive(ql_ctx *ctx, double *drive);
 /* TimeVariables::Save_Mean_Fields values (TimeVariables.cpp:202-228):
  * fac1D * Re(ifft(pad(MF))) -> [nMF][NZ] doubles. */
 int ql_mf_real(ql_ctx *ctx, double *out);
+
+/* Model::Calculate_CFL (MHD_FullQlin.cpp:769-780, MHD_BQlin.cpp:540-552):
+ * CFL/dt of the next variable step, kmax (max|By| + max|Bx| + max|Ux| +
+ * max|Uy|) + q over the mean-field profiles the last rhs read (MHD_BQlin:
+ * kmax max|By| + kmax max|Bx| + q, Bx of the current state).  The maxima are
+ * recorded on the device by every mean-field update. */
+int ql_calculate_cfl(ql_ctx *ctx, double *cfl_over_dt);
+
+/* solution::Check_Solution (solution.cpp:169-203) as a device reduction of the
+ * mean fields.  sums (may be NULL): [0] sum |Re ifft(pad(By))|, [1] sum
+ * |Im ifft(pad(By))|, [2] max |By|, [3] max |Bx|, [4] sum |By|.  *verdict (may
+ * be NULL): 0 passed, 1 the mean field has developed an imaginary part, 2 the
+ * solution is very large (> 1e20), 3 it holds NaN or Inf.  A failed check
+ * returns QL_ERR_STATE with the reference's message (where it aborts). */
+int ql_check_solution(ql_ctx *ctx, double sums[5], int *verdict);
 
 /* Waits for all work queued on the context's stream. */
 int ql_synchronize(ql_ctx *ctx);
@@ -142,6 +165,9 @@ int ql_batch_set_state(ql_batch *b, int run, const double *lin, const double *mf
 int ql_batch_get_state(ql_batch *b, int run, double *lin, double *mf, double *kx0);
 int ql_batch_diagnostics(ql_batch *b, int run, double t, double out[18]);
 int ql_batch_get_drive(ql_batch *b, int run, double *drive);
+/* per-run ql_calculate_cfl / ql_check_solution */
+int ql_batch_calculate_cfl(ql_batch *b, int run, double *cfl_over_dt);
+int ql_batch_check_solution(ql_batch *b, int run, double sums[5], int *verdict);
 /* integrator priming at t0 of every run (RK3CN.cpp:57, RK3CN_Elin.cpp:257) */
 int ql_batch_linear_ops_init(ql_batch *b, double t0);
 /* nsteps of main.cpp:108-123 for every run (as ql_steps), one clock */

@@ -30,7 +30,8 @@ class _Params(ctypes.Structure):
                 ("L", ctypes.c_double * 3), ("q", ctypes.c_double), ("omega", ctypes.c_double),
                 ("nu", ctypes.c_double), ("eta", ctypes.c_double), ("B0z", ctypes.c_double),
                 ("viscosity_order", ctypes.c_int), ("quasilinear", ctypes.c_int),
-                ("remap", ctypes.c_int), ("dt", ctypes.c_double), ("nthreads", ctypes.c_int)]
+                ("remap", ctypes.c_int), ("dt", ctypes.c_double), ("nthreads", ctypes.c_int),
+                ("CFL", ctypes.c_double)]
 
 
 def build(native: bool = False) -> str:
@@ -58,6 +59,10 @@ def lib(native: bool = False):
     L.orc_tables.argtypes = [P, D, D, I, I, D]
     L.orc_linear_ops_init.argtypes = [P, ctypes.c_double, D, D]
     L.orc_rhs.argtypes = [P, ctypes.c_double, ctypes.c_double, D, D, D]
+    L.orc_calculate_cfl.restype = ctypes.c_double
+    L.orc_calculate_cfl.argtypes = [P]
+    L.orc_check_sums.argtypes = [P, D]
+    L.orc_check_solution.argtypes = [P]
     L.orc_step.restype = ctypes.c_double
     L.orc_step.argtypes = [P, ctypes.c_double]
     L.orc_steps.restype = ctypes.c_double
@@ -102,10 +107,10 @@ def fft_plain(x: np.ndarray, sign: int) -> np.ndarray:
 class Oracle:
     def __init__(self, model=0, integrator=1, NX=16, NY=16, NZ=32, L=(1.0, 4.0, 1.0), q=1.5,
                  omega=1.0, nu=5e-4, eta=5e-4, B0z=0.0, viscosity_order=2, quasilinear=1,
-                 remap=1, dt=0.01, nthreads=1, native=False):
+                 remap=1, dt=0.01, CFL=1.0, nthreads=1, native=False):
         self._lib = lib(native)
         p = _Params(model, integrator, NX, NY, NZ, (ctypes.c_double * 3)(*L), q, omega, nu, eta,
-                    B0z, viscosity_order, quasilinear, remap, dt, nthreads)
+                    B0z, viscosity_order, quasilinear, remap, dt, nthreads, CFL)
         self._h = self._lib.orc_create(ctypes.byref(p))
         d = (ctypes.c_int * 6)()
         self._lib.orc_dims(self._h, d)
@@ -169,6 +174,20 @@ class Oracle:
 
     def step(self, t):
         return self._lib.orc_step(self._h, t)
+
+    def calculate_cfl(self):
+        """Model::Calculate_CFL: CFL / dt of the next variable step."""
+        return self._lib.orc_calculate_cfl(self._h)
+
+    def check_sums(self):
+        """Check_Solution's reductions: sum|Re|, sum|Im| of ifft(pad(By)),
+        max|By|, max|Bx|, sum|By|."""
+        out = np.zeros(5)
+        self._lib.orc_check_sums(self._h, _dp(out))
+        return out
+
+    def check_solution(self):
+        return self._lib.orc_check_solution(self._h)
 
     def steps(self, n, t=0.0):
         return self._lib.orc_steps(self._h, n, t)

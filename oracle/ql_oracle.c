@@ -800,10 +800,96 @@ void orc_rhs(orc *o, double t, double dt_lin, double *rhs_lin, double *rhs_mf, d
 /* Integrators                                                               */
 /* ------------------------------------------------------------------------ */
 
-/* RK3CN_Elin::Step (Integrators/RK3CN_Elin.cpp:275-389), fixed dt */
+/* kmax of the CFL condition (MHD_FullQlin.cpp:92-97, MHD_BQlin.cpp:70-75) */
+static double cfl_kmax(const orc *o)
+{
+    const double dealiasfac[3] = {2.0, 2.0, 3.0};
+    const int N[3] = {o->p.NX, o->p.NY, o->p.NZ};
+    double kmax = 0;
+    for (int i = 0; i < 3; ++i)
+        if (kmax < 2 * QL_PI / o->p.L[i] * (N[i] / dealiasfac[i])) kmax = 2 * QL_PI / o->p.L[i] * (N[i] / dealiasfac[i]);
+    return kmax;
+}
+
+/* sqrt(max |z|^2) of a complex profile (Eigen's abs2().maxCoeff()) */
+static double max_abs(const cplx *z, int n)
+{
+    double m = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double a = creal(z[j]) * creal(z[j]) + cimag(z[j]) * cimag(z[j]);
+        if (a > m) m = a;
+    }
+    return sqrt(m);
+}
+
+double orc_calculate_cfl(orc *o)
+{
+    const int NZ = o->NZ;
+    const double kmax = cfl_kmax(o);
+    if (o->p.model == ORC_MODEL_MHD_FULLQLIN) {
+        /* MHD_FullQlin.cpp:773-778: profiles MBx_, MBy_, MUx_, MUy_ of the last rhs */
+        double Bxmax = max_abs(o->mfz + 0 * NZ, NZ), Bymax = max_abs(o->mfz + 2 * NZ, NZ);
+        double Uxmax = max_abs(o->mfz + 4 * NZ, NZ), Uymax = max_abs(o->mfz + 6 * NZ, NZ);
+        return kmax * (Bymax + Bxmax + Uxmax + Uymax) + o->p.q;
+    }
+    if (o->p.model == ORC_MODEL_MHD_BQLIN) {
+        /* MHD_BQlin.cpp:544-550: By_ of the last rhs, Bx of the current state */
+        double Bymax = max_abs(o->mfz, NZ);
+        cplx *tmp = (cplx *)xcalloc(o->nz, sizeof(cplx)), *bx = (cplx *)xcalloc(NZ, sizeof(cplx));
+        for (int m = 0; m < o->nz; ++m) tmp[m] = o->mf[m] * o->fac1D;
+        fft_inverse(&o->plan, o->nz, tmp, bx);
+        double Bxmax = max_abs(bx, NZ);
+        free(tmp);
+        free(bx);
+        return kmax * Bymax + kmax * Bxmax + o->p.q;
+    }
+    return o->p.q;
+}
+
+/* the step's dt: fixed, or CFL / Calculate_CFL capped at -dt
+ * (RK3CN.cpp:20-30, 79-83; RK3CN_Elin.cpp:76-80) */
+static double step_dt(orc *o)
+{
+    if (o->p.dt > 0) return o->p.dt;
+    double dt = o->p.CFL / orc_calculate_cfl(o);
+    if (dt > -o->p.dt) dt = -o->p.dt;
+    return dt;
+}
+
+void orc_check_sums(const orc *o, double *out)
+{
+    const int nz = o->nz, NZ = o->NZ;
+    const cplx *Bx = o->mf, *By = o->mf + nz;
+    cplx *r = (cplx *)xcalloc(NZ, sizeof(cplx));
+    fft_inverse(&o->plan, nz, By, r);
+    double sre = 0, sim = 0, mby = 0, mbx = 0, sby = 0;
+    for (int j = 0; j < NZ; ++j) {
+        sre += fabs(creal(r[j]));
+        sim += fabs(cimag(r[j]));
+    }
+    for (int m = 0; m < nz; ++m) {
+        if (cabs(By[m]) > mby) mby = cabs(By[m]);
+        if (cabs(Bx[m]) > mbx) mbx = cabs(Bx[m]);
+        sby += cabs(By[m]);
+    }
+    free(r);
+    out[0] = sre; out[1] = sim; out[2] = mby; out[3] = mbx; out[4] = sby;
+}
+
+int orc_check_solution(const orc *o)
+{
+    double s[5];
+    orc_check_sums(o, s);
+    if (s[0] / s[1] < 1e10 && s[0] > 1e-2) return 1;  /* solution.cpp:174 */
+    if (s[2] > 1e20 || s[3] > 1e20) return 2;          /* solution.cpp:184 */
+    if (!isfinite(s[4])) return 3;                      /* solution.cpp:194 */
+    return 0;
+}
+
+/* RK3CN_Elin::Step (Integrators/RK3CN_Elin.cpp:275-389) */
 static double step_elin(orc *o, double t)
 {
-    const double dt = o->p.dt;
+    const double dt = step_dt(o);
     const int nz = o->nz, nMF = o->nMF;
     const long nl = (long)o->nxy * 4 * nz;
     double linC, NLC1, NLC2, lin_dt;
@@ -836,10 +922,10 @@ static double step_elin(orc *o, double t)
     return dt;
 }
 
-/* RK3CN::Step (Integrators/RK3CN.cpp:77-201), fixed dt */
+/* RK3CN::Step (Integrators/RK3CN.cpp:77-201) */
 static double step_cn(orc *o, double t)
 {
-    const double dt = o->p.dt;
+    const double dt = step_dt(o);
     const int nz = o->nz, nMF = o->nMF;
     const long nl = (long)o->nxy * 4 * nz;
     double linC, NLC1, NLC2, lin_dt;

@@ -50,8 +50,11 @@ typedef struct orc_params {
     int viscosity_order;
     int quasilinear;      /* QuasiLinear?                                  */
     int remap;            /* Remap?                                        */
-    double dt;            /* fixed time step (> 0)                         */
+    double dt;            /* fixed time step (> 0); < 0: variable steps of
+                             at most -dt (RK3CN.cpp:20-30)                 */
     int nthreads;         /* OpenMP threads (<= 0: all)                    */
+    double CFL;           /* Inputs::CFL (Input_parameters.cpp:77), used
+                             when dt < 0                                   */
 } orc_params;
 
 typedef struct orc orc;
@@ -69,6 +72,17 @@ void orc_tables(const orc *o, double *kx0, double *ky, int *kx_index, int *ky_in
 void orc_linear_ops_init(orc *o, double t0, double *linop, double *linop_mf);
 /* Model::rhs(t, dt_lin, SolIn = current state, SolOut, linOpFluct) */
 void orc_rhs(orc *o, double t, double dt_lin, double *rhs_lin, double *rhs_mf, double *linop);
+/* Model::Calculate_CFL (MHD_FullQlin.cpp:769-780, MHD_BQlin.cpp:540-552):
+ * CFL/dt from the profiles of the last rhs (and, for MHD_BQlin, Bx of the
+ * current mean field) */
+double orc_calculate_cfl(orc *o);
+/* solution::Check_Solution (solution.cpp:169-203) reductions:
+ * out[0] = sum |Re ifft(pad(By))|, out[1] = sum |Im ifft(pad(By))|,
+ * out[2] = max |By|, out[3] = max |Bx| (spectral), out[4] = sum |By| */
+void orc_check_sums(const orc *o, double *out5);
+/* the verdict of Check_Solution on those sums: 0 passes, 1 imaginary mean
+ * field, 2 very large, 3 NaN or Inf */
+int orc_check_solution(const orc *o);
 /* Integrator::Step(t, sol); returns dt */
 double orc_step(orc *o, double t);
 /* main.cpp:108-123 without output: Step, t += dt, remap(q t); returns t */

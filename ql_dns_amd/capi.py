@@ -23,7 +23,10 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_create", "ql_batch_destroy", "ql_batch_of", "ql_batch_size", "ql_batch_dims",
            "ql_batch_set_state", "ql_batch_get_state", "ql_batch_diagnostics", "ql_batch_get_drive",
            "ql_batch_linear_ops_init", "ql_batch_steps", "ql_batch_tune", "ql_batch_get_config",
-           "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward")
+           "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward", "ql_calculate_cfl",
+           "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution")
+
+QL_ERR_STATE = -4
 
 
 class QLError(RuntimeError):
@@ -38,7 +41,8 @@ class ql_params(ctypes.Structure):
                 ("L", ctypes.c_double * 3), ("q", ctypes.c_double), ("omega", ctypes.c_double),
                 ("nu", ctypes.c_double), ("eta", ctypes.c_double), ("B0z", ctypes.c_double),
                 ("viscosity_order", ctypes.c_int), ("quasilinear", ctypes.c_int),
-                ("remap", ctypes.c_int), ("dt", ctypes.c_double), ("device", ctypes.c_int)]
+                ("remap", ctypes.c_int), ("dt", ctypes.c_double), ("device", ctypes.c_int),
+                ("CFL", ctypes.c_double)]
 
 
 _LIB = None
@@ -87,6 +91,10 @@ def load(path: str = LIB_PATH):
         "ql_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "ql_adjoint_forward": ([P, ctypes.c_int, ctypes.c_int, D], ctypes.c_int),
         "ql_adjoint_backward": ([P, D, D, D, D, D], ctypes.c_int),
+        "ql_calculate_cfl": ([P, D], ctypes.c_int),
+        "ql_check_solution": ([P, D, I], ctypes.c_int),
+        "ql_batch_calculate_cfl": ([P, ctypes.c_int, D], ctypes.c_int),
+        "ql_batch_check_solution": ([P, ctypes.c_int, D, I], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -114,6 +122,17 @@ def _raise(L, rc: int):
         raise QLError(rc, L.ql_last_error().decode())
 
 
+def _check_solution(L, fn, args, strict: bool):
+    """Check_Solution: (verdict, sums); a failed check raises when strict
+    (the reference aborts, solution.cpp:169-203)."""
+    sums = np.zeros(5)
+    v = ctypes.c_int(0)
+    rc = fn(*args, _dp(sums), ctypes.byref(v))
+    if rc != 0 and (strict or rc != QL_ERR_STATE):
+        _raise(L, rc)
+    return v.value, sums
+
+
 def _get_config(L, h):
     n = ctypes.c_int(0)
     _raise(L, L.ql_batch_get_config(h, 0, ctypes.byref(n), None))
@@ -132,7 +151,7 @@ def _tune(L, h, trial_steps: int) -> float:
 def to_c_params(p: QLParams, device: int = 0) -> ql_params:
     return ql_params(p.model, p.integrator, p.precision, p.NX, p.NY, p.NZ,
                      (ctypes.c_double * 3)(p.Lx, p.Ly, p.Lz), p.q, p.omega, p.nu, p.eta, p.B0z,
-                     p.viscosity_order, p.quasilinear, p.remap, p.dt, device)
+                     p.viscosity_order, p.quasilinear, p.remap, p.dt, device, p.CFL)
 
 
 class QLContext:
@@ -222,6 +241,15 @@ class QLContext:
         out = np.zeros((self.nMF, self.NZ))
         self._check(self._L.ql_mf_real(self._h, _dp(out)))
         return out
+
+    def calculate_cfl(self) -> float:
+        """Model::Calculate_CFL: CFL / dt of the next variable step."""
+        v = ctypes.c_double(0.0)
+        self._check(self._L.ql_calculate_cfl(self._h, ctypes.byref(v)))
+        return v.value
+
+    def check_solution(self, strict: bool = False):
+        return _check_solution(self._L, self._L.ql_check_solution, (self._h,), strict)
 
     def synchronize(self):
         self._check(self._L.ql_synchronize(self._h))
@@ -334,6 +362,14 @@ class QLBatch:
         out = np.zeros((d["nMF"], d["nz"]), dtype=np.complex128)
         self._check(self._L.ql_batch_get_drive(self._h, run, _dp(out)))
         return out
+
+    def calculate_cfl(self, run: int) -> float:
+        v = ctypes.c_double(0.0)
+        self._check(self._L.ql_batch_calculate_cfl(self._h, run, ctypes.byref(v)))
+        return v.value
+
+    def check_solution(self, run: int, strict: bool = False):
+        return _check_solution(self._L, self._L.ql_batch_check_solution, (self._h, run), strict)
 
     def tune(self, trial_steps: int = 5) -> float:
         return _tune(self._L, self._h, trial_steps)

@@ -22,7 +22,6 @@ int integrator_id(const std::string &s)
     throw Error(QL_ERR_ARG, "integrator '" + s + "' is not available on the GPU path (RK3CN, RK3CN_Elin)");
 }
 
-const double kPi = 3.14159265358979323846;
 
 }  // namespace
 
@@ -58,6 +57,7 @@ GpuModel::GpuModel(const Inputs &sp) : Model(sp.NZ, sp.NXY, sp.L), sp_(sp), f_no
     p.quasilinear = sp.QuasiLinearQ ? 1 : 0;
     p.remap = sp.remapQ ? 1 : 0;
     p.dt = sp.dt;
+    p.CFL = sp.CFL;
     p.device = sp.device;
     check(ql_create(&p, &ctx_));
     check(ql_dims(ctx_, dims_));
@@ -137,22 +137,12 @@ void GpuModel::Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *
 
 double GpuModel::Calculate_CFL(const solution *sol)
 {
-    // MHD_FullQlin.cpp:769-780: kmax * (max|By| + max|Bx| + max|Ux| + max|Uy|) + q,
-    // kmax from :92-97; evaluated on the current mean fields.
+    // MHD_FullQlin.cpp:769-780 / MHD_BQlin.cpp:540-552 on the maxima the
+    // device records: profiles of the last rhs (and MHD_BQlin's current Bx)
     const_cast<solution *>(sol)->push();
-    const int nMF = num_MFs(), NZf = NZfull();
-    std::vector<double> prof((size_t)nMF * NZf);
-    check(ql_mf_real(ctx_, prof.data()));
-    double kmax = 0;
-    const double dealiasfac[3] = {2.0, 2.0, 3.0};
-    for (int i = 0; i < 3; ++i) kmax = std::max(kmax, 2 * kPi / L(i) * (N(i) / dealiasfac[i]));
-    double s = 0;
-    for (int f = 0; f < nMF; ++f) {
-        double mx = 0;
-        for (int z = 0; z < NZf; ++z) mx = std::max(mx, std::fabs(prof[(size_t)f * NZf + z]));
-        s += mx;
-    }
-    return kmax * s + sp_.q;
+    double v = 0.0;
+    check(ql_calculate_cfl(ctx_, &v));
+    return v;
 }
 
 std::vector<double> GpuModel::kx() const
@@ -208,6 +198,13 @@ void solution::push()
     stale_ = false;
 }
 
+void solution::Check_Solution()
+{
+    // solution.cpp:169-203; the reference aborts where this throws
+    push();
+    gm_->check(ql_check_solution(gm_->ctx(), nullptr, nullptr));
+}
+
 void solution::pull() const
 {
     if (gm_->resident != this || !stale_) return;
@@ -221,8 +218,7 @@ void solution::pull() const
 
 GpuIntegrator::GpuIntegrator(double t0, const Inputs &SP, GpuModel *mod) : model_(mod), dt_(SP.dt)
 {
-    if (!(SP.dt > 0))
-        throw Error(QL_ERR_UNSUPPORTED, "variable (CFL) time steps are not available on the GPU path");
+    // SP.dt < 0: variable steps (RK3CN.cpp:20-30), chosen by the device in ql_step
     // RK3CN.cpp:57 / RK3CN_Elin.cpp:257
     model_->linearOPs_Init(t0, nullptr, nullptr);
 }

@@ -66,7 +66,8 @@ struct Inputs {
     int NXY[2] = {16, 16};
     int NZ = 16;
     double L[3] = {6.283185307179586, 6.283185307179586, 6.283185307179586};
-    double dt = 0.33333333333333;
+    double dt = 0.33333333333333;  // < 0: variable steps of at most -dt
+    double CFL = 1.0;              // Input_parameters.cpp:77
     double t_start = 0.0, t_final = 10.0;
     double nu = 0.01, eta = 0.01, q = 1.5, omega = 1.0, B0z = 0.0;
     int viscosity_order = 2;
@@ -188,6 +189,9 @@ public:
         dirty_ = true;
         return &mf_[i];
     }
+    // solution.cpp:169-203: mean fields real, bounded and finite, as a device
+    // reduction; throws Error (QL_ERR_STATE) where the reference aborts
+    void Check_Solution();
     // device <-> host mirror management (used by GpuModel / GpuIntegrator)
     void push();              // host -> device if the host copy was modified
     void pull() const;        // device -> host if the host copy is stale

@@ -51,7 +51,7 @@ void release(Batch &b)
         dfree(g.u), dfree(g.N1), dfree(g.N2), dfree(g.mf), dfree(g.mfN1), dfree(g.mfN2), dfree(g.drive);
         dfree(g.tw), dfree(g.tw32), dfree(g.kx0), dfree(g.ky), dfree(g.kzv), dfree(g.mfz), dfree(g.part);
         dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg);
-        dfree(g.clock);
+        dfree(g.clock), dfree(g.pm);
         if (g.join) (void)hipEventDestroy(g.join);
         if (g.s) (void)hipStreamDestroy(g.s);
         g.join = nullptr;
@@ -91,8 +91,10 @@ int check_params(const ql_params *p, const char *who)
         return fail(QL_ERR_ARG, "%s: NX=%d, NY=%d must be even and >= 2", who, p->NX, p->NY);
     if (p->NX > 4096 || p->NY > 4096) return fail(QL_ERR_ARG, "%s: NX=%d, NY=%d above 4096", who, p->NX, p->NY);
     if (!(p->L[0] > 0) || !(p->L[1] > 0) || !(p->L[2] > 0)) return fail(QL_ERR_ARG, "%s: box sizes must be > 0", who);
-    if (!(p->dt > 0))
-        return fail(QL_ERR_UNSUPPORTED, "%s: dt=%g; variable (CFL) time steps are not supported", who, p->dt);
+    if (!(p->dt != 0) || !std::isfinite(p->dt))
+        return fail(QL_ERR_ARG, "%s: dt=%g (> 0 fixed steps, < 0 variable steps of at most -dt)", who, p->dt);
+    if (p->dt < 0 && !(p->CFL > 0 && std::isfinite(p->CFL)))
+        return fail(QL_ERR_ARG, "%s: variable steps need CFL > 0 (CFL=%g)", who, p->CFL);
     if (p->viscosity_order < 0) return fail(QL_ERR_ARG, "%s: viscosity_order < 0", who);
     return QL_OK;
 }
@@ -113,6 +115,12 @@ void fill_run(RunInfo &r, const ql_params &p)
     long tn = (long)p.NX * p.NY * p.NZ;
     r.totalN2 = (double)(tn * tn);
     r.kzfac = 2 * QL_PI / p.L[2] * (p.NZ / 3);
+    // MHD_FullQlin.cpp:92-97 (MHD_BQlin.cpp:70-75)
+    const double dealiasfac[3] = {2.0, 2.0, 3.0};
+    const int N[3] = {p.NX, p.NY, p.NZ};
+    r.kmax = 0;
+    for (int i = 0; i < 3; ++i)
+        if (r.kmax < 2 * QL_PI / p.L[i] * (N[i] / dealiasfac[i])) r.kmax = 2 * QL_PI / p.L[i] * (N[i] / dealiasfac[i]);
     const double kxL = 2 * QL_PI / p.L[0], kyL = 2 * QL_PI / p.L[1], kzL = 2 * QL_PI / p.L[2];
     const int nx = p.NX / 2, ny = p.NY / 2;
     r.h_kx0.assign(r.nxy, 0.0);
@@ -148,6 +156,9 @@ qlk::RunParams run_params(const RunInfo &r)
     rp.remap = r.p.remap;
     rp.nxy = r.nxy;
     rp.mode_off = r.mode_off;
+    rp.kmax = r.kmax;
+    rp.cfl = r.p.CFL;
+    rp.dtmax = r.p.dt < 0 ? -r.p.dt : r.p.dt;
     return rp;
 }
 
@@ -205,7 +216,9 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
     b.integrator = p0.integrator;
     b.precision = p0.precision;
     b.device = p0.device;
-    b.dt = p0.dt;
+    b.vdt = p0.dt < 0 ? 1 : 0;
+    b.dt = b.vdt ? -p0.dt : p0.dt;
+    b.dt_last = b.dt;
     b.nMF = (p0.model == QL_MODEL_MHD_BQLIN) ? 2 : 4;
     b.runs.resize(nruns);
     std::vector<int> nzs;
@@ -258,6 +271,7 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_RC(dalloc(&g.mfz, (size_t)g.nruns * 8 * g.NZ));
         QL_RC(dalloc(&g.sums, (size_t)g.nruns * b.nMF * g.NZ));
         QL_RC(dalloc(&g.rp, g.nruns));
+        QL_RC(dalloc(&g.pm, (size_t)g.nruns * qlk::PM_STRIDE));
         QL_RC(dalloc(&g.clock, 1));
         std::vector<double> kx0(g.modes), ky(g.modes);
         for (int id : g.run_ids) {
@@ -342,6 +356,7 @@ qlk::ModeArgs mode_args(const Batch &b, const Group &g)
     a.runs = g.rp;
     a.wg = g.wg;
     a.clock = g.clock;
+    a.vdt = b.vdt;
     const double dt = b.dt;
     a.dt = dt;
     a.dtP1 = dt * P1;
@@ -364,10 +379,11 @@ qlk::MFArgs mf_args(const Batch &b, const Group &g)
     m.kzv = g.kzv;
     m.tw = g.tw;
     m.clock = g.clock;
+    m.pm = g.pm;
+    m.vdt = b.vdt;
     m.model = b.model;
     m.nMF = b.nMF;
     m.fac = 1.0 / g.NZ;
-    m.dt = b.dt;
     return m;
 }
 
@@ -445,6 +461,7 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
     m.mfn_prev = (s == 1) ? nullptr : (s == 2 ? g.mfN1 : g.mfN2);
     m.mfn_out = (s == 2) ? g.mfN2 : g.mfN1;
     m.advance = (advance && s == 3) ? 1 : 0;
+    m.s = s;
     return enqueue_sequence(b, g, a, m, st);
 }
 
@@ -476,8 +493,18 @@ int set_clock(Batch &b, double t)
     c.t_prev = b.t_prev;
     c.t_init = b.t_init;
     c.old_init = b.old_init;
-    for (Group &g : b.groups) qlk::k_clock_set<<<1, 64, 0, b.s>>>(g.clock, c);
+    c.dt = b.dt;
+    c.dt_last = b.dt_last;
+    for (Group &g : b.groups) qlk::k_clock_set<<<1, 64, 0, b.s>>>(g.clock, c, g.pm, g.rp, b.model, b.vdt);
     QL_HIP(hipGetLastError());
+    return QL_OK;
+}
+
+// the host's view of the clock after variable steps
+int read_clock(Batch &b, qlk::Clock *c)
+{
+    QL_HIP(hipMemcpyAsync(c, b.groups[0].clock, sizeof(qlk::Clock), hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
 
@@ -518,13 +545,23 @@ int run_steps(Batch &b, int nsteps, double *t_inout)
             for (; left > 0; --left) QL_HIP(hipGraphLaunch(b.g1, b.s));
         }
     }
-    // the same accumulation as the device clock (main.cpp:116)
-    for (int n = 0; n < nsteps; ++n) {
-        b.t_prev = t;
-        t = t + b.dt;
+    QL_HIP(hipStreamSynchronize(b.s));
+    if (b.vdt) {
+        if (nsteps > 0) {
+            qlk::Clock c{};
+            QL_RC(read_clock(b, &c));
+            t = c.t;
+            b.t_prev = c.t_prev;
+            b.dt_last = c.dt_last;
+        }
+    } else {
+        // the same accumulation as the device clock (main.cpp:116)
+        for (int n = 0; n < nsteps; ++n) {
+            b.t_prev = t;
+            t = t + b.dt;
+        }
     }
     if (nsteps > 0) b.old_init = 0;
-    QL_HIP(hipStreamSynchronize(b.s));
     *t_inout = t;
     return QL_OK;
 }
@@ -662,7 +699,7 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
     // snapshot of what the trial steps overwrite
     struct Snap {
         double2 *u = nullptr, *mf = nullptr, *drive = nullptr;
-        double *kx0 = nullptr;
+        double *kx0 = nullptr, *pm = nullptr;
     };
     std::vector<Snap> snaps(b.groups.size());
     int rc = QL_OK;
@@ -670,10 +707,12 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
         Group &g = b.groups[gi];
         Snap &sn = snaps[gi];
         const size_t nl = (size_t)g.modes * 4 * g.nz, nm = (size_t)g.nruns * b.nMF * g.nz;
+        const size_t npm = (size_t)g.nruns * qlk::PM_STRIDE;
         if ((rc = dalloc(&sn.u, nl)) || (rc = dalloc(&sn.mf, nm)) || (rc = dalloc(&sn.drive, nm)) ||
-            (rc = dalloc(&sn.kx0, g.modes)))
+            (rc = dalloc(&sn.kx0, g.modes)) || (rc = dalloc(&sn.pm, npm)))
             break;
-        if (hipMemcpyAsync(sn.u, g.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+        if (hipMemcpyAsync(sn.pm, g.pm, sizeof(double) * npm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
+            hipMemcpyAsync(sn.u, g.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
             hipMemcpyAsync(sn.mf, g.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
             hipMemcpyAsync(sn.drive, g.drive, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
             hipMemcpyAsync(sn.kx0, g.kx0, sizeof(double) * g.modes, hipMemcpyDeviceToDevice, b.s) != hipSuccess)
@@ -735,10 +774,14 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
                 hipMemcpyAsync(g.kx0, sn.kx0, sizeof(double) * g.modes, hipMemcpyDeviceToDevice, b.s) != hipSuccess)
                 if (rc == QL_OK) rc = fail(QL_ERR_HIP, "tune: state restore failed");
             if (rc == QL_OK) rc = refresh_profiles(b, g, b.s);
+            // after the refresh, which rewrites the current-state maxima
+            if (sn.pm && hipMemcpyAsync(g.pm, sn.pm, sizeof(double) * g.nruns * qlk::PM_STRIDE,
+                                        hipMemcpyDeviceToDevice, b.s) != hipSuccess)
+                if (rc == QL_OK) rc = fail(QL_ERR_HIP, "tune: state restore failed");
         }
     }
     if (hipStreamSynchronize(b.s) != hipSuccess && rc == QL_OK) rc = fail(QL_ERR_HIP, "tune: synchronize failed");
-    for (Snap &sn : snaps) dfree(sn.u), dfree(sn.mf), dfree(sn.drive), dfree(sn.kx0);
+    for (Snap &sn : snaps) dfree(sn.u), dfree(sn.mf), dfree(sn.drive), dfree(sn.kx0), dfree(sn.pm);
     if (rc == QL_OK && best_ms_out) *best_ms_out = total_best;
     return rc;
 }
@@ -752,6 +795,8 @@ int init_batch(Batch &b, const ql_params *ps, int nruns, const char *who)
         if (c.model != a.model || c.integrator != a.integrator || c.precision != a.precision || c.device != a.device)
             return fail(QL_ERR_ARG, "%s: run %d: runs of a set share model, integrator, precision and device", who, r);
         if (c.dt != a.dt) return fail(QL_ERR_ARG, "%s: run %d: runs of a set share dt", who, r);
+        if (c.dt < 0 && nruns > 1)
+            return fail(QL_ERR_UNSUPPORTED, "%s: variable steps (dt < 0) need a set of one run", who);
         for (int q = 0; q < r; ++q)
             if (ps[q].NZ == c.NZ && ps[q].L[2] != c.L[2])
                 return fail(QL_ERR_ARG, "%s: runs %d and %d: runs of one NZ share Lz", who, q, r);
@@ -868,7 +913,14 @@ int ql_step(ql_ctx *c, double t, double *dt_out)
     b.t_prev = t;
     b.old_init = 0;
     QL_HIP(hipStreamSynchronize(b.s));
-    if (dt_out) *dt_out = b.dt;
+    double dt = b.dt;
+    if (b.vdt) {
+        qlk::Clock ck{};
+        QL_RC(read_clock(b, &ck));
+        dt = ck.dt;  // the step's (set_clock computed it; no advance here)
+    }
+    b.dt_last = dt;
+    if (dt_out) *dt_out = dt;
     return QL_OK;
 }
 
@@ -921,6 +973,18 @@ int ql_mf_real(ql_ctx *c, double *out)
     QL_HIP(hipMemcpyAsync(out, b.mfr_tmp, sizeof(double) * b.nMF * g.NZ, hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
+}
+
+int ql_calculate_cfl(ql_ctx *c, double *cfl_over_dt)
+{
+    if (!c || !cfl_over_dt) return fail(QL_ERR_ARG, "ql_calculate_cfl: null argument");
+    return ql_batch_calculate_cfl(&c->bt, 0, cfl_over_dt);
+}
+
+int ql_check_solution(ql_ctx *c, double sums[5], int *verdict)
+{
+    if (!c) return fail(QL_ERR_ARG, "ql_check_solution: null context");
+    return ql_batch_check_solution(&c->bt, 0, sums, verdict);
 }
 
 int ql_synchronize(ql_ctx *c)
@@ -1044,6 +1108,51 @@ int ql_batch_get_drive(ql_batch *bt, int run, double *drive)
     QL_HIP(hipMemcpyAsync(drive, g.drive + (size_t)r.gi * b.nMF * g.nz, sizeof(double2) * b.nMF * g.nz,
                           hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    return QL_OK;
+}
+
+int ql_batch_calculate_cfl(ql_batch *bt, int run, double *cfl_over_dt)
+{
+    if (!bt || !cfl_over_dt) return fail(QL_ERR_ARG, "ql_batch_calculate_cfl: null argument");
+    QL_RC(check_run(bt->b, run, "ql_batch_calculate_cfl"));
+    Batch &b = bt->b;
+    const RunInfo &r = b.runs[run];
+    Group &g = b.groups[r.group];
+    double pm[qlk::PM_STRIDE];
+    QL_HIP(hipMemcpyAsync(pm, g.pm + (size_t)r.gi * qlk::PM_STRIDE, sizeof(pm), hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
+    *cfl_over_dt = qlk::cfl_value(pm, g.h_rp[r.gi], b.model);
+    return QL_OK;
+}
+
+int ql_batch_check_solution(ql_batch *bt, int run, double sums[5], int *verdict)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_check_solution: null handle");
+    QL_RC(check_run(bt->b, run, "ql_batch_check_solution"));
+    Batch &b = bt->b;
+    const RunInfo &r = b.runs[run];
+    Group &g = b.groups[r.group];
+    const double2 *mf = g.mf + (size_t)r.gi * b.nMF * g.nz;
+    switch (g.NZ) {
+    case 32: qlk::k_check_sums<32><<<1, qlk::NT, 0, b.s>>>(mf, g.tw, b.dout); break;
+    case 64: qlk::k_check_sums<64><<<1, qlk::NT, 0, b.s>>>(mf, g.tw, b.dout); break;
+    case 128: qlk::k_check_sums<128><<<1, qlk::NT, 0, b.s>>>(mf, g.tw, b.dout); break;
+    default: qlk::k_check_sums<256><<<1, qlk::NT, 0, b.s>>>(mf, g.tw, b.dout); break;
+    }
+    QL_HIP(hipGetLastError());
+    double sm[5];
+    QL_HIP(hipMemcpyAsync(sm, b.dout, sizeof(sm), hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
+    if (sums) std::memcpy(sums, sm, sizeof(sm));
+    // solution.cpp:174, :184, :194, in that order
+    int v = 0;
+    if (sm[0] / sm[1] < 1e10 && sm[0] > 1e-2) v = 1;
+    else if (sm[2] > 1e20 || sm[3] > 1e20) v = 2;
+    else if (!std::isfinite(sm[4])) v = 3;
+    if (verdict) *verdict = v;
+    if (v == 1) return fail(QL_ERR_STATE, "run %d: ERROR: Mean field has developed an imaginary part!!", run);
+    if (v == 2) return fail(QL_ERR_STATE, "run %d: ERROR: Solution is very large, probably unstable!!", run);
+    if (v == 3) return fail(QL_ERR_STATE, "run %d: ERROR: Solution contains NaN or Inf!!", run);
     return QL_OK;
 }
 

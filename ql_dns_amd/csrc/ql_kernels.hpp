@@ -23,6 +23,16 @@ namespace qlk {
 
 enum { MODE_RHS = 0, MODE_ELIN = 1, MODE_CN = 2, MODE_PROFILES = 3 };
 
+// Integrator coefficients (Integrators/RK3CN.cpp:19-20, RK3CN_Elin.cpp:19-20)
+constexpr double A0 = 8.0 / 15.0, A1 = 5.0 / 12.0, A2 = 3.0 / 4.0;
+constexpr double B1 = -17.0 / 60.0, B2 = -5.0 / 12.0;
+constexpr double P1 = 8.0 / 15.0, P2 = 2.0 / 3.0;
+
+// per-run record of the mean-field profile maxima that variable steps use:
+// [0, 8) max |profile line| of the current state, [8, 16) those of the
+// profiles the last right-hand side read (the reference's MBx_ ... arrays)
+constexpr int PM_CUR = 0, PM_RHS = 8, PM_STRIDE = 16;
+
 // physical parameters of one run
 struct Phys {
     double q, omega, B0z, nu, eta;
@@ -39,6 +49,8 @@ struct RunParams {
     int nxy;             // modes of the run
     int mode_off;        // first mode of the run in the group arrays
     int wg_off, nwg;     // its mode-kernel workgroups = its partial rows
+    double kmax;         // CFL wavenumber (MHD_FullQlin.cpp:92-97)
+    double cfl, dtmax;   // CFL number and -dt of variable steps (RK3CN.cpp:20-30)
 };
 
 // one mode-kernel workgroup: its run, its mode-group range [g0, g1), and the
@@ -49,10 +61,12 @@ struct WgInfo {
 
 // time of a set of runs stepped together: t as main.cpp:116 accumulates it,
 // t_prev the previous step start; old_init: the RK3CN "old" operator is the
-// one of linearOPs_Init at t_init (RK3CN.cpp:57)
+// one of linearOPs_Init at t_init (RK3CN.cpp:57); dt the step about to be
+// taken, dt_last the one taken last
 struct Clock {
     double t, t_prev, t_init;
     int old_init, pad;
+    double dt, dt_last;
 };
 
 struct ModeArgs {
@@ -72,46 +86,95 @@ struct ModeArgs {
     const Clock *clock;
     int s;                  // substep 1..3, or 0 for the explicit times below
     int mode, do_remap;
+    int vdt;                // variable steps: dt and its coefficients from the clock
     double ts_x, dtl_x;     // explicit rhs time and dt_lin (s == 0)
     double dt, linC, nlc1, nlc2;
     double dtP1, dtP2, dtl1, dtl2, dtl3;  // dt*8/15, dt*2/3, and the three dt_lin
     double fac;             // 1/NZ
 };
 
+// substep coefficients of RK3CN_Elin::Step (RK3CN_Elin.cpp:88-90, :118-121,
+// :151-154) and RK3CN::Step (linC with dt/2, RK3CN.cpp:92-95, :130-133,
+// :166-169), the host's products rounded one by one
+struct StepCoefs {
+    double linC, nlc1, nlc2;
+};
+__device__ __forceinline__ StepCoefs step_coefs(double dt, int s, bool cn)
+{
+    const double h = cn ? dt * 0.5 : dt;
+    const double a = s == 1 ? A0 : (s == 2 ? A1 : A2);
+    const double ab = s == 1 ? A0 : (s == 2 ? A1 + B1 : A2 + B2);
+    const double b = s == 1 ? 0.0 : (s == 2 ? B1 : B2);
+    return StepCoefs{__dmul_rn(h, ab), __dmul_rn(a, dt), __dmul_rn(b, dt)};
+}
+
 // substep times from the clock, with the host's roundings (unfused): ts,
-// dt_lin, where the RK3CN "old" operator was evaluated, and t + dt
+// dt_lin, where the RK3CN "old" operator was evaluated, and t + dt; with
+// variable steps also the substep's coefficients (fixed steps take them from
+// the arguments)
 struct SubTimes {
     double ts, dtl, tso, dtlo, tnew;
+    double linC, nlc1, nlc2;
     bool oinit;
 };
+template <bool CN>
 __device__ __forceinline__ SubTimes sub_times(const ModeArgs &a)
 {
-    SubTimes r{};
-    if (a.s == 0) {
-        r.ts = a.ts_x;
-        r.dtl = a.dtl_x;
-        return r;
-    }
+    // scalars first, the struct once at the end (partial stores of a local
+    // struct in separate branches can leave it in scratch memory)
+    if (a.s == 0) return SubTimes{a.ts_x, a.dtl_x, 0.0, 0.0, 0.0, a.linC, a.nlc1, a.nlc2, false};
     const double t = a.clock->t;
-    r.tnew = __dadd_rn(t, a.dt);
-    if (a.s == 1) {
-        r.ts = t;
-        r.dtl = a.dtl1;
-        r.oinit = a.clock->old_init != 0;
-        r.tso = r.oinit ? a.clock->t_init : __dadd_rn(a.clock->t_prev, a.dtP2);
-        r.dtlo = r.oinit ? 0.0 : a.dtl3;
-    } else if (a.s == 2) {
-        r.ts = __dadd_rn(t, a.dtP1);
-        r.dtl = a.dtl2;
-        r.tso = t;
-        r.dtlo = a.dtl1;
-    } else {
-        r.ts = __dadd_rn(t, a.dtP2);
-        r.dtl = a.dtl3;
-        r.tso = __dadd_rn(t, a.dtP1);
-        r.dtlo = a.dtl2;
+    double dt = a.dt, dtP1 = a.dtP1, dtP2 = a.dtP2, dtl2 = a.dtl2, dtl3 = a.dtl3;
+    double dtP2o = a.dtP2, dtl3o = a.dtl3;  // the previous step's (old operator of substep 1)
+    StepCoefs c{a.linC, a.nlc1, a.nlc2};
+    if (a.vdt) {
+        dt = a.clock->dt;
+        dtP1 = __dmul_rn(dt, P1);
+        dtP2 = __dmul_rn(dt, P2);
+        dtl2 = __dmul_rn(dt, P2 - P1);
+        dtl3 = __dmul_rn(dt, 1.0 - P2);
+        const double dl = a.clock->dt_last;
+        dtP2o = __dmul_rn(dl, P2);
+        dtl3o = __dmul_rn(dl, 1.0 - P2);
+        c = step_coefs(dt, a.s, CN);
     }
-    return r;
+    const double dtl1 = dtP1;  // dt * p1 (RK3CN_Elin.cpp:90)
+    const double tnew = __dadd_rn(t, dt);
+    const bool s1 = a.s == 1, s2 = a.s == 2;
+    const bool oinit = s1 && a.clock->old_init != 0;
+    const double ts = s1 ? t : __dadd_rn(t, s2 ? dtP1 : dtP2);
+    const double dtl = s1 ? dtl1 : (s2 ? dtl2 : dtl3);
+    const double tso = s1 ? (oinit ? a.clock->t_init : __dadd_rn(a.clock->t_prev, dtP2o))
+                          : (s2 ? t : __dadd_rn(t, dtP1));
+    const double dtlo = s1 ? (oinit ? 0.0 : dtl3o) : (s2 ? dtl1 : dtl2);
+    return SubTimes{ts, dtl, tso, dtlo, tnew, c.linC, c.nlc1, c.nlc2, oinit};
+}
+
+// dt of the next variable step (RK3CN_Elin.cpp:76-80) from the recorded
+// profile maxima: CFL / Calculate_CFL, at most dtmax.  Calculate_CFL is
+// kmax (max|By| + max|Bx| + max|Ux| + max|Uy|) + q over the profiles of the
+// last rhs (MHD_FullQlin.cpp:773-778), or kmax max|By| + kmax max|Bx| + q with
+// By of the last rhs and Bx of the current state (MHD_BQlin.cpp:544-550).
+// Profile lines: Bx 0, By 2, Ux 4, Uy 6 (MHD_FullQlin); By 0, Bx 3 (MHD_BQlin).
+__host__ __device__ inline double cfl_value(const double *pm, const RunParams &rp, int model)
+{
+#pragma clang fp contract(off)
+    const double *cur = pm + PM_CUR, *rhs = pm + PM_RHS;
+    if (model == 0) {
+        const double s = ((rhs[2] + rhs[0]) + rhs[4]) + rhs[6];
+        const double k = rp.kmax * s;
+        return k + rp.ph.q;
+    }
+    const double ky = rp.kmax * rhs[0], kx = rp.kmax * cur[3];
+    const double s = ky + kx;
+    return s + rp.ph.q;
+}
+__host__ __device__ inline double cfl_dt(const double *pm, const RunParams &rp, int model)
+{
+#pragma clang fp contract(off)
+    double dt = rp.cfl / cfl_value(pm, rp, model);
+    if (dt > rp.dtmax) dt = rp.dtmax;
+    return dt;
 }
 
 // -|x|^(order/2): the dissipation exponent uses integer division
@@ -132,18 +195,24 @@ __device__ __forceinline__ bool remap_hit(double qt, double thr, double kx0, dou
     return __dadd_rn(kx0, __dmul_rn(qt, ky)) > thr;
 }
 
-// the host's view of the clock, passed by value (no host buffer involved)
-[[maybe_unused]] static __global__ void k_clock_set(Clock *c, Clock v)
+// the host's view of the clock, passed by value (no host buffer involved);
+// with variable steps dt is the next step's, from run 0's profile maxima
+[[maybe_unused]] static __global__ void k_clock_set(Clock *c, Clock v, const double *pm, const RunParams *rp,
+                                                    int model, int vdt)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *c = v;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (vdt) v.dt = cfl_dt(pm, rp[0], model);
+        *c = v;
+    }
 }
 
 // t_prev <- t, t <- t + dt (main.cpp:116), once per step
-__device__ __forceinline__ void clock_advance(Clock *c, double dt)
+__device__ __forceinline__ void clock_advance(Clock *c)
 {
-    const double t = c->t;
+    const double t = c->t, dt = c->dt;
     c->t_prev = t;
     c->t = __dadd_rn(t, dt);
+    c->dt_last = dt;
     c->old_init = 0;
 }
 
@@ -209,8 +278,10 @@ struct MFArgs {
     const double *kzv;
     const double2 *tw;
     Clock *clock;  // advanced by dt when advance != 0 (last substep of a step)
+    double *pm;    // profile maxima [runs][PM_STRIDE]
     int model, nMF, mode, advance;
-    double fac, linC, nlc1, nlc2, dt;
+    int s, vdt;    // substep; variable steps (coefficients from the clock)
+    double fac, linC, nlc1, nlc2;
 };
 
 template <int NZ>
@@ -227,8 +298,17 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     __shared__ double2 smfn[4 * NZD];   // updated mean fields
     const int tid = threadIdx.x;
     const int run = blockIdx.x;
-    // the substep's mode kernel, the only reader of the clock, has finished
-    if (a.advance && run == 0 && tid == 0) clock_advance(a.clock, a.dt);
+    double *pm = a.pm + (size_t)run * PM_STRIDE;
+    StepCoefs cf{a.linC, a.nlc1, a.nlc2};
+    if (a.vdt && a.mode != MODE_PROFILES && a.s > 0) cf = step_coefs(a.clock->dt, a.s, a.mode == MODE_CN);
+    const double linC = cf.linC, nlc1 = cf.nlc1, nlc2 = cf.nlc2;
+    // the right-hand side of this substep read the profiles of the current
+    // state (Calculate_CFL reads those, MHD_FullQlin.cpp:772)
+    if (a.mode != MODE_PROFILES && tid < 8) pm[PM_RHS + tid] = pm[PM_CUR + tid];
+    // the substep's mode kernel, the only reader of the clock, has finished;
+    // dt is read above, before this block can change it (the next step's dt is
+    // written at the end)
+    if (a.advance && run == 0 && tid == 0) clock_advance(a.clock);
     const Phys ph = a.runs[run].ph;
     const int nMF = a.nMF, nslot = nMF * NZD;
     const size_t mo = (size_t)run * nslot;
@@ -299,14 +379,14 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
             const double2 M = smf[k];
             double2 Mn;
             if (a.mode == MODE_ELIN) {
-                double2 sacc = cscale(a.nlc1, N);
-                if (a.mfn_prev) sacc = cadd(sacc, cscale(a.nlc2, pprev[w]));
-                Mn = cscale(exp(a.linC * Lmf), cadd(sacc, M));
+                double2 sacc = cscale(nlc1, N);
+                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2, pprev[w]));
+                Mn = cscale(exp(linC * Lmf), cadd(sacc, M));
             } else {
-                const double denom = 1.0 / (1.0 - a.linC * Lmf);
-                double2 sacc = cscale(a.nlc1 * denom, N);
-                if (a.mfn_prev) sacc = cadd(sacc, cscale(a.nlc2 * denom, pprev[w]));
-                Mn = cadd(sacc, cscale((1 + a.linC * Lmf) * denom, M));
+                const double denom = 1.0 / (1.0 - linC * Lmf);
+                double2 sacc = cscale(nlc1 * denom, N);
+                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2 * denom, pprev[w]));
+                Mn = cadd(sacc, cscale((1 + linC * Lmf) * denom, M));
             }
             a.mf[mo + k] = Mn;
             smfn[k] = Mn;
@@ -318,20 +398,22 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
         __syncthreads();
     }
     // real-space profiles for the next rhs (MHD_FullQlin.cpp:180-198,
-    // MHD_BQlin.cpp:122-127): fac1D * ifft(pad(...)), real part
+    // MHD_BQlin.cpp:122-127): fac1D * ifft(pad(...)), real part; for
+    // MHD_BQlin line 3 is Bx, which only the CFL condition reads
+    // (MHD_BQlin.cpp:546-548)
     const int nlines = (a.model == 0) ? 8 : 3;
     for (int idx = tid; idx < 8 * NZ; idx += NT) {
         const int l = idx / NZ, j = idx % NZ;
         double2 v = make_double2(0.0, 0.0);
-        if (l < nlines && D::kept(j)) {
+        if (D::kept(j)) {
             const int m = D::slot(j);
             const double kz = skz[m];
             if (a.model == 0) {
                 const double2 M = smfn[(l >> 1) * NZD + m];
                 v = (l & 1) ? imul(a.fac * kz, M) : cscale(a.fac, M);
-            } else {
-                const double2 M = smfn[NZD + m];  // By
-                if (l == 0) v = cscale(a.fac, M);
+            } else if (l < 4) {
+                const double2 M = smfn[(l == 3 ? 0 : NZD) + m];  // By, or Bx for line 3
+                if (l == 0 || l == 3) v = cscale(a.fac, M);
                 else if (l == 1) v = imul(a.fac * kz, M);
                 else v = cscale(a.fac * (-(kz * kz)), M);
             }
@@ -341,6 +423,25 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     __syncthreads();
     fft_lines<NZ, 8, true>(buf, tw);
     for (int idx = tid; idx < nlines * NZ; idx += NT) a.mfz[(size_t)run * 8 * NZ + idx] = buf[idx].x;
+    // max |profile| per line, sqrt(max |z|^2) as Eigen's abs2().maxCoeff()
+    // (MHD_FullQlin.cpp:773-776): 32 threads per line, fixed order
+    {
+        static_assert(NT == 8 * 32, "one 32-lane group per profile line");
+        const int l = tid / 32, j0 = tid % 32;
+        double m2 = 0.0;
+        for (int j = j0; j < NZ; j += 32) {
+            const double2 v = buf[l * NZ + j];
+            m2 = fmax(m2, __dadd_rn(__dmul_rn(v.x, v.x), __dmul_rn(v.y, v.y)));
+        }
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o, 32));
+        if (j0 == 0) pm[PM_CUR + l] = sqrt(m2);
+    }
+    if (a.vdt && a.advance && run == 0) {
+        __syncthreads();
+        // the next step's dt (RK3CN_Elin.cpp:76-80)
+        if (tid == 0) a.clock->dt = cfl_dt(pm, a.runs[0], a.model);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -483,6 +584,50 @@ __global__ __launch_bounds__(NT) void k_mf_real(const double2 *mf, int nMF, cons
     __syncthreads();
     fft_lines<NZ, 4, true>(buf, tw);
     for (int idx = threadIdx.x; idx < nMF * NZ; idx += NT) out[idx] = fac * buf[idx].x;
+}
+
+// solution::Check_Solution (solution.cpp:169-203) reductions over one run's
+// mean fields: out[0] = sum |Re ifft(pad(By))|, out[1] = sum |Im ...| (the
+// transform unnormalised, as fftwPlans::inverse), out[2] = max |By|,
+// out[3] = max |Bx|, out[4] = sum |By| over the dealiased coefficients.
+// One workgroup; sums in a fixed order.
+template <int NZ>
+__global__ __launch_bounds__(NT) void k_check_sums(const double2 *mf, const double2 *twg, double *out)
+{
+    using D = Dealias<NZ>;
+    __shared__ double2 buf[NZ];
+    __shared__ double2 tw[NZ];
+    __shared__ double red[5][NT];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < NZ; k += NT) tw[k] = twg[k];
+    for (int j = tid; j < NZ; j += NT) buf[j] = D::kept(j) ? mf[D::nz + D::slot(j)] : make_double2(0.0, 0.0);
+    __syncthreads();
+    fft_lines<NZ, 1, true>(buf, tw);
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = tid; j < NZ; j += NT) {
+        s[0] += fabs(buf[j].x);
+        s[1] += fabs(buf[j].y);
+    }
+    for (int m = tid; m < D::nz; m += NT) {
+        const double2 by = mf[D::nz + m], bx = mf[m];
+        const double aby = hypot(by.x, by.y);
+        s[2] = fmax(s[2], aby);
+        s[3] = fmax(s[3], hypot(bx.x, bx.y));
+        s[4] += aby;
+    }
+    for (int c = 0; c < 5; ++c) red[c][tid] = s[c];
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[0][tid] += red[0][tid + w];
+            red[1][tid] += red[1][tid + w];
+            red[2][tid] = fmax(red[2][tid], red[2][tid + w]);
+            red[3][tid] = fmax(red[3][tid], red[3][tid + w]);
+            red[4][tid] += red[4][tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid < 5) out[tid] = red[tid][0];
 }
 
 }  // namespace qlk

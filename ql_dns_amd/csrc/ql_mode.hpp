@@ -342,7 +342,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     const RunParams rp = a.runs[run];
     const Phys ph = rp.ph;
     // q t products: wave-uniform, kept in scalar registers
-    const SubTimes st = sub_times(a);
+    const SubTimes st = sub_times<MODE == MODE_CN>(a);
+    const double linC = to_sgpr(st.linC), nlc1 = to_sgpr(st.nlc1), nlc2 = to_sgpr(st.nlc2);
     const double qt = to_sgpr(__dmul_rn(ph.q, st.ts)), qdtl = to_sgpr(__dmul_rn(ph.q, st.dtl));
     const double qt_old = to_sgpr(__dmul_rn(ph.q, st.tso)), qdtl_old = to_sgpr(__dmul_rn(ph.q, st.dtlo));
     const double remap_qt = to_sgpr(__dmul_rn(ph.q, st.tnew));
@@ -485,11 +486,11 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                 double2 un0, un1;
                 if constexpr (MODE == MODE_ELIN) {
                     // RK3CN_Elin.cpp:303-305, :335-337, :368-370
-                    const double e = exp(a.linC * Ln);
-                    double2 s0 = cscale(a.nlc1, N0), s1 = cscale(a.nlc1, N1);
+                    const double e = exp(linC * Ln);
+                    double2 s0 = cscale(nlc1, N0), s1 = cscale(nlc1, N1);
                     if (a.n_prev) {
-                        s0 = cadd(s0, cscale(a.nlc2, p0));
-                        s1 = cadd(s1, cscale(a.nlc2, p1));
+                        s0 = cadd(s0, cscale(nlc2, p0));
+                        s1 = cadd(s1, cscale(nlc2, p1));
                     }
                     un0 = cscale(e, cadd(s0, u0));
                     un1 = cscale(e, cadd(s1, u1));
@@ -504,11 +505,11 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                     } else {
                         Lo = P::lin(pr, kxo + qdtl_old * ky, ky, kz2, ph);
                     }
-                    const double denom = 1.0 / (1.0 - a.linC * Ln);
-                    const double c1 = a.nlc1 * denom, co = (1 + a.linC * Lo) * denom;
+                    const double denom = 1.0 / (1.0 - linC * Ln);
+                    const double c1 = nlc1 * denom, co = (1 + linC * Lo) * denom;
                     double2 s0 = cscale(c1, N0), s1 = cscale(c1, N1);
                     if (a.n_prev) {
-                        const double c2 = a.nlc2 * denom;
+                        const double c2 = nlc2 * denom;
                         s0 = cadd(s0, cscale(c2, p0));
                         s1 = cadd(s1, cscale(c2, p1));
                     }

@@ -33,10 +33,8 @@ int fail(int code, const char *fmt, ...);
         if (rc_ != QL_OK) return rc_; \
     } while (0)
 
-// Integrator coefficients (Integrators/RK3CN.cpp:19-20, RK3CN_Elin.cpp:19-20)
-constexpr double A0 = 8.0 / 15.0, A1 = 5.0 / 12.0, A2 = 3.0 / 4.0;
-constexpr double B1 = -17.0 / 60.0, B2 = -5.0 / 12.0;
-constexpr double P1 = 8.0 / 15.0, P2 = 2.0 / 3.0;
+// integrator coefficients, shared with the kernels
+using qlk::A0, qlk::A1, qlk::A2, qlk::B1, qlk::B2, qlk::P1, qlk::P2;
 // General_Definitions.h:236: PI = atan(1)*4, which rounds to this double
 constexpr double QL_PI = 3.14159265358979323846;
 
@@ -56,7 +54,7 @@ struct RunInfo {
     ql_params p{};
     int group = 0, gi = 0, mode_off = 0;
     int nxy = 0, nz = 0, nzl = 0, half = 0;
-    double omega_eff = 0, rsf = 0, remap_thr = 0, remap_shift = 0, totalN2 = 0, kzfac = 0;
+    double omega_eff = 0, rsf = 0, remap_thr = 0, remap_shift = 0, totalN2 = 0, kzfac = 0, kmax = 0;
     std::vector<double> h_kx0, h_ky, h_kz;
 };
 
@@ -73,6 +71,7 @@ struct Group {
     float2 *tw32 = nullptr;
     double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr, *part = nullptr, *sums = nullptr;
     double *lin_tmp = nullptr;
+    double *pm = nullptr;  // profile maxima of the CFL condition [runs][PM_STRIDE]
     qlk::RunParams *rp = nullptr;
     qlk::WgInfo *wg = nullptr;
     qlk::Clock *clock = nullptr;  // the group's step times
@@ -83,7 +82,9 @@ struct AdjState;  // adjoint checkpoints and buffers (ql_adjoint.hip)
 
 struct Batch {
     int model = 0, integrator = 0, precision = 0, device = 0, nMF = 4;
-    double dt = 0;
+    double dt = 0;   // the fixed step, or the largest variable one
+    int vdt = 0;     // variable steps (RK3CN.cpp:20-30): dt and coefficients on the device
+    double dt_last = 0;  // the step taken last
     std::vector<RunInfo> runs;
     std::vector<Group> groups;
     hipStream_t s = nullptr;

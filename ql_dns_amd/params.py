@@ -38,7 +38,8 @@ class QLParams:
     viscosity_order: int = 2
     quasilinear: int = 1
     remap: int = 1
-    dt: float = 0.01
+    dt: float = 0.01      # < 0: variable steps of at most -dt
+    CFL: float = 1.0      # CFL number of variable steps (Input_parameters.cpp:77)
     model: int = MODEL_MHD_FULLQLIN
     integrator: int = INT_RK3CN_ELIN
     precision: int = PREC_FP64
@@ -79,7 +80,7 @@ class QLParams:
                     NZ=self.NZ, L=(self.Lx, self.Ly, self.Lz), q=self.q, omega=self.omega,
                     nu=self.nu, eta=self.eta, B0z=self.B0z,
                     viscosity_order=self.viscosity_order, quasilinear=self.quasilinear,
-                    remap=self.remap, dt=self.dt)
+                    remap=self.remap, dt=self.dt, CFL=self.CFL)
 
 
 def _re(Re: float, Rm: float) -> dict:

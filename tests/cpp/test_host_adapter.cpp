@@ -3,8 +3,9 @@
 // integrator construction, Calc_Energy_AM_Diss, the Step / t += dt / remap
 // loop), plus one Model::rhs call.
 //
-// usage: test_host_adapter <model> <integrator> <nsteps> <state.bin> <out.bin>
-// box: 16x16x32, L = (1, 4, 1), q = 1.5, Omega = 1, nu = eta = 1/2000, dt = 0.01.
+// usage: test_host_adapter <model> <integrator> <nsteps> <state.bin> <out.bin> [dt CFL]
+// box: 16x16x32, L = (1, 4, 1), q = 1.5, Omega = 1, nu = eta = 1/2000, dt = 0.01
+// unless given (dt < 0: variable steps of at most -dt, RK3CN.cpp:20-30).
 // state.bin: lin [nxy][4][nz] then mf [nMF][nz], complex128.
 // out.bin: rhs_lin, rhs_mf, linop (from the initial state at t = 0.0, dt_lin = 0.0053),
 //          then lin, mf, kx, t, and 18 diagnostics after nsteps.
@@ -18,8 +19,8 @@ using namespace qlgpu;
 
 int main(int argc, char **argv)
 {
-    if (argc != 6) {
-        std::fprintf(stderr, "usage: %s model integrator nsteps state.bin out.bin\n", argv[0]);
+    if (argc != 6 && argc != 8) {
+        std::fprintf(stderr, "usage: %s model integrator nsteps state.bin out.bin [dt CFL]\n", argv[0]);
         return 2;
     }
     try {
@@ -34,7 +35,8 @@ int main(int argc, char **argv)
         SP.L[1] = 4.0;
         SP.L[2] = 1.0;
         SP.nu = SP.eta = 1.0 / 2000;
-        SP.dt = 0.01;
+        SP.dt = argc == 8 ? std::atof(argv[6]) : 0.01;
+        SP.CFL = argc == 8 ? std::atof(argv[7]) : 1.0;
         SP.q = 1.5;
         SP.omega = 1.0;
 
@@ -76,6 +78,7 @@ int main(int argc, char **argv)
             if (SP.remapQ) fluidEqs->ShearingBox_Remap(SP.q * t, sol);
         }
         fluidEqs->Calc_Energy_AM_Diss(&tv, t, sol);
+        sol->Check_Solution();  // main.cpp:136; throws where the reference aborts
 
         for (int i = 0; i < nxy; ++i)
             for (int j = 0; j < 4; ++j) std::fwrite(sol->pLin(i, j)->data(), sizeof(dcmplx), nz, fo);

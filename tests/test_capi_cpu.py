@@ -26,16 +26,17 @@ def test_library_exports_all_symbols():
         assert hasattr(L, name), name
 
 
-@pytest.mark.parametrize("field,value,code", [
-    ("NZ", 48, -1), ("NZ", 512, -1), ("NX", 7, -1), ("NY", 0, -1), ("model", 5, -1),
-    ("integrator", 3, -1), ("precision", 7, -3), ("dt", -0.01, -3), ("dt", 0.0, -3),
-    ("viscosity_order", -2, -1)])
-def test_create_rejects_invalid_params(field, value, code):
+@pytest.mark.parametrize("over,code", [
+    (dict(NZ=48), -1), (dict(NZ=512), -1), (dict(NX=7), -1), (dict(NY=0), -1), (dict(model=5), -1),
+    (dict(integrator=3), -1), (dict(precision=7), -3), (dict(dt=0.0), -1), (dict(dt=float("nan")), -1),
+    (dict(dt=-0.01, CFL=0.0), -1), (dict(dt=-0.01, CFL=float("inf")), -1),
+    (dict(viscosity_order=-2), -1)])
+def test_create_rejects_invalid_params(over, code):
     import ctypes
     from ql_dns_amd import capi
     from ql_dns_amd.params import QLParams
     L = capi.load()
-    p = QLParams().replace(**{field: value})
+    p = QLParams().replace(**over)
     cp = capi.to_c_params(p)
     h = ctypes.c_void_p()
     rc = L.ql_create(ctypes.byref(cp), ctypes.byref(h))
@@ -71,7 +72,7 @@ def _create_set(params):
     (dict(precision=1), -1),             # ... and precision
     (dict(Lz=2.0), -1),                  # runs of one NZ share Lz
     (dict(NZ=96), -1),                   # each run is checked like ql_create
-    (dict(dt=-1.0), -3)])
+    (dict(dt=-1.0), -1)])
 def test_set_create_rejects_inconsistent_runs(second, code):
     from ql_dns_amd.params import QLParams
     p = QLParams()
@@ -79,6 +80,15 @@ def test_set_create_rejects_inconsistent_runs(second, code):
     assert rc == code
     assert h.value is None
     assert len(L.ql_last_error()) > 0
+
+
+def test_variable_steps_need_a_set_of_one_run():
+    from ql_dns_amd.params import QLParams
+    p = QLParams(dt=-0.01)
+    rc, h, L = _create_set([p, p.replace(NZ=64)])
+    assert rc == -3
+    assert h.value is None
+    assert b"one run" in L.ql_last_error()
 
 
 def test_set_null_arguments_are_errors():
@@ -97,3 +107,8 @@ def test_set_null_arguments_are_errors():
     assert L.ql_batch_size(None, ctypes.byref(n)) == -1
     assert L.ql_adjoint_forward(None, 1, 1, None) == -1
     assert L.ql_adjoint_backward(None, None, None, None, None, None) == -1
+    v = ctypes.c_double(0.0)
+    assert L.ql_calculate_cfl(None, ctypes.byref(v)) == -1
+    assert L.ql_check_solution(None, None, None) == -1
+    assert L.ql_batch_calculate_cfl(None, 0, ctypes.byref(v)) == -1
+    assert L.ql_batch_check_solution(None, 0, None, None) == -1

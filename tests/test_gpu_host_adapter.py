@@ -13,13 +13,17 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(ROOT, "tests", "cpp", "test_host_adapter")
 
 
-@pytest.mark.parametrize("model,integrator", [("MHD_FullQlin", "RK3CN_Elin"), ("MHD_FullQlin", "RK3CN"),
-                                              ("MHD_BQlin", "RK3CN_Elin")])
-def test_main_loop_through_cpp_adapter(tmp_path, model, integrator):
+@pytest.mark.parametrize("model,integrator,dt,cfl", [
+    ("MHD_FullQlin", "RK3CN_Elin", 0.01, 1.0), ("MHD_FullQlin", "RK3CN", 0.01, 1.0),
+    ("MHD_BQlin", "RK3CN_Elin", 0.01, 1.0),
+    ("MHD_FullQlin", "RK3CN_Elin", -0.05, 0.5), ("MHD_FullQlin", "RK3CN", -0.05, 0.5),
+    ("MHD_BQlin", "RK3CN_Elin", -0.05, 0.5)])
+def test_main_loop_through_cpp_adapter(tmp_path, model, integrator, dt, cfl):
     from ql_dns_amd.ic import initial_state
     from ql_dns_amd.params import INTEGRATOR_NAMES, MODEL_NAMES, workload
     from oracle.oracle import Oracle
-    p = workload("box16x16x32", model=MODEL_NAMES[model], integrator=INTEGRATOR_NAMES[integrator])
+    p = workload("box16x16x32", model=MODEL_NAMES[model], integrator=INTEGRATOR_NAMES[integrator], dt=dt,
+                 CFL=cfl)
     lin, mf = initial_state(p)
     src = tmp_path / "state.bin"
     out = tmp_path / "out.bin"
@@ -27,8 +31,8 @@ def test_main_loop_through_cpp_adapter(tmp_path, model, integrator):
         f.write(np.ascontiguousarray(lin).tobytes())
         f.write(np.ascontiguousarray(mf).tobytes())
     nsteps = 50
-    r = subprocess.run([BIN, model, integrator, str(nsteps), str(src), str(out)], capture_output=True,
-                       text=True, timeout=120)
+    r = subprocess.run([BIN, model, integrator, str(nsteps), str(src), str(out), repr(dt), repr(cfl)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = np.fromfile(out, dtype=np.float64)
     nl, nm = p.nxy * 4 * p.nz, p.nMF * p.nz
@@ -53,7 +57,11 @@ def test_main_loop_through_cpp_adapter(tmp_path, model, integrator):
     assert rel_err(linop, ro[2].ravel()) < 1e-10
     to = o.steps(nsteps, 0.0)
     lo, mo, ko = o.get_state()
-    assert t == to
+    if dt > 0:
+        assert t == to
+    else:
+        # each step's dt comes from transformed profiles: equal to rounding
+        assert abs(t - to) < 1e-12 * to and t < nsteps * -dt
     assert np.array_equal(kx, ko)
     assert rel_err(fl, lo.ravel()) < 1e-10
     assert rel_err(fm, mo.ravel()) < 1e-10
