@@ -131,6 +131,30 @@ int ql_calculate_cfl(ql_ctx *ctx, double *cfl_over_dt);
  * returns QL_ERR_STATE with the reference's message (where it aborts). */
 int ql_check_solution(ql_ctx *ctx, double sums[5], int *verdict);
 
+/* Driving noise of a run, as the Inputs set it and ChangeNoiseRange resets it
+ * (Input_parameters.cpp:107-111; MHD_FullQlin.h:46-50).  Each driven entry
+ * gets mult * (N(0, f_noise^2/2) + i N(0, f_noise^2/2)) with the reference's
+ * multipliers, |k| range and ky = 0 Hermitian pairs (MHD_FullQlin.cpp:786-1053;
+ * MHD_BQlin drives ky != 0 only and ignores the drive_only flags).  The
+ * normals come from Philox4x64-10 keyed by seed and counted by (t, mode, kz
+ * slot), so a replayed graph draws new noise at every t and a restart at the
+ * same t draws the same noise; the reference's clock-seeded mt19937 stream is
+ * matched in distribution, not in values. */
+typedef struct ql_noise {
+    double f_noise;                 /* f_noise_ (0: off, the default)          */
+    double range_low, range_high;   /* noise_range_low_, noise_range_high_ in |k| */
+    int drive_only_velocity;        /* drive_only_velocity?                    */
+    int drive_only_magnetic;        /* drive_only_magnetic?                    */
+    unsigned long long seed;        /* generator key                           */
+} ql_noise;
+int ql_set_noise(ql_ctx *ctx, const ql_noise *cfg);
+
+/* Model::DrivingNoise(t, dt, sol): adds one step's noise at step-start time t.
+ * ql_steps applies it inside every step when f_noise != 0, between Step and
+ * the remap (main.cpp:111-123); ql_step does not (a caller mirroring main.cpp
+ * calls this after it). */
+int ql_driving_noise(ql_ctx *ctx, double t, double dt);
+
 /* Waits for all work queued on the context's stream. */
 int ql_synchronize(ql_ctx *ctx);
 
@@ -168,6 +192,9 @@ int ql_batch_get_drive(ql_batch *b, int run, double *drive);
 /* per-run ql_calculate_cfl / ql_check_solution */
 int ql_batch_calculate_cfl(ql_batch *b, int run, double *cfl_over_dt);
 int ql_batch_check_solution(ql_batch *b, int run, double sums[5], int *verdict);
+/* per-run ql_set_noise; ql_driving_noise for every driven run of the set */
+int ql_batch_set_noise(ql_batch *b, int run, const ql_noise *cfg);
+int ql_batch_driving_noise(ql_batch *b, double t, double dt);
 /* integrator priming at t0 of every run (RK3CN.cpp:57, RK3CN_Elin.cpp:257) */
 int ql_batch_linear_ops_init(ql_batch *b, double t0);
 /* nsteps of main.cpp:108-123 for every run (as ql_steps), one clock */

@@ -59,6 +59,12 @@ def lib(native: bool = False):
     L.orc_tables.argtypes = [P, D, D, I, I, D]
     L.orc_linear_ops_init.argtypes = [P, ctypes.c_double, D, D]
     L.orc_rhs.argtypes = [P, ctypes.c_double, ctypes.c_double, D, D, D]
+    U = ctypes.POINTER(ctypes.c_uint64)
+    L.orc_set_noise.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_uint64]
+    L.orc_driving_noise.argtypes = [P, ctypes.c_double, ctypes.c_double]
+    L.orc_noise_normals.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, D]
+    L.orc_philox4x64.argtypes = [U, U, U]
     L.orc_calculate_cfl.restype = ctypes.c_double
     L.orc_calculate_cfl.argtypes = [P]
     L.orc_check_sums.argtypes = [P, D]
@@ -81,6 +87,16 @@ def lib(native: bool = False):
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def philox4x64(ctr, key) -> np.ndarray:
+    """Philox4x64-10 block of counter ctr[4] under key[2]."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint64)
+    k = np.ascontiguousarray(key, dtype=np.uint64)
+    out = np.zeros(4, dtype=np.uint64)
+    P = ctypes.POINTER(ctypes.c_uint64)
+    lib().orc_philox4x64(c.ctypes.data_as(P), k.ctypes.data_as(P), out.ctypes.data_as(P))
+    return out
 
 
 def fft_inverse(NZ: int, x: np.ndarray) -> np.ndarray:
@@ -174,6 +190,19 @@ class Oracle:
 
     def step(self, t):
         return self._lib.orc_step(self._h, t)
+
+    def set_noise(self, f_noise, range_low=0.0, range_high=1e16, drive_only_velocity=0, drive_only_magnetic=0,
+                  seed=0x5EED0004):
+        self._lib.orc_set_noise(self._h, f_noise, range_low, range_high, drive_only_velocity,
+                                drive_only_magnetic, seed)
+
+    def driving_noise(self, t, dt):
+        self._lib.orc_driving_noise(self._h, t, dt)
+
+    def noise_normals(self, t, i, jj):
+        z = np.zeros(8)
+        self._lib.orc_noise_normals(self._h, t, i, jj, _dp(z))
+        return z
 
     def calculate_cfl(self):
         """Model::Calculate_CFL: CFL / dt of the next variable step."""

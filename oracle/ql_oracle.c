@@ -16,6 +16,7 @@
 
 #include <complex.h>
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -139,6 +140,11 @@ struct orc {
     cplx *drive;  /* [nMF][nz] */
     cplx *mfz;    /* [8][NZ] real-space mean-field profiles */
     work_t *work;
+    /* DrivingNoise settings (MHD_FullQlin.cpp:18-22, 47-52, 65-66) */
+    double f_noise, noise_range[2], mult_noise_fac;
+    int drive_only_velocity, drive_only_magnetic, dont_drive_ky0;
+    uint64_t noise_seed;
+    int *ky0pkx, *ky0nkx; /* ky = 0 pairs, kx > 0 and kx < 0 (Kdata.cpp:122-146) */
 };
 
 static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); return p; }
@@ -269,6 +275,22 @@ orc *orc_create(const orc_params *p)
         w->ilap2t = (double *)xcalloc(nz, sizeof(double));
         w->send = (cplx *)xcalloc((size_t)nMF * o->half, sizeof(cplx));
     }
+    /* noise: off by default (Input_parameters.cpp:107-111) */
+    o->noise_range[0] = 0.0;
+    o->noise_range[1] = 1e16 * 1e16;
+    o->mult_noise_fac = 1.0 / (16 * 32 * 32);
+    o->mult_noise_fac = o->mult_noise_fac * o->mult_noise_fac;
+    o->dont_drive_ky0 = (p->model == ORC_MODEL_MHD_BQLIN); /* MHD_BQlin.cpp:18 */
+    {
+        /* Kdata.cpp:122-131 for a single rank */
+        int nx = o->nx, ny = o->ny;
+        o->ky0pkx = (int *)xcalloc(nx > 1 ? nx - 1 : 1, sizeof(int));
+        o->ky0nkx = (int *)xcalloc(nx > 1 ? nx - 1 : 1, sizeof(int));
+        for (int i = 0; i < 2 * nx - 2; ++i) {
+            if (i < nx - 1) o->ky0pkx[i] = (i + 1) * ny;
+            if (i >= nx - 1) o->ky0nkx[2 * (nx - 1) - i - 1] = (i + 1) * ny;
+        }
+    }
     orc_linear_ops_init(o, 0.0, NULL, NULL);
     return o;
 }
@@ -292,6 +314,7 @@ void orc_destroy(orc *o)
     free(o->mf); free(o->rhs1_mf); free(o->rhs2_mf);
     free(o->linop); free(o->linop_old); free(o->linop_mf);
     free(o->drive); free(o->mfz);
+    free(o->ky0pkx); free(o->ky0nkx);
     free(o);
 }
 
@@ -999,11 +1022,116 @@ void orc_remap(orc *o, double qt)
     }
 }
 
-/* main.cpp:108-123 (noise off, no output) */
+/* ------------------------------------------------------------------------ */
+/* Driving noise                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Philox4x64-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC 2011): the counter-based generator that stands in for
+ * the reference's clock-seeded mt19937 */
+void orc_philox4x64(const uint64_t *ctr, const uint64_t *key, uint64_t *out)
+{
+    const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+    const uint64_t W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+    uint64_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        unsigned __int128 p0 = (unsigned __int128)M0 * c0, p1 = (unsigned __int128)M1 * c2;
+        uint64_t hi0 = (uint64_t)(p0 >> 64), lo0 = (uint64_t)p0, hi1 = (uint64_t)(p1 >> 64), lo1 = (uint64_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Box-Muller: two standard normals from two 64-bit words, u1 in (0, 1) */
+static void box_muller(uint64_t a, uint64_t b, double *z1, double *z2)
+{
+    double u1 = ((double)(a >> 11) + 0.5) * 0x1.0p-53;
+    double u2 = (double)(b >> 11) * 0x1.0p-53;
+    double r = sqrt(-2.0 * log(u1));
+    double th = (2.0 * QL_PI) * u2;
+    *z1 = r * cos(th);
+    *z2 = r * sin(th);
+}
+
+/* the normals of entry (t, mode i, slot jj): z[2f], z[2f+1] for field f */
+static void noise_normals(const orc *o, double t, int i, int jj, double *z)
+{
+    const uint64_t key[2] = {o->noise_seed, 0x514C4E4F49534500ull};
+    uint64_t tb;
+    memcpy(&tb, &t, sizeof tb);
+    for (int k = 0; k < 2; ++k) {
+        uint64_t c[4] = {tb, (uint64_t)i, (uint64_t)jj | ((uint64_t)k << 32), 0}, w[4];
+        orc_philox4x64(c, key, w);
+        box_muller(w[0], w[1], &z[4 * k + 0], &z[4 * k + 1]);
+        box_muller(w[2], w[3], &z[4 * k + 2], &z[4 * k + 3]);
+    }
+}
+
+void orc_noise_normals(const orc *o, double t, int i, int jj, double *z8) { noise_normals(o, t, i, jj, z8); }
+
+void orc_set_noise(orc *o, double f_noise, double range_low, double range_high, int drive_only_velocity,
+                   int drive_only_magnetic, uint64_t seed)
+{
+    /* ChangeNoiseRange (MHD_FullQlin.h:46-50) plus the input flags */
+    o->f_noise = f_noise;
+    o->noise_range[0] = range_low * range_low;
+    o->noise_range[1] = range_high * range_high;
+    o->drive_only_velocity = o->p.model == ORC_MODEL_MHD_FULLQLIN ? drive_only_velocity : 0;
+    o->drive_only_magnetic = o->p.model == ORC_MODEL_MHD_FULLQLIN ? drive_only_magnetic : 0;
+    o->noise_seed = seed;
+}
+
+/* noise of one mode: slots jj0..nz-1; for a ky = 0 pair the conjugate goes to
+ * slot nz - jj of mode ip (MHD_FullQlin.cpp:962-1046) */
+static void noise_mode(orc *o, int i, int ip, int jj0, double t, double dt, work_t *w)
+{
+    const int nz = o->nz;
+    kmode k;
+    assign_laplacians(o, i, t, 0, w, &k);
+    double noise_multfac = dt * o->totalN2 * o->mult_noise_fac;
+    const double sigma = o->f_noise / sqrt(2.0); /* normal_distribution(0, f/sqrt 2), :47 */
+    const int on[4] = {!o->drive_only_magnetic, !o->drive_only_magnetic, !o->drive_only_velocity,
+                       !o->drive_only_velocity};
+    for (int jj = jj0; jj < nz; ++jj) {
+        double lapF = w->lapF[jj], lap2 = w->lap2t[jj];
+        if (!(lapF > -o->noise_range[1] && lapF < -o->noise_range[0])) continue; /* :806 */
+        double multU = sqrt(noise_multfac * lap2 / lapF);                        /* :808-811 */
+        double multZ = sqrt(-noise_multfac * lap2);
+        const double mult[4] = {multU, multZ, multU, multZ};
+        double z[8];
+        noise_normals(o, t, i, jj, z);
+        for (int f = 0; f < 4; ++f) {
+            if (!on[f]) continue;
+            cplx n = mult[f] * mk(sigma * z[2 * f], sigma * z[2 * f + 1]);
+            o->lin[((size_t)i * 4 + f) * nz + jj] += n;
+            if (ip >= 0) o->lin[((size_t)ip * 4 + f) * nz + (nz - jj)] += conj(n);
+        }
+    }
+}
+
+/* Model::DrivingNoise (MHD_FullQlin.cpp:786-1053; MHD_BQlin.cpp:558-740) */
+void orc_driving_noise(orc *o, double t, double dt)
+{
+    if (o->f_noise == 0.0) return;
+    work_t *w = &o->work[0];
+    for (int i = 0; i < o->nxy; ++i)
+        if (o->ky_index[i] != 0) noise_mode(o, i, -1, 0, t, dt, w); /* :796-845 */
+    if (!o->dont_drive_ky0)
+        for (int m = 0; m < o->nx - 1; ++m) /* the no-MPI branch, :962-1046 */
+            noise_mode(o, o->ky0nkx[m], o->ky0pkx[m], 1, t, dt, w);
+}
+
+/* main.cpp:108-123 (no output): Step, DrivingNoise, t += dt, remap(q t) */
 double orc_steps(orc *o, int nsteps, double t)
 {
     for (int n = 0; n < nsteps; ++n) {
         double dt = orc_step(o, t);
+        if (o->f_noise != 0.0) orc_driving_noise(o, t, dt);
         t += dt;
         if (o->p.remap) orc_remap(o, o->p.q * t);
     }

@@ -33,6 +33,8 @@
 #ifndef QL_ORACLE_H
 #define QL_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -83,9 +85,23 @@ void orc_check_sums(const orc *o, double *out5);
 /* the verdict of Check_Solution on those sums: 0 passes, 1 imaginary mean
  * field, 2 very large, 3 NaN or Inf */
 int orc_check_solution(const orc *o);
+/* Model::ChangeNoiseRange with the noise inputs (Input_parameters.cpp:107-111)
+ * and the generator key; f_noise = 0 (the default) turns the noise off */
+void orc_set_noise(orc *o, double f_noise, double range_low, double range_high, int drive_only_velocity,
+                   int drive_only_magnetic, uint64_t seed);
+/* Model::DrivingNoise(t, dt, sol) (MHD_FullQlin.cpp:786-1053, MHD_BQlin.cpp:
+ * 558-740) with normals from Philox4x64-10 counted by (t, mode, kz slot) in
+ * place of the reference's clock-seeded mt19937 */
+void orc_driving_noise(orc *o, double t, double dt);
+/* the eight standard normals of entry (t, mode i, slot jj): re, im of fields
+ * ux, zeta, bx, eta before the multipliers and f_noise / sqrt(2) */
+void orc_noise_normals(const orc *o, double t, int i, int jj, double *z8);
+/* the generator: out = Philox4x64-10(ctr[4], key[2]) */
+void orc_philox4x64(const uint64_t *ctr, const uint64_t *key, uint64_t *out);
 /* Integrator::Step(t, sol); returns dt */
 double orc_step(orc *o, double t);
-/* main.cpp:108-123 without output: Step, t += dt, remap(q t); returns t */
+/* main.cpp:108-123 without output: Step, DrivingNoise (if f_noise != 0),
+ * t += dt, remap(q t); returns t */
 double orc_steps(orc *o, int nsteps, double t);
 /* Model::ShearingBox_Remap(qt, sol) */
 void orc_remap(orc *o, double qt);

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libql_dns.so")
 SOURCES = ["ql_dns.hip", "ql_adjoint.hip", "host/ql_model.cpp"]
-HEADERS = ["ql_fft.hpp", "ql_kernels.hpp", "ql_mode.hpp", "ql_adjoint.hpp", "ql_runtime.hpp", "host/ql_model.hpp"]
+HEADERS = ["ql_fft.hpp", "ql_kernels.hpp", "ql_mode.hpp", "ql_adjoint.hpp", "ql_noise.hpp", "ql_runtime.hpp", "host/ql_model.hpp"]
 HOST_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_host_adapter.cpp")
 HOST_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_host_adapter")
 

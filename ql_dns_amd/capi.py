@@ -24,7 +24,8 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_set_state", "ql_batch_get_state", "ql_batch_diagnostics", "ql_batch_get_drive",
            "ql_batch_linear_ops_init", "ql_batch_steps", "ql_batch_tune", "ql_batch_get_config",
            "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward", "ql_calculate_cfl",
-           "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution")
+           "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution", "ql_set_noise",
+           "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise")
 
 QL_ERR_STATE = -4
 
@@ -43,6 +44,18 @@ class ql_params(ctypes.Structure):
                 ("viscosity_order", ctypes.c_int), ("quasilinear", ctypes.c_int),
                 ("remap", ctypes.c_int), ("dt", ctypes.c_double), ("device", ctypes.c_int),
                 ("CFL", ctypes.c_double)]
+
+
+class ql_noise(ctypes.Structure):
+    _fields_ = [("f_noise", ctypes.c_double), ("range_low", ctypes.c_double), ("range_high", ctypes.c_double),
+                ("drive_only_velocity", ctypes.c_int), ("drive_only_magnetic", ctypes.c_int),
+                ("seed", ctypes.c_ulonglong)]
+
+
+def noise_config(f_noise: float, range_low: float = 0.0, range_high: float = 1e16, drive_only_velocity: int = 0,
+                 drive_only_magnetic: int = 0, seed: int = 0x5EED0004) -> ql_noise:
+    """The reference's noise inputs (Input_parameters.cpp:107-111), defaults included."""
+    return ql_noise(f_noise, range_low, range_high, drive_only_velocity, drive_only_magnetic, seed)
 
 
 _LIB = None
@@ -95,6 +108,10 @@ def load(path: str = LIB_PATH):
         "ql_check_solution": ([P, D, I], ctypes.c_int),
         "ql_batch_calculate_cfl": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_batch_check_solution": ([P, ctypes.c_int, D, I], ctypes.c_int),
+        "ql_set_noise": ([P, ctypes.POINTER(ql_noise)], ctypes.c_int),
+        "ql_driving_noise": ([P, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "ql_batch_set_noise": ([P, ctypes.c_int, ctypes.POINTER(ql_noise)], ctypes.c_int),
+        "ql_batch_driving_noise": ([P, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -251,6 +268,14 @@ class QLContext:
     def check_solution(self, strict: bool = False):
         return _check_solution(self._L, self._L.ql_check_solution, (self._h,), strict)
 
+    def set_noise(self, f_noise: float, **kw):
+        """Model::ChangeNoiseRange plus the noise inputs; see noise_config."""
+        cfg = noise_config(f_noise, **kw)
+        self._check(self._L.ql_set_noise(self._h, ctypes.byref(cfg)))
+
+    def driving_noise(self, t: float, dt: float):
+        self._check(self._L.ql_driving_noise(self._h, t, dt))
+
     def synchronize(self):
         self._check(self._L.ql_synchronize(self._h))
 
@@ -370,6 +395,13 @@ class QLBatch:
 
     def check_solution(self, run: int, strict: bool = False):
         return _check_solution(self._L, self._L.ql_batch_check_solution, (self._h, run), strict)
+
+    def set_noise(self, run: int, f_noise: float, **kw):
+        cfg = noise_config(f_noise, **kw)
+        self._check(self._L.ql_batch_set_noise(self._h, run, ctypes.byref(cfg)))
+
+    def driving_noise(self, t: float, dt: float):
+        self._check(self._L.ql_batch_driving_noise(self._h, t, dt))
 
     def tune(self, trial_steps: int = 5) -> float:
         return _tune(self._L, self._h, trial_steps)

@@ -61,6 +61,10 @@ GpuModel::GpuModel(const Inputs &sp) : Model(sp.NZ, sp.NXY, sp.L), sp_(sp), f_no
     p.device = sp.device;
     check(ql_create(&p, &ctx_));
     check(ql_dims(ctx_, dims_));
+    // Input_parameters.cpp:107-111 (MHD_FullQlin.cpp:18-22, 47-52)
+    noise_ = ql_noise{sp.f_noise, sp.noise_range_low, sp.noise_range_high, sp.drive_only_velocityQ ? 1 : 0,
+                      sp.drive_only_magneticQ ? 1 : 0, sp.noise_seed};
+    if (sp.f_noise != 0.0) check(ql_set_noise(ctx_, &noise_));
 }
 
 GpuModel::~GpuModel()
@@ -109,15 +113,23 @@ void GpuModel::ShearingBox_Remap(double qt, solution *sol)
     sol->mark_device_newer();
 }
 
-void GpuModel::DrivingNoise(double, double, solution *)
+void GpuModel::DrivingNoise(double t, double dt, solution *sol)
 {
-    // MHD_FullQlin::DrivingNoise (MHD_FullQlin.cpp:786-1053) is not part of the
-    // GPU path; main.cpp:112 calls it only when f_noise != 0.
-    if (f_noise_ != 0.0)
-        throw Error(QL_ERR_UNSUPPORTED, "driving noise (f_noise != 0) is not available on the GPU path");
+    // MHD_FullQlin.cpp:786-1053 / MHD_BQlin.cpp:558-740 on the device
+    sol->push();
+    check(ql_driving_noise(ctx_, t, dt));
+    sol->mark_device_newer();
 }
 
-void GpuModel::ChangeNoiseRange(double fnoise, double, double) { f_noise_ = fnoise; }
+void GpuModel::ChangeNoiseRange(double fnoise, double kmin, double kmax)
+{
+    // MHD_FullQlin.h:46-50: amplitude and |k| range; the flags and seed stay
+    f_noise_ = fnoise;
+    noise_.f_noise = fnoise;
+    noise_.range_low = kmin;
+    noise_.range_high = kmax;
+    check(ql_set_noise(ctx_, &noise_));
+}
 
 void GpuModel::Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *sol)
 {

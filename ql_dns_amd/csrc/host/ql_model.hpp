@@ -72,6 +72,9 @@ struct Inputs {
     double nu = 0.01, eta = 0.01, q = 1.5, omega = 1.0, B0z = 0.0;
     int viscosity_order = 2;
     double f_noise = 0.0;
+    double noise_range_low = 0.0, noise_range_high = 1e16;
+    bool drive_only_velocityQ = false, drive_only_magneticQ = false;
+    unsigned long long noise_seed = 0x5EED0004ull;  // the device generator's key
     bool remapQ = true, QuasiLinearQ = true;
     int device = 0;
 };
@@ -171,6 +174,7 @@ private:
     ql_ctx *ctx_ = nullptr;
     int dims_[6] = {0, 0, 0, 0, 0, 0};
     double f_noise_ = 0.0;
+    ql_noise noise_{};
 };
 
 // solution.h:23-49

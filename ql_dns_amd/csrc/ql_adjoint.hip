@@ -68,6 +68,7 @@ int check_adjoint(const Batch &b, const char *who)
         return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin only", who);
     if (b.precision != QL_PREC_FP64) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint runs in FP64 only", who);
     if (b.vdt) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs fixed steps (dt > 0)", who);
+    if (b.groups[0].noise) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs the driving noise off", who);
     return QL_OK;
 }
 

@@ -99,9 +99,9 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
     const int tid = threadIdx.x;
     const Phys ph = a.ph;
     const double q = ph.q, Om = ph.omega, B0z = ph.B0z;
-    const double qt = to_sgpr(__dmul_rn(q, a.ts)), qdtl = to_sgpr(__dmul_rn(q, a.dtl));
-    const double qt_old = to_sgpr(__dmul_rn(q, a.tso)), qdtl_old = to_sgpr(__dmul_rn(q, a.dtlo));
-    const double remap_qt = to_sgpr(__dmul_rn(q, a.tnew));
+    const double qt = to_sgpr(rn_mul(q, a.ts)), qdtl = to_sgpr(rn_mul(q, a.dtl));
+    const double qt_old = to_sgpr(rn_mul(q, a.tso)), qdtl_old = to_sgpr(rn_mul(q, a.dtlo));
+    const double remap_qt = to_sgpr(rn_mul(q, a.tnew));
     const bool remap_now = a.s == 3 && a.do_remap;
     for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
     for (int k = tid; k < NZD; k += NT) skz[k] = a.kzv[k];
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
                 const double kx0 = a.kx0[i], ky = a.ky[i];
                 skx0[tid] = kx0;
                 sky[tid] = ky;
-                skx[tid] = __dadd_rn(kx0, __dmul_rn(qt, ky));
+                skx[tid] = rn_add(kx0, rn_mul(qt, ky));
             } else {
                 skx0[tid] = sky[tid] = skx[tid] = 0.0;
             }
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
                         gLn += a.linC * redot(uo, go);
                     } else {
                         double Lo;
-                        const double kxo = __dadd_rn(skx0[mb], __dmul_rn(qt_old, ky));
+                        const double kxo = rn_add(skx0[mb], rn_mul(qt_old, ky));
                         if (a.oinit) {
                             const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
                             xo = -kxo * kxo + lap2;

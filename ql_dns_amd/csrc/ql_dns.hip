@@ -51,7 +51,7 @@ void release(Batch &b)
         dfree(g.u), dfree(g.N1), dfree(g.N2), dfree(g.mf), dfree(g.mfN1), dfree(g.mfN2), dfree(g.drive);
         dfree(g.tw), dfree(g.tw32), dfree(g.kx0), dfree(g.ky), dfree(g.kzv), dfree(g.mfz), dfree(g.part);
         dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg);
-        dfree(g.clock), dfree(g.pm);
+        dfree(g.clock), dfree(g.pm), dfree(g.mode_run), dfree(g.np);
         if (g.join) (void)hipEventDestroy(g.join);
         if (g.s) (void)hipStreamDestroy(g.s);
         g.join = nullptr;
@@ -209,6 +209,27 @@ int configure(Batch &b, Group &g, int line, int target)
     return QL_OK;
 }
 
+// DrivingNoise settings of one run (MHD_FullQlin.cpp:20-22, 47-52, 65-66;
+// MHD_BQlin.cpp:18, 54-55)
+void noise_params(qlk::NoiseParams &n, const RunInfo &r, const ql_noise &c)
+{
+    n = qlk::NoiseParams{};
+    n.sigma = c.f_noise / std::sqrt(2.0);
+    n.lo2 = c.range_low * c.range_low;
+    n.hi2 = c.range_high * c.range_high;
+    // totalN2 * (1/(16 32 32))^2; the factor is 2^-28, so the product with dt
+    // rounds as dt * totalN2 * mult_noise_fac does
+    const double mnf = 1.0 / (16 * 32 * 32);
+    n.nfac = r.totalN2 * (mnf * mnf);
+    n.seed = c.seed;
+    const bool full = r.p.model == QL_MODEL_MHD_FULLQLIN;
+    n.drive_u = full ? !c.drive_only_magnetic : 1;
+    n.drive_b = full ? !c.drive_only_velocity : 1;
+    n.drive_ky0 = full ? 1 : 0;
+    n.nyh = r.p.NY / 2;
+    n.nxh = r.p.NX / 2;
+}
+
 int create_batch(Batch &b, const ql_params *ps, int nruns)
 {
     const ql_params &p0 = ps[0];
@@ -272,13 +293,21 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_RC(dalloc(&g.sums, (size_t)g.nruns * b.nMF * g.NZ));
         QL_RC(dalloc(&g.rp, g.nruns));
         QL_RC(dalloc(&g.pm, (size_t)g.nruns * qlk::PM_STRIDE));
+        QL_RC(dalloc(&g.mode_run, g.modes));
+        QL_RC(dalloc(&g.np, g.nruns));
         QL_RC(dalloc(&g.clock, 1));
         std::vector<double> kx0(g.modes), ky(g.modes);
+        std::vector<int> mrun(g.modes);
+        g.h_np.assign(g.nruns, qlk::NoiseParams{});
         for (int id : g.run_ids) {
             const RunInfo &ri = b.runs[id];
             std::copy(ri.h_kx0.begin(), ri.h_kx0.end(), kx0.begin() + ri.mode_off);
             std::copy(ri.h_ky.begin(), ri.h_ky.end(), ky.begin() + ri.mode_off);
+            std::fill(mrun.begin() + ri.mode_off, mrun.begin() + ri.mode_off + ri.nxy, ri.gi);
+            noise_params(g.h_np[ri.gi], ri, ql_noise{0.0, 0.0, 1e16, 0, 0, 0ull});
         }
+        QL_HIP(hipMemcpy(g.mode_run, mrun.data(), sizeof(int) * g.modes, hipMemcpyHostToDevice));
+        QL_HIP(hipMemcpy(g.np, g.h_np.data(), sizeof(qlk::NoiseParams) * g.nruns, hipMemcpyHostToDevice));
         std::vector<double2> htw(g.NZ);
         std::vector<float2> htw32(g.NZ);
         for (int k = 0; k < g.NZ; ++k) {
@@ -387,6 +416,29 @@ qlk::MFArgs mf_args(const Batch &b, const Group &g)
     return m;
 }
 
+int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool remap, hipStream_t st)
+{
+    qlk::NoiseArgs a{};
+    a.u = g.u;
+    a.kx0 = g.kx0;
+    a.ky = g.ky;
+    a.kzv = g.kzv;
+    a.runs = g.rp;
+    a.np = g.np;
+    a.mode_run = g.mode_run;
+    a.clock = g.clock;
+    a.nz = g.nz;
+    a.use_clock = use_clock ? 1 : 0;
+    a.noise = noise ? 1 : 0;
+    a.do_remap = remap ? 1 : 0;
+    a.t_x = t;
+    a.dt_x = dt;
+    const int threads = (g.nz + 63) / 64 * 64;  // one thread per kz slot (nz <= 171)
+    qlk::k_noise_remap<<<g.modes, threads, 0, st>>>(a);
+    QL_HIP(hipGetLastError());
+    return QL_OK;
+}
+
 // mode kernel -> column reduction of its partials -> mean-field kernel
 int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFArgs &m, hipStream_t st)
 {
@@ -443,6 +495,9 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
         nlc1 = A2 * dt;
         nlc2 = B2 * dt;
     }
+    // with driving noise the step ends with the noise kernel, which remaps
+    // after it (main.cpp:112-123)
+    const bool noisy = g.noise && advance && s == 3;
     qlk::ModeArgs a = mode_args(b, g);
     a.s = s;
     a.mode = cn ? qlk::MODE_CN : qlk::MODE_ELIN;
@@ -451,7 +506,7 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
     a.nlc2 = nlc2;
     a.n_prev = (s == 1) ? nullptr : (s == 2 ? g.N1 : g.N2);
     a.n_out = (s == 1) ? g.N1 : (s == 2 ? g.N2 : nullptr);
-    a.do_remap = remap_after ? 1 : 0;
+    a.do_remap = (remap_after && !noisy) ? 1 : 0;
 
     qlk::MFArgs m = mf_args(b, g);
     m.mode = a.mode;
@@ -462,7 +517,9 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
     m.mfn_out = (s == 2) ? g.mfN2 : g.mfN1;
     m.advance = (advance && s == 3) ? 1 : 0;
     m.s = s;
-    return enqueue_sequence(b, g, a, m, st);
+    QL_RC(enqueue_sequence(b, g, a, m, st));
+    if (noisy) QL_RC(launch_noise(g, true, 0.0, 0.0, true, remap_after, st));
+    return QL_OK;
 }
 
 // substep s of every group: groups fork from the handle's stream and join it
@@ -987,6 +1044,18 @@ int ql_check_solution(ql_ctx *c, double sums[5], int *verdict)
     return ql_batch_check_solution(&c->bt, 0, sums, verdict);
 }
 
+int ql_set_noise(ql_ctx *c, const ql_noise *cfg)
+{
+    if (!c) return fail(QL_ERR_ARG, "ql_set_noise: null context");
+    return ql_batch_set_noise(&c->bt, 0, cfg);
+}
+
+int ql_driving_noise(ql_ctx *c, double t, double dt)
+{
+    if (!c) return fail(QL_ERR_ARG, "ql_driving_noise: null context");
+    return ql_batch_driving_noise(&c->bt, t, dt);
+}
+
 int ql_synchronize(ql_ctx *c)
 {
     if (!c) return fail(QL_ERR_ARG, "ql_synchronize: null context");
@@ -1153,6 +1222,36 @@ int ql_batch_check_solution(ql_batch *bt, int run, double sums[5], int *verdict)
     if (v == 1) return fail(QL_ERR_STATE, "run %d: ERROR: Mean field has developed an imaginary part!!", run);
     if (v == 2) return fail(QL_ERR_STATE, "run %d: ERROR: Solution is very large, probably unstable!!", run);
     if (v == 3) return fail(QL_ERR_STATE, "run %d: ERROR: Solution contains NaN or Inf!!", run);
+    return QL_OK;
+}
+
+int ql_batch_set_noise(ql_batch *bt, int run, const ql_noise *cfg)
+{
+    if (!bt || !cfg) return fail(QL_ERR_ARG, "ql_batch_set_noise: null argument");
+    QL_RC(check_run(bt->b, run, "ql_batch_set_noise"));
+    if (!(cfg->f_noise >= 0) || !std::isfinite(cfg->f_noise))
+        return fail(QL_ERR_ARG, "ql_batch_set_noise: f_noise=%g must be finite and >= 0", cfg->f_noise);
+    if (std::isnan(cfg->range_low) || std::isnan(cfg->range_high))
+        return fail(QL_ERR_ARG, "ql_batch_set_noise: NaN noise range");
+    Batch &b = bt->b;
+    const RunInfo &r = b.runs[run];
+    Group &g = b.groups[r.group];
+    QL_RC(sync_all(b));
+    noise_params(g.h_np[r.gi], r, *cfg);
+    g.noise = false;
+    for (const qlk::NoiseParams &n : g.h_np) g.noise = g.noise || n.sigma != 0.0;
+    QL_HIP(hipMemcpy(g.np, g.h_np.data(), sizeof(qlk::NoiseParams) * g.nruns, hipMemcpyHostToDevice));
+    drop_graphs(b);  // the noise kernel joins or leaves the captured step
+    return QL_OK;
+}
+
+int ql_batch_driving_noise(ql_batch *bt, double t, double dt)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_driving_noise: null handle");
+    Batch &b = bt->b;
+    for (Group &g : b.groups)
+        if (g.noise) QL_RC(launch_noise(g, false, t, dt, true, false, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
 

@@ -17,6 +17,27 @@ namespace qlk {
 
 constexpr int NT = 256;  // threads per workgroup of every kernel here
 
+// IEEE double operations the compiler may not contract into an FMA with a
+// neighbouring multiply or add.  HIP contracts across statements by default
+// and its __dadd_rn / __dmul_rn are plain operators, so values that must round
+// exactly as the reference's host expressions do (times, wavenumbers, remap
+// thresholds, step coefficients) go through these.
+__device__ __forceinline__ double rn_add(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double rn_sub(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double rn_mul(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
 // complex arithmetic on double2 (FP64 path) and float2 (FP32 transforms)
 template <class C>
 struct Cx;

@@ -105,7 +105,7 @@ __device__ __forceinline__ StepCoefs step_coefs(double dt, int s, bool cn)
     const double a = s == 1 ? A0 : (s == 2 ? A1 : A2);
     const double ab = s == 1 ? A0 : (s == 2 ? A1 + B1 : A2 + B2);
     const double b = s == 1 ? 0.0 : (s == 2 ? B1 : B2);
-    return StepCoefs{__dmul_rn(h, ab), __dmul_rn(a, dt), __dmul_rn(b, dt)};
+    return StepCoefs{rn_mul(h, ab), rn_mul(a, dt), rn_mul(b, dt)};
 }
 
 // substep times from the clock, with the host's roundings (unfused): ts,
@@ -129,23 +129,23 @@ __device__ __forceinline__ SubTimes sub_times(const ModeArgs &a)
     StepCoefs c{a.linC, a.nlc1, a.nlc2};
     if (a.vdt) {
         dt = a.clock->dt;
-        dtP1 = __dmul_rn(dt, P1);
-        dtP2 = __dmul_rn(dt, P2);
-        dtl2 = __dmul_rn(dt, P2 - P1);
-        dtl3 = __dmul_rn(dt, 1.0 - P2);
+        dtP1 = rn_mul(dt, P1);
+        dtP2 = rn_mul(dt, P2);
+        dtl2 = rn_mul(dt, P2 - P1);
+        dtl3 = rn_mul(dt, 1.0 - P2);
         const double dl = a.clock->dt_last;
-        dtP2o = __dmul_rn(dl, P2);
-        dtl3o = __dmul_rn(dl, 1.0 - P2);
+        dtP2o = rn_mul(dl, P2);
+        dtl3o = rn_mul(dl, 1.0 - P2);
         c = step_coefs(dt, a.s, CN);
     }
     const double dtl1 = dtP1;  // dt * p1 (RK3CN_Elin.cpp:90)
-    const double tnew = __dadd_rn(t, dt);
+    const double tnew = rn_add(t, dt);
     const bool s1 = a.s == 1, s2 = a.s == 2;
     const bool oinit = s1 && a.clock->old_init != 0;
-    const double ts = s1 ? t : __dadd_rn(t, s2 ? dtP1 : dtP2);
+    const double ts = s1 ? t : rn_add(t, s2 ? dtP1 : dtP2);
     const double dtl = s1 ? dtl1 : (s2 ? dtl2 : dtl3);
-    const double tso = s1 ? (oinit ? a.clock->t_init : __dadd_rn(a.clock->t_prev, dtP2o))
-                          : (s2 ? t : __dadd_rn(t, dtP1));
+    const double tso = s1 ? (oinit ? a.clock->t_init : rn_add(a.clock->t_prev, dtP2o))
+                          : (s2 ? t : rn_add(t, dtP1));
     const double dtlo = s1 ? (oinit ? 0.0 : dtl3o) : (s2 ? dtl1 : dtl2);
     return SubTimes{ts, dtl, tso, dtlo, tnew, c.linC, c.nlc1, c.nlc2, oinit};
 }
@@ -192,7 +192,7 @@ __device__ __forceinline__ double neg_abs_pow(double x, int vo2)
 // with unfused roundings so it agrees bit for bit with the host expression
 __device__ __forceinline__ bool remap_hit(double qt, double thr, double kx0, double ky)
 {
-    return __dadd_rn(kx0, __dmul_rn(qt, ky)) > thr;
+    return rn_add(kx0, rn_mul(qt, ky)) > thr;
 }
 
 // the host's view of the clock, passed by value (no host buffer involved);
@@ -211,7 +211,7 @@ __device__ __forceinline__ void clock_advance(Clock *c)
 {
     const double t = c->t, dt = c->dt;
     c->t_prev = t;
-    c->t = __dadd_rn(t, dt);
+    c->t = rn_add(t, dt);
     c->dt_last = dt;
     c->old_init = 0;
 }
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
         double m2 = 0.0;
         for (int j = j0; j < NZ; j += 32) {
             const double2 v = buf[l * NZ + j];
-            m2 = fmax(m2, __dadd_rn(__dmul_rn(v.x, v.x), __dmul_rn(v.y, v.y)));
+            m2 = fmax(m2, rn_add(rn_mul(v.x, v.x), rn_mul(v.y, v.y)));
         }
 #pragma unroll
         for (int o = 16; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o, 32));
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
         __syncthreads();
         if (hit) {
             for (int k = threadIdx.x; k < 4 * nz; k += NT) u[(size_t)i * 4 * nz + k] = make_double2(0.0, 0.0);
-            if (threadIdx.x == 0) kx0[i] = __dsub_rn(k0, shift);
+            if (threadIdx.x == 0) kx0[i] = rn_sub(k0, shift);
         }
     }
 }
@@ -480,7 +480,7 @@ struct DiagArgs {
     const int nz = a.nz;
     for (int i = a.m0 + blockIdx.x; i < a.m0 + a.nxy; i += gridDim.x) {
         const double ky = a.ky[i];
-        const double kx = __dadd_rn(a.kx0[i], __dmul_rn(a.qt, ky));
+        const double kx = rn_add(a.kx0[i], rn_mul(a.qt, ky));
         const double mult = (ky == 0.0) ? 1.0 : 2.0;
         for (int m = threadIdx.x; m < nz; m += NT) {
             const double kz = a.kzv[m], kz2 = -(kz * kz);

@@ -344,9 +344,9 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     // q t products: wave-uniform, kept in scalar registers
     const SubTimes st = sub_times<MODE == MODE_CN>(a);
     const double linC = to_sgpr(st.linC), nlc1 = to_sgpr(st.nlc1), nlc2 = to_sgpr(st.nlc2);
-    const double qt = to_sgpr(__dmul_rn(ph.q, st.ts)), qdtl = to_sgpr(__dmul_rn(ph.q, st.dtl));
-    const double qt_old = to_sgpr(__dmul_rn(ph.q, st.tso)), qdtl_old = to_sgpr(__dmul_rn(ph.q, st.dtlo));
-    const double remap_qt = to_sgpr(__dmul_rn(ph.q, st.tnew));
+    const double qt = to_sgpr(rn_mul(ph.q, st.ts)), qdtl = to_sgpr(rn_mul(ph.q, st.dtl));
+    const double qt_old = to_sgpr(rn_mul(ph.q, st.tso)), qdtl_old = to_sgpr(rn_mul(ph.q, st.dtlo));
+    const double remap_qt = to_sgpr(rn_mul(ph.q, st.tnew));
     const bool do_remap = a.do_remap && rp.remap;
 
     const C *twg = reinterpret_cast<const C *>(sizeof(C) == 8 ? a.tw32 : (const void *)a.tw);
@@ -406,7 +406,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         const size_t off = (size_t)(mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
         double kx = 0.0, ilapF = 1.0;
         if (has_it) {
-            if (l >= 0) kx = __dadd_rn(kx0, __dmul_rn(qt, ky));  // kx(t), MHD_FullQlin.cpp:727
+            if (l >= 0) kx = rn_add(kx0, rn_mul(qt, ky));  // kx(t), MHD_FullQlin.cpp:727
             const bool zero = (ky == 0.0 && im == 0);
             const double lap2 = zero ? 0.0 : (-ky * ky + kz2);
             const double ilap2 = zero ? 1.0 : 1.0 / lap2;
@@ -498,7 +498,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                     // RK3CN.cpp:105-107, :141-143, :177-179; the "old" operator is
                     // the one of the previous substep (or linearOPs_Init at t0)
                     double Lo;
-                    const double kxo = __dadd_rn(kx0, __dmul_rn(qt_old, ky));
+                    const double kxo = rn_add(kx0, rn_mul(qt_old, ky));
                     if (st.oinit) {
                         const double lap2 = (ky == 0.0 && im == 0) ? 0.0 : (-ky * ky + kz2);
                         Lo = P::lin_init(pr, kxo, lap2, ph);
@@ -524,7 +524,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                     a.n_out[off1] = N1;
                 }
             }
-            if (hit && im == 0) a.kx0[mode_off + l] = __dsub_rn(kx0, rp.remap_shift);
+            if (hit && im == 0) a.kx0[mode_off + l] = rn_sub(kx0, rp.remap_shift);
         }
         // no barrier here: the next group's B touches only sbase and the mode
         // tables, which nothing in F reads; its C starts after B's barrier

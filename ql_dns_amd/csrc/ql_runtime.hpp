@@ -13,7 +13,7 @@
 #include <vector>
 
 #include "../../include/ql_dns.h"
-#include "ql_kernels.hpp"
+#include "ql_noise.hpp"
 
 namespace qlh {
 
@@ -72,6 +72,10 @@ struct Group {
     double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr, *part = nullptr, *sums = nullptr;
     double *lin_tmp = nullptr;
     double *pm = nullptr;  // profile maxima of the CFL condition [runs][PM_STRIDE]
+    int *mode_run = nullptr;        // [modes] run (within the group) of each mode
+    qlk::NoiseParams *np = nullptr;  // [runs] driving noise
+    std::vector<qlk::NoiseParams> h_np;
+    bool noise = false;              // some run of the group is driven
     qlk::RunParams *rp = nullptr;
     qlk::WgInfo *wg = nullptr;
     qlk::Clock *clock = nullptr;  // the group's step times
@@ -124,6 +128,7 @@ int sync_all(Batch &b);
 int set_clock(Batch &b, double t);
 int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, hipStream_t st);
 void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st);
+int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool remap, hipStream_t st);
 qlk::MFArgs mf_args(const Batch &b, const Group &g);
 int refresh_profiles(Batch &b, Group &g, hipStream_t st);
 int run_steps(Batch &b, int nsteps, double *t_inout);
