@@ -33,7 +33,10 @@ extern "C" {
 #define QL_ERR_UNSUPPORTED (-3) /* feature outside this build               */
 #define QL_ERR_STATE (-4)       /* non-finite or runaway state detected      */
 
-enum { QL_MODEL_MHD_FULLQLIN = 0, QL_MODEL_MHD_BQLIN = 1 };
+/* HD_fullU (Models/HD_fullU.cpp) has nLin = 2 fluctuation fields (ux, zeta)
+ * and nMF = 2 mean fields (Ux, Uy) in every host layout below; it runs on the
+ * MHD_FullQlin kernels with the magnetic fields held at zero. */
+enum { QL_MODEL_MHD_FULLQLIN = 0, QL_MODEL_MHD_BQLIN = 1, QL_MODEL_HD_FULLU = 2 };
 enum { QL_INT_RK3CN = 0, QL_INT_RK3CN_ELIN = 1 };
 enum { QL_PREC_FP64 = 0, QL_PREC_MIXED = 1 };
 
@@ -116,7 +119,13 @@ int ql_get_drive(ql_ctx *ctx, double *drive);
  * fac1D * Re(ifft(pad(MF))) -> [nMF][NZ] doubles. */
 int ql_mf_real(ql_ctx *ctx, double *out);
 
-/* Model::Calculate_CFL (MHD_FullQlin.cpp:769-780, MHD_BQlin.cpp:540-552):
+/* Full Reynolds-stress / EMF profiles of the last rhs, fac1D * Re(ifft(pad(
+ * drive))) -> [nMF][NZ] doubles: HD_fullU's Reynolds saves
+ * (HD_fullU.cpp:524-535, save_full_reynolds_). */
+int ql_drive_real(ql_ctx *ctx, double *out);
+
+/* Model::Calculate_CFL (MHD_FullQlin.cpp:769-780, MHD_BQlin.cpp:540-552,
+ * HD_fullU.cpp:604-628 with its mean-field source term):
  * CFL/dt of the next variable step, kmax (max|By| + max|Bx| + max|Ux| +
  * max|Uy|) + q over the mean-field profiles the last rhs read (MHD_BQlin:
  * kmax max|By| + kmax max|Bx| + q, Bx of the current state).  The maxima are

@@ -70,6 +70,13 @@ int qlio_dat_init(const char *dir);
 int qlio_dat_append(const char *dir, double t, const double diag[18], int nrey, const double *mf_real, int nmf,
                     int NZ);
 
+/* the same from TimeVariables' arrays: energy, angular momentum and
+ * dissipation (4 each) and nrey Reynolds values (HD_fullU saves 2 NZ stress
+ * profile values, HD_fullU.cpp:524-535) */
+int qlio_dat_append_values(const char *dir, double t, const double energy[4], const double am[4],
+                           const double diss[4], const double *reynolds, int nrey, const double *mf_real, int nmf,
+                           int NZ);
+
 const char *qlio_last_error(void);
 
 #ifdef __cplusplus

@@ -21,7 +21,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-MODEL_MHD_FULLQLIN, MODEL_MHD_BQLIN, MODEL_CONSTANT_DAMPING, MODEL_LINEAR_ONLY = 0, 1, 2, 3
+MODEL_MHD_FULLQLIN, MODEL_MHD_BQLIN, MODEL_CONSTANT_DAMPING, MODEL_LINEAR_ONLY, MODEL_HD_FULLU = 0, 1, 2, 3, 4
 
 
 class _Params(ctypes.Structure):
@@ -152,7 +152,7 @@ class Oracle:
             return None if a is None else np.ascontiguousarray(a, dtype=dt)
         lin, mf, kx0 = c(lin, np.complex128), c(mf, np.complex128), c(kx0, np.float64)
         if lin is not None:
-            assert lin.size == self.nxy * 4 * self.nz
+            assert lin.size == self.nxy * self.nLin * self.nz
         if mf is not None:
             assert mf.size == self.nMF * self.nz
         self._lib.orc_set_state(self._h, _dp(lin) if lin is not None else None,
@@ -160,7 +160,7 @@ class Oracle:
                                 _dp(kx0) if kx0 is not None else None)
 
     def get_state(self):
-        lin = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        lin = np.zeros((self.nxy, self.nLin, self.nz), dtype=np.complex128)
         mf = np.zeros((self.nMF, self.nz), dtype=np.complex128)
         kx0 = np.zeros(self.nxy)
         self._lib.orc_get_state(self._h, _dp(lin), _dp(mf), _dp(kx0))
@@ -176,15 +176,15 @@ class Oracle:
         return kx0, ky, kxi, kyi, kz
 
     def linear_ops_init(self, t0=0.0):
-        L = np.zeros((self.nxy, 4, self.nz))
+        L = np.zeros((self.nxy, self.nLin, self.nz))
         Lmf = np.zeros((self.nMF, self.nz))
         self._lib.orc_linear_ops_init(self._h, t0, _dp(L), _dp(Lmf))
         return L, Lmf
 
     def rhs(self, t, dt_lin):
-        out = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        out = np.zeros((self.nxy, self.nLin, self.nz), dtype=np.complex128)
         omf = np.zeros((self.nMF, self.nz), dtype=np.complex128)
-        L = np.zeros((self.nxy, 4, self.nz))
+        L = np.zeros((self.nxy, self.nLin, self.nz))
         self._lib.orc_rhs(self._h, t, dt_lin, _dp(out), _dp(omf), _dp(L))
         return out, omf, L
 

@@ -153,6 +153,7 @@ static int model_nMF(int model)
 {
     switch (model) {
     case ORC_MODEL_MHD_BQLIN: return 2;
+    case ORC_MODEL_HD_FULLU: return 2; /* Ux, Uy (HD_fullU.cpp:14) */
     case ORC_MODEL_CONSTANT_DAMPING: return 1;
     default: return 4;
     }
@@ -208,7 +209,9 @@ orc *orc_create(const orc_params *p)
     o->nzl = (o->nz - 1) / 2;
     o->half = (o->nz - 1) / 2 + 1;               /* MHD_FullQlin.cpp:86 */
     o->nMF = model_nMF(p->model);
-    o->nLin = 4;
+    /* HD_fullU has 2 fluctuation fields (u, zeta; HD_fullU.cpp:14); they are
+     * stored in the first 2 of 4 slots per mode, the other 2 stay zero */
+    o->nLin = p->model == ORC_MODEL_HD_FULLU ? 2 : 4;
     o->fac1D = 1.0 / p->NZ;                      /* fftwPlans.cpp:30 */
     o->rs_ft_fac = 1.0 / (p->NY * p->NX) / (p->NY * p->NX); /* MHD_FullQlin.cpp:67 */
     o->totalN2 = (long)(p->NX * p->NY * p->NZ);
@@ -323,16 +326,34 @@ void orc_dims(const orc *o, int *d)
     d[0] = o->nxy; d[1] = o->nz; d[2] = o->nMF; d[3] = o->nLin; d[4] = o->half; d[5] = o->NZ;
 }
 
+/* external [nxy][nLin][nz] <-> the 4-slot storage */
+static void lin_in(const orc *o, cplx *dst, const cplx *src)
+{
+    const size_t nz = o->nz, nL = o->nLin;
+    if (nL == 4) {
+        memcpy(dst, src, sizeof(cplx) * o->nxy * 4 * nz);
+        return;
+    }
+    memset(dst, 0, sizeof(cplx) * o->nxy * 4 * nz);
+    for (int i = 0; i < o->nxy; ++i) memcpy(dst + (size_t)i * 4 * nz, src + (size_t)i * nL * nz, sizeof(cplx) * nL * nz);
+}
+static void lin_out(const orc *o, void *dst, const void *src, size_t esz)
+{
+    const size_t nz = o->nz, nL = o->nLin;
+    for (int i = 0; i < o->nxy; ++i)
+        memcpy((char *)dst + (size_t)i * nL * nz * esz, (const char *)src + (size_t)i * 4 * nz * esz, esz * nL * nz);
+}
+
 void orc_set_state(orc *o, const double *lin, const double *mf, const double *kx0)
 {
-    if (lin) memcpy(o->lin, lin, sizeof(cplx) * (size_t)o->nxy * 4 * o->nz);
+    if (lin) lin_in(o, o->lin, (const cplx *)lin);
     if (mf) memcpy(o->mf, mf, sizeof(cplx) * (size_t)o->nMF * o->nz);
     if (kx0) memcpy(o->kx0, kx0, sizeof(double) * (size_t)o->nxy);
 }
 
 void orc_get_state(const orc *o, double *lin, double *mf, double *kx0)
 {
-    if (lin) memcpy(lin, o->lin, sizeof(cplx) * (size_t)o->nxy * 4 * o->nz);
+    if (lin) lin_out(o, lin, o->lin, sizeof(cplx));
     if (mf) memcpy(mf, o->mf, sizeof(cplx) * (size_t)o->nMF * o->nz);
     if (kx0) memcpy(kx0, o->kx0, sizeof(double) * (size_t)o->nxy);
 }
@@ -557,6 +578,82 @@ static void rhs_fullqlin_mode(orc *o, work_t *w, int i, double t, double dt_lin,
 }
 
 /* ------------------------------------------------------------------------ */
+/* HD_fullU::rhs (Models/HD_fullU.cpp:145-309)                               */
+/* ------------------------------------------------------------------------ */
+static void rhs_hd_mode(orc *o, work_t *w, int i, double t, double dt_lin, const cplx *lin, cplx *out,
+                        double *linop)
+{
+    const int nz = o->nz, NZ = o->NZ, half = o->half;
+    const double fac = o->fac1D, q = o->p.q, Om = o->omega_eff;
+    const fftplan *pl = &o->plan;
+    const cplx *kz = o->kz;
+    const cplx *MUx = o->mfz + 4 * NZ, *dzMUx = o->mfz + 5 * NZ, *MUy = o->mfz + 6 * NZ, *dzMUy = o->mfz + 7 * NZ;
+    kmode k;
+    assign_laplacians(o, i, t, 1, w, &k);
+    const cplx kxc = k.kxc, kyc = k.kyc;
+    const cplx *ux = lin + ((size_t)i * 4 + 0) * nz, *ze = lin + ((size_t)i * 4 + 1) * nz;
+    cplx *o0 = out + ((size_t)i * 4 + 0) * nz, *o1 = out + ((size_t)i * 4 + 1) * nz;
+    cplx *fy = w->fy, *fz = w->fz, *t1 = w->tmp1_k, *t2 = w->tmp2_k, *t3 = w->tmp3_k;
+    cplx **u = w->u;
+    const double *ilap2 = w->ilap2t, *ilapF = w->ilapF;
+
+    /* :184-185 */
+    for (int m = 0; m < nz; ++m) {
+        fy[m] = ((-kyc * kxc) * ux[m] + kz[m] * ze[m]) * ilap2[m];
+        fz[m] = (-kxc * kz[m] * ux[m] - kyc * ze[m]) * ilap2[m];
+    }
+    /* :189-206 */
+    const cplx *src[3] = {ux, fy, fz};
+    for (int d = 0; d < 3; ++d) {
+        const cplx f = d == 0 ? fac : (d == 1 ? fac * kxc : fac * kyc);
+        for (int c = 0; c < 3; ++c) {
+            for (int m = 0; m < nz; ++m) t1[m] = f * src[c][m];
+            fft_inverse(pl, nz, t1, u[3 * d + c]);
+        }
+    }
+    /* :213-228 */
+    cplx *z1 = w->tmp1_z;
+    for (int j = 0; j < NZ; ++j) z1[j] = MUx[j] * u[3][j] + MUy[j] * u[6][j] + u[2][j] * dzMUx[j];
+    fft_forward(pl, nz, z1, t1);
+    for (int j = 0; j < NZ; ++j) z1[j] = MUx[j] * u[4][j] + MUy[j] * u[7][j] + u[2][j] * dzMUy[j];
+    fft_forward(pl, nz, z1, t2);
+    for (int j = 0; j < NZ; ++j) z1[j] = MUx[j] * u[5][j] + MUy[j] * u[8][j];
+    fft_forward(pl, nz, z1, t3);
+    for (int m = 0; m < nz; ++m) {
+        o0[m] = ((-2.0 * kxc * kyc * q) * ux[m] + 2 * Om * kz[m] * ze[m] + (kxc * kxc) * t1[m] +
+                 (kxc * kyc) * t2[m] + kxc * kz[m] * t3[m]) * ilapF[m] - t1[m];
+        o1[m] = (q - 2.0 * Om) * kz[m] * ux[m] - (kz[m] * t2[m] - kyc * t3[m]);
+    }
+    /* Reynolds stresses, :235-264 */
+    double mult_fac = 2.0;
+    if (k.ky == 0.0) mult_fac = 1.0;
+    double ftfac = mult_fac * o->rs_ft_fac;
+    cplx *rz = w->rey_z, *send = w->send;
+    for (int m = 0; m < nz; ++m) t1[m] = fac * kz[m] * ux[m];
+    fft_inverse(pl, nz, t1, z1);
+    for (int j = 0; j < NZ; ++j)
+        rz[j] = -(creal(u[0][j] * conj(u[3][j])) + creal(u[1][j] * conj(u[6][j])) + creal(u[2][j] * conj(z1[j])));
+    fft_forward(pl, nz, rz, t1);
+    for (int m = 0; m < half; ++m) send[m] += ftfac * t1[m];
+    for (int m = 0; m < nz; ++m) fz[m] = ((-kyc * kxc) * ux[m] + kz[m] * ze[m]) * ilap2[m];
+    for (int m = 0; m < nz; ++m) t1[m] = fac * kz[m] * fz[m];
+    fft_inverse(pl, nz, t1, z1);
+    for (int j = 0; j < NZ; ++j)
+        rz[j] = -(creal(u[0][j] * conj(u[4][j])) + creal(u[1][j] * conj(u[7][j])) + creal(u[2][j] * conj(z1[j])));
+    fft_forward(pl, nz, rz, t1);
+    for (int m = 0; m < half; ++m) send[half + m] += ftfac * t1[m];
+    /* linear part, :270-273 */
+    double kxt = k.kx + o->p.q * dt_lin * k.ky;
+    double *L0 = linop + ((size_t)i * 4 + 0) * nz;
+    for (int m = 0; m < nz; ++m) {
+        L0[m] = o->p.nu * ((-kxt * kxt - k.ky * k.ky) + o->kz2[m]);
+        L0[nz + m] = L0[m];
+        L0[2 * nz + m] = 0.0;
+        L0[3 * nz + m] = 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* MHD_BQlin::rhs (Models/MHD_BQlin.cpp:117-284)                             */
 /* ------------------------------------------------------------------------ */
 static void rhs_bqlin_mode(orc *o, work_t *w, int i, double t, double dt_lin,
@@ -706,6 +803,14 @@ static void model_rhs(orc *o, double t, double dt_lin, const cplx *lin, const cp
             for (int m = 0; m < nz; ++m) tmp[m] = fac * o->kz[m] * mf[(size_t)f * nz + m];
             fft_inverse(&o->plan, nz, tmp, o->mfz + (size_t)(2 * f + 1) * NZ);
         }
+    } else if (model == ORC_MODEL_HD_FULLU) {
+        /* HD_fullU.cpp:153-162: MUx, dzMUx, MUy, dzMUy (kept at profile lines 4-7) */
+        for (int f = 0; f < 2; ++f) {
+            for (int m = 0; m < nz; ++m) tmp[m] = mf[(size_t)f * nz + m] * fac;
+            fft_inverse(&o->plan, nz, tmp, o->mfz + (size_t)(4 + 2 * f) * NZ);
+            for (int m = 0; m < nz; ++m) tmp[m] = fac * o->kz[m] * mf[(size_t)f * nz + m];
+            fft_inverse(&o->plan, nz, tmp, o->mfz + (size_t)(5 + 2 * f) * NZ);
+        }
     } else if (model == ORC_MODEL_MHD_BQLIN) {
         /* MHD_BQlin.cpp:122-127 */
         const cplx *By = mf + nz;
@@ -733,6 +838,7 @@ static void model_rhs(orc *o, double t, double dt_lin, const cplx *lin, const cp
             switch (model) {
             case ORC_MODEL_MHD_FULLQLIN: rhs_fullqlin_mode(o, w, i, t, dt_lin, lin, out_lin, linop); break;
             case ORC_MODEL_MHD_BQLIN: rhs_bqlin_mode(o, w, i, t, dt_lin, lin, out_lin, linop); break;
+            case ORC_MODEL_HD_FULLU: rhs_hd_mode(o, w, i, t, dt_lin, lin, out_lin, linop); break;
             case ORC_MODEL_CONSTANT_DAMPING: rhs_damping_mode(o, i, lin, out_lin, linop); break;
             default: rhs_linear_mode(o, w, i, t, dt_lin, out_lin, linop); break;
             }
@@ -754,6 +860,12 @@ static void model_rhs(orc *o, double t, double dt_lin, const cplx *lin, const cp
             out_mf[m] = B0z * kz[m] * M2[m] + D0[m];
             out_mf[2 * nz + m] = 2 * Om * M3[m] + B0z * kz[m] * M0[m] + D2[m];
             out_mf[3 * nz + m] = (q - 2.0 * Om) * M2[m] + B0z * kz[m] * M1[m] + D3[m];
+        }
+    } else if (model == ORC_MODEL_HD_FULLU && o->p.quasilinear) {
+        /* HD_fullU.cpp:297-305 */
+        for (int m = 0; m < nz; ++m) {
+            out_mf[m] = 2 * Om * mf[nz + m] + o->drive[m];
+            out_mf[nz + m] = (q - 2.0 * Om) * mf[m] + o->drive[nz + m];
         }
     } else if (model == ORC_MODEL_MHD_BQLIN && o->p.quasilinear) {
         /* MHD_BQlin.cpp:274-276 */
@@ -782,7 +894,13 @@ void orc_linear_ops_init(orc *o, double t0, double *linop_out, double *linop_mf_
         }
         assign_laplacians(o, i, t0, 0, w, &k);
         for (int m = 0; m < nz; ++m) {
-            if (o->p.model == ORC_MODEL_MHD_BQLIN) {
+            if (o->p.model == ORC_MODEL_HD_FULLU) {
+                /* HD_fullU.cpp:314-318 */
+                L0[m] = o->p.nu * w->lapF[m];
+                L0[nz + m] = L0[m];
+                L0[2 * nz + m] = 0.0;
+                L0[3 * nz + m] = 0.0;
+            } else if (o->p.model == ORC_MODEL_MHD_BQLIN) {
                 /* MHD_BQlin.cpp:289-295 */
                 L0[m] = o->p.nu * w->lapF[m];
                 L0[nz + m] = L0[m];
@@ -805,18 +923,20 @@ void orc_linear_ops_init(orc *o, double t0, double *linop_out, double *linop_mf_
                 v = (f < 2 ? o->p.eta : o->p.nu) * o->kz2[m]; /* MHD_FullQlin.cpp:444-447 */
             else if (o->p.quasilinear && o->p.model == ORC_MODEL_MHD_BQLIN)
                 v = o->p.eta * o->kz2[m]; /* MHD_BQlin.cpp:298-299 */
+            else if (o->p.quasilinear && o->p.model == ORC_MODEL_HD_FULLU)
+                v = o->p.nu * o->kz2[m]; /* HD_fullU.cpp:321-323 */
             o->linop_mf[(size_t)f * nz + m] = v;
         }
-    if (linop_out) memcpy(linop_out, Lf, sizeof(double) * (size_t)nxy * 4 * nz);
+    if (linop_out) lin_out(o, linop_out, Lf, sizeof(double));
     if (linop_mf_out) memcpy(linop_mf_out, o->linop_mf, sizeof(double) * (size_t)nMF * nz);
 }
 
 void orc_rhs(orc *o, double t, double dt_lin, double *rhs_lin, double *rhs_mf, double *linop)
 {
     model_rhs(o, t, dt_lin, o->lin, o->mf, o->rhs1_lin, o->rhs1_mf, o->linop);
-    if (rhs_lin) memcpy(rhs_lin, o->rhs1_lin, sizeof(cplx) * (size_t)o->nxy * 4 * o->nz);
+    if (rhs_lin) lin_out(o, rhs_lin, o->rhs1_lin, sizeof(cplx));
     if (rhs_mf) memcpy(rhs_mf, o->rhs1_mf, sizeof(cplx) * (size_t)o->nMF * o->nz);
-    if (linop) memcpy(linop, o->linop, sizeof(double) * (size_t)o->nxy * 4 * o->nz);
+    if (linop) lin_out(o, linop, o->linop, sizeof(double));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -854,6 +974,31 @@ double orc_calculate_cfl(orc *o)
         double Bxmax = max_abs(o->mfz + 0 * NZ, NZ), Bymax = max_abs(o->mfz + 2 * NZ, NZ);
         double Uxmax = max_abs(o->mfz + 4 * NZ, NZ), Uymax = max_abs(o->mfz + 6 * NZ, NZ);
         return kmax * (Bymax + Bxmax + Uxmax + Uymax) + o->p.q;
+    }
+    if (o->p.model == ORC_MODEL_HD_FULLU) {
+        /* HD_fullU.cpp:604-628: profiles and drives of the last rhs, Ux of the
+         * current state */
+        double Uxmax = max_abs(o->mfz + 4 * NZ, NZ), Uymax = max_abs(o->mfz + 6 * NZ, NZ);
+        double MFcont = 0.;
+        if (o->p.quasilinear) {
+            double e = 0, e2 = 0;
+            for (int j = 0; j < NZ; ++j) {
+                cplx a = o->mfz[4 * NZ + j], b = o->mfz[6 * NZ + j];
+                e += creal(a) * creal(a) + cimag(a) * cimag(a);
+                e2 += creal(b) * creal(b) + cimag(b) * cimag(b);
+            }
+            if ((e + e2) * o->p.L[2] > 1e-6) {
+                double dx = 0, dy = 0, ux = 0;
+                for (int m = 0; m < o->nz; ++m) {
+                    dx += cabs(o->drive[m] * o->kz2[m]);
+                    dy += cabs(o->drive[o->nz + m] * o->kz2[m]);
+                    ux += cabs(o->mf[m] * o->kz2[m]);
+                }
+                dx /= o->nz; dy /= o->nz; ux /= o->nz;
+                MFcont = dx / ux + dy / ux;
+            }
+        }
+        return kmax * (Uxmax + Uymax) + o->p.q + MFcont;
     }
     if (o->p.model == ORC_MODEL_MHD_BQLIN) {
         /* MHD_BQlin.cpp:544-550: By_ of the last rhs, Bx of the current state */
@@ -1081,6 +1226,7 @@ void orc_set_noise(orc *o, double f_noise, double range_low, double range_high, 
     o->f_noise = f_noise;
     o->noise_range[0] = range_low * range_low;
     o->noise_range[1] = range_high * range_high;
+    /* only MHD_FullQlin reads the flags (MHD_FullQlin.cpp:21-22) */
     o->drive_only_velocity = o->p.model == ORC_MODEL_MHD_FULLQLIN ? drive_only_velocity : 0;
     o->drive_only_magnetic = o->p.model == ORC_MODEL_MHD_FULLQLIN ? drive_only_magnetic : 0;
     o->noise_seed = seed;
@@ -1095,8 +1241,9 @@ static void noise_mode(orc *o, int i, int ip, int jj0, double t, double dt, work
     assign_laplacians(o, i, t, 0, w, &k);
     double noise_multfac = dt * o->totalN2 * o->mult_noise_fac;
     const double sigma = o->f_noise / sqrt(2.0); /* normal_distribution(0, f/sqrt 2), :47 */
-    const int on[4] = {!o->drive_only_magnetic, !o->drive_only_magnetic, !o->drive_only_velocity,
-                       !o->drive_only_velocity};
+    const int hd = o->p.model == ORC_MODEL_HD_FULLU; /* u and zeta only (HD_fullU.cpp:635-852) */
+    const int on[4] = {!o->drive_only_magnetic, !o->drive_only_magnetic, !o->drive_only_velocity && !hd,
+                       !o->drive_only_velocity && !hd};
     for (int jj = jj0; jj < nz; ++jj) {
         double lapF = w->lapF[jj], lap2 = w->lap2t[jj];
         if (!(lapF > -o->noise_range[1] && lapF < -o->noise_range[0])) continue; /* :806 */
@@ -1214,6 +1361,13 @@ void orc_diagnostics(orc *o, double t, double *out)
     energy_MB = (s0 + s1) * divfavMF;
     AM_MB = divfavMF * am_b;
     diss_MB = (divfavMF * eta) * (-d0 - d1);
+    if (o->p.model == ORC_MODEL_HD_FULLU) {
+        /* HD_fullU.cpp:455-460: the two mean fields are Ux, Uy */
+        energy_MU = energy_MB;
+        AM_MU = AM_MB;
+        diss_MU = (divfavMF * nu) * (-d0 - d1);
+        energy_MB = AM_MB = diss_MB = 0;
+    }
     if (o->nMF == 4) {
         energy_MU = (s2 + s3) * divfavMF;
         AM_MU = divfavMF * am_u;
@@ -1224,7 +1378,7 @@ void orc_diagnostics(orc *o, double t, double *out)
     out[8] = diss_MU; out[9] = diss_MB; out[10] = diss_u; out[11] = diss_b;
     double kzfac = 2 * QL_PI / o->p.L[2] * (o->NZ / 3);
     out[16] = pow(nu * nu * nu / (diss_MU + diss_u), 0.25) * kzfac;
-    out[17] = pow(eta * eta * eta / (diss_MB + diss_b), 0.25) * kzfac;
+    out[17] = o->p.model == ORC_MODEL_HD_FULLU ? 0.0 : pow(eta * eta * eta / (diss_MB + diss_b), 0.25) * kzfac;
     if (o->p.model == ORC_MODEL_MHD_FULLQLIN) {
         /* :669-676 */
         for (int f = 0; f < 4; ++f) {

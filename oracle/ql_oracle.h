@@ -12,6 +12,8 @@
  *                              assign_laplacians_ :722-738)
  *   Models/MHD_BQlin.cpp      (rhs :117-284, linearOPs_Init :286-306,
  *                              Calc_Energy_AM_Diss :337-488)
+ *   Models/HD_fullU.cpp       (rhs :145-309, linearOPs_Init :311-327,
+ *                              Calc_Energy_AM_Diss :360-550, CFL :604-628)
  *   Models/ConstantDamping.cpp (rhs :73-111)            -- integrator pin
  *   Integrators/RK3CN.cpp      (Step :77-201)
  *   Integrators/RK3CN_Elin.cpp (Step :74-188)
@@ -40,7 +42,7 @@ extern "C" {
 #endif
 
 enum { ORC_MODEL_MHD_FULLQLIN = 0, ORC_MODEL_MHD_BQLIN = 1,
-       ORC_MODEL_CONSTANT_DAMPING = 2, ORC_MODEL_LINEAR_ONLY = 3 };
+       ORC_MODEL_CONSTANT_DAMPING = 2, ORC_MODEL_LINEAR_ONLY = 3, ORC_MODEL_HD_FULLU = 4 };
 enum { ORC_INT_RK3CN = 0, ORC_INT_RK3CN_ELIN = 1 };
 
 typedef struct orc_params {

@@ -25,7 +25,7 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_linear_ops_init", "ql_batch_steps", "ql_batch_tune", "ql_batch_get_config",
            "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward", "ql_calculate_cfl",
            "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution", "ql_set_noise",
-           "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise")
+           "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise", "ql_drive_real")
 
 QL_ERR_STATE = -4
 
@@ -105,6 +105,7 @@ def load(path: str = LIB_PATH):
         "ql_adjoint_forward": ([P, ctypes.c_int, ctypes.c_int, D], ctypes.c_int),
         "ql_adjoint_backward": ([P, D, D, D, D, D], ctypes.c_int),
         "ql_calculate_cfl": ([P, D], ctypes.c_int),
+        "ql_drive_real": ([P, D], ctypes.c_int),
         "ql_check_solution": ([P, D, I], ctypes.c_int),
         "ql_batch_calculate_cfl": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_batch_check_solution": ([P, ctypes.c_int, D, I], ctypes.c_int),
@@ -206,28 +207,28 @@ class QLContext:
         self.close()
 
     def set_state(self, lin=None, mf=None, kx0=None):
-        lin = _checked(lin, np.complex128, self.nxy * 4 * self.nz)
+        lin = _checked(lin, np.complex128, self.nxy * self.nLin * self.nz)
         mf = _checked(mf, np.complex128, self.nMF * self.nz)
         kx0 = _checked(kx0, np.float64, self.nxy)
         self._check(self._L.ql_set_state(self._h, _dp(lin), _dp(mf), _dp(kx0)))
 
     def get_state(self):
-        lin = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        lin = np.zeros((self.nxy, self.nLin, self.nz), dtype=np.complex128)
         mf = np.zeros((self.nMF, self.nz), dtype=np.complex128)
         kx0 = np.zeros(self.nxy)
         self._check(self._L.ql_get_state(self._h, _dp(lin), _dp(mf), _dp(kx0)))
         return lin, mf, kx0
 
     def linear_ops_init(self, t0: float = 0.0):
-        L = np.zeros((self.nxy, 4, self.nz))
+        L = np.zeros((self.nxy, self.nLin, self.nz))
         Lmf = np.zeros((self.nMF, self.nz))
         self._check(self._L.ql_linear_ops_init(self._h, t0, _dp(L), _dp(Lmf)))
         return L, Lmf
 
     def rhs(self, t: float, dt_lin: float):
-        out = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        out = np.zeros((self.nxy, self.nLin, self.nz), dtype=np.complex128)
         omf = np.zeros((self.nMF, self.nz), dtype=np.complex128)
-        L = np.zeros((self.nxy, 4, self.nz))
+        L = np.zeros((self.nxy, self.nLin, self.nz))
         self._check(self._L.ql_rhs(self._h, t, dt_lin, _dp(out), _dp(omf), _dp(L)))
         return out, omf, L
 
@@ -257,6 +258,12 @@ class QLContext:
     def mf_real(self):
         out = np.zeros((self.nMF, self.NZ))
         self._check(self._L.ql_mf_real(self._h, _dp(out)))
+        return out
+
+    def drive_real(self):
+        """fac1D Re(ifft(pad(drive))) of the last rhs: [nMF][NZ]."""
+        out = np.zeros((self.nMF, self.NZ))
+        self._check(self._L.ql_drive_real(self._h, _dp(out)))
         return out
 
     def calculate_cfl(self) -> float:
@@ -356,14 +363,14 @@ class QLBatch:
 
     def set_state(self, run: int, lin=None, mf=None, kx0=None):
         d = self.dims[run]
-        lin = _checked(lin, np.complex128, d["nxy"] * 4 * d["nz"])
+        lin = _checked(lin, np.complex128, d["nxy"] * d["nLin"] * d["nz"])
         mf = _checked(mf, np.complex128, d["nMF"] * d["nz"])
         kx0 = _checked(kx0, np.float64, d["nxy"])
         self._check(self._L.ql_batch_set_state(self._h, run, _dp(lin), _dp(mf), _dp(kx0)))
 
     def get_state(self, run: int):
         d = self.dims[run]
-        lin = np.zeros((d["nxy"], 4, d["nz"]), dtype=np.complex128)
+        lin = np.zeros((d["nxy"], d["nLin"], d["nz"]), dtype=np.complex128)
         mf = np.zeros((d["nMF"], d["nz"]), dtype=np.complex128)
         kx0 = np.zeros(d["nxy"])
         self._check(self._L.ql_batch_get_state(self._h, run, _dp(lin), _dp(mf), _dp(kx0)))

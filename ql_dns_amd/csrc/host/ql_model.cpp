@@ -12,7 +12,8 @@ int model_id(const std::string &s)
 {
     if (s == "MHD_FullQlin") return QL_MODEL_MHD_FULLQLIN;
     if (s == "MHD_BQlin") return QL_MODEL_MHD_BQLIN;
-    throw Error(QL_ERR_ARG, "model '" + s + "' is not available on the GPU path (MHD_FullQlin, MHD_BQlin)");
+    if (s == "HD_fullU") return QL_MODEL_HD_FULLU;
+    throw Error(QL_ERR_ARG, "model '" + s + "' is not available on the GPU path (MHD_FullQlin, MHD_BQlin, HD_fullU)");
 }
 
 int integrator_id(const std::string &s)
@@ -33,8 +34,8 @@ void TimeVariables::Save_Data(double t)
         r.energy[k] = energy_[k];
         r.AM[k] = AM_[k];
         r.diss[k] = diss_[k];
-        r.reynolds[k] = reynolds_[k];
     }
+    r.reynolds = reynolds_;
     records_.push_back(r);
 }
 
@@ -42,6 +43,7 @@ GpuModel::GpuModel(const Inputs &sp) : Model(sp.NZ, sp.NXY, sp.L), sp_(sp), f_no
 {
     ql_params p{};
     p.model = model_id(sp.equations_to_use);
+    hd_ = p.model == QL_MODEL_HD_FULLU;
     p.integrator = integrator_id(sp.integrator);
     p.precision = QL_PREC_FP64;
     p.NX = sp.NXY[0];
@@ -80,27 +82,27 @@ void GpuModel::check(int rc) const
 void GpuModel::rhs(const double t, const double dt_lin, solution *SolIn, solution *SolOut, doubVec **linOpFluct)
 {
     SolIn->push();
-    const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs();
-    std::vector<dcmplx> out((size_t)nxy * 4 * nz), omf((size_t)nMF * nz);
-    std::vector<double> L((size_t)nxy * 4 * nz);
+    const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs(), nL = num_Lin();
+    std::vector<dcmplx> out((size_t)nxy * nL * nz), omf((size_t)nMF * nz);
+    std::vector<double> L((size_t)nxy * nL * nz);
     check(ql_rhs(ctx_, t, dt_lin, reinterpret_cast<double *>(out.data()), reinterpret_cast<double *>(omf.data()),
                  L.data()));
     if (SolOut) SolOut->unpack(out, omf);
     if (linOpFluct)
         for (int i = 0; i < nxy; ++i)
-            for (int j = 0; j < 4; ++j)
-                for (int m = 0; m < nz; ++m) linOpFluct[i][j](m) = L[((size_t)i * 4 + j) * nz + m];
+            for (int j = 0; j < nL; ++j)
+                for (int m = 0; m < nz; ++m) linOpFluct[i][j](m) = L[((size_t)i * nL + j) * nz + m];
 }
 
 void GpuModel::linearOPs_Init(double t0, doubVec **linOpFluct, doubVec *linOpMF)
 {
-    const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs();
-    std::vector<double> L((size_t)nxy * 4 * nz), Lmf((size_t)nMF * nz);
+    const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs(), nL = num_Lin();
+    std::vector<double> L((size_t)nxy * nL * nz), Lmf((size_t)nMF * nz);
     check(ql_linear_ops_init(ctx_, t0, L.data(), Lmf.data()));
     if (linOpFluct)
         for (int i = 0; i < nxy; ++i)
-            for (int j = 0; j < 4; ++j)
-                for (int m = 0; m < nz; ++m) linOpFluct[i][j](m) = L[((size_t)i * 4 + j) * nz + m];
+            for (int j = 0; j < nL; ++j)
+                for (int m = 0; m < nz; ++m) linOpFluct[i][j](m) = L[((size_t)i * nL + j) * nz + m];
     if (linOpMF)
         for (int f = 0; f < nMF; ++f)
             for (int m = 0; m < nz; ++m) linOpMF[f](m) = Lmf[(size_t)f * nz + m];
@@ -136,14 +138,27 @@ void GpuModel::Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *
     const_cast<solution *>(sol)->push();
     double out[18];
     check(ql_diagnostics(ctx_, t, out));
-    for (int k = 0; k < 4; ++k) {
-        tv->current_energy()[k] = out[k];
-        tv->current_AM()[k] = out[4 + k];
-        tv->current_diss()[k] = out[8 + k];
-        tv->current_reynolds()[k] = out[12 + k];
+    if (hd_) {
+        // HD_fullU.cpp:466-535: (MU, u) pairs and the full stress profiles
+        const int ix[2] = {0, 2};
+        for (int k = 0; k < 4; ++k) {
+            tv->current_energy()[k] = k < 2 ? out[ix[k]] : 0.0;
+            tv->current_AM()[k] = k < 2 ? out[4 + ix[k]] : 0.0;
+            tv->current_diss()[k] = k < 2 ? out[8 + ix[k]] : 0.0;
+        }
+        check(ql_drive_real(ctx_, tv->current_reynolds(num_reynolds_saves())));
+        tv->etaK_times_kmax[0] = out[16];
+        tv->etaK_times_kmax[1] = 0.0;
+    } else {
+        for (int k = 0; k < 4; ++k) {
+            tv->current_energy()[k] = out[k];
+            tv->current_AM()[k] = out[4 + k];
+            tv->current_diss()[k] = out[8 + k];
+            tv->current_reynolds()[k] = out[12 + k];
+        }
+        tv->etaK_times_kmax[0] = out[16];
+        tv->etaK_times_kmax[1] = out[17];
     }
-    tv->etaK_times_kmax[0] = out[16];
-    tv->etaK_times_kmax[1] = out[17];
     tv->Save_Data(t);
 }
 

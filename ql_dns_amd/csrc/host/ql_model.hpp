@@ -128,20 +128,27 @@ public:
     double *current_energy() { return energy_; }
     double *current_AM() { return AM_; }
     double *current_diss() { return diss_; }
-    double *current_reynolds() { return reynolds_; }
+    // num_reynolds_saves values (TimeVariables.cpp:30-40)
+    double *current_reynolds(int n = 4)
+    {
+        if ((int)reynolds_.size() < n) reynolds_.resize(n, 0.0);
+        return reynolds_.data();
+    }
     double etaK_times_kmax[2] = {0, 0};
     void Save_Data(double t);
     struct Record {
-        double t, energy[4], AM[4], diss[4], reynolds[4];
+        double t, energy[4], AM[4], diss[4];
+        std::vector<double> reynolds;
     };
     const std::vector<Record> &records() const { return records_; }
 
 private:
-    double energy_[4] = {0, 0, 0, 0}, AM_[4] = {0, 0, 0, 0}, diss_[4] = {0, 0, 0, 0}, reynolds_[4] = {0, 0, 0, 0};
+    double energy_[4] = {0, 0, 0, 0}, AM_[4] = {0, 0, 0, 0}, diss_[4] = {0, 0, 0, 0};
+    std::vector<double> reynolds_ = std::vector<double>(4, 0.0);
     std::vector<Record> records_;
 };
 
-// The GPU path behind the Model interface (MHD_FullQlin or MHD_BQlin).
+// The GPU path behind the Model interface (MHD_FullQlin, MHD_BQlin or HD_fullU).
 class GpuModel : public Model {
 public:
     explicit GpuModel(const Inputs &sp);
@@ -157,7 +164,8 @@ public:
     void ShearingBox_Remap(double qt, solution *sol) override;
     void DrivingNoise(double t, double dt, solution *sol) override;
     void ChangeNoiseRange(double fnoise, double kmin, double kmax) override;
-    int num_reynolds_saves() override { return 4; }
+    // HD_fullU saves the full Ux, Uy stress profiles (HD_fullU.cpp:50-52)
+    int num_reynolds_saves() override { return hd_ ? 2 * NZfull() : 4; }
     void Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *sol) override;
     double Calculate_CFL(const solution *sol) override;
 
@@ -175,6 +183,7 @@ private:
     int dims_[6] = {0, 0, 0, 0, 0, 0};
     double f_noise_ = 0.0;
     ql_noise noise_{};
+    bool hd_ = false;
 };
 
 // solution.h:23-49

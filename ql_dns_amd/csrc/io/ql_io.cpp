@@ -175,8 +175,8 @@ int qlio_layout_from_params(const ql_params *p, qlio_layout *out)
     L.NZ = p->NZ;
     L.nxy = (p->NX - 1) * (p->NY / 2);
     L.nz = 1 + 2 * (int)(p->NZ / 3.0);
-    L.nlin = 4;
-    L.nmf = (p->model == QL_MODEL_MHD_BQLIN) ? 2 : 4;
+    L.nlin = (p->model == QL_MODEL_HD_FULLU) ? 2 : 4;
+    L.nmf = (p->model == QL_MODEL_MHD_FULLQLIN) ? 4 : 2;
     for (int k = 0; k < 3; ++k) L.L[k] = p->L[k];
     if (!check_layout(&L)) return fail("qlio_layout_from_params: invalid sizes");
     *out = L;
@@ -292,10 +292,11 @@ int qlio_dat_init(const char *dir)
     return QL_OK;
 }
 
-int qlio_dat_append(const char *dir, double t, const double diag[18], int nrey, const double *mf_real, int nmf,
-                    int NZ)
+int qlio_dat_append_values(const char *dir, double t, const double energy[4], const double am[4],
+                           const double diss[4], const double *reynolds, int nrey, const double *mf_real, int nmf,
+                           int NZ)
 {
-    if (!dir || !diag) return fail("qlio_dat_append: null argument");
+    if (!dir || !energy || !am || !diss || (nrey > 0 && !reynolds)) return fail("qlio_dat_append: null argument");
     if (nrey < 0 || (mf_real && (nmf < 1 || NZ < 1))) return fail("qlio_dat_append: bad sizes");
     auto app = [&](const char *name, const double *v, size_t n) {
         std::ofstream o(join(dir, name), std::ios::binary | std::ios::app);
@@ -304,15 +305,23 @@ int qlio_dat_append(const char *dir, double t, const double diag[18], int nrey, 
         return o ? QL_OK : fail("qlio_dat_append: write to %s failed", join(dir, name).c_str());
     };
     int rc;
-    if ((rc = app("energy.dat", diag, 4))) return rc;
-    if ((rc = app("angular_momentum.dat", diag + 4, 4))) return rc;
-    if ((rc = app("dissipation.dat", diag + 8, 4))) return rc;
-    std::vector<double> rey(nrey, 0.0);
-    for (int k = 0; k < nrey && k < 4; ++k) rey[k] = diag[12 + k];
-    if ((rc = app("reynolds_stress.dat", rey.data(), rey.size()))) return rc;
+    if ((rc = app("energy.dat", energy, 4))) return rc;
+    if ((rc = app("angular_momentum.dat", am, 4))) return rc;
+    if ((rc = app("dissipation.dat", diss, 4))) return rc;
+    if ((rc = app("reynolds_stress.dat", reynolds, (size_t)nrey))) return rc;
     if ((rc = app("time_vec.dat", &t, 1))) return rc;
     if (mf_real && (rc = app("mean_fields.dat", mf_real, (size_t)nmf * NZ))) return rc;
     return QL_OK;
+}
+
+int qlio_dat_append(const char *dir, double t, const double diag[18], int nrey, const double *mf_real, int nmf,
+                    int NZ)
+{
+    if (!dir || !diag) return fail("qlio_dat_append: null argument");
+    if (nrey < 0) return fail("qlio_dat_append: bad sizes");
+    std::vector<double> rey(nrey, 0.0);
+    for (int k = 0; k < nrey && k < 4; ++k) rey[k] = diag[12 + k];
+    return qlio_dat_append_values(dir, t, diag, diag + 4, diag + 8, rey.data(), nrey, mf_real, nmf, NZ);
 }
 
 }  // extern "C"
@@ -390,14 +399,8 @@ TimeVariablesFiles::TimeVariablesFiles(const std::string &simulation_dir, bool a
 
 void TimeVariablesFiles::Save_Data(double t, TimeVariables *tv, int nrey)
 {
-    double d[18] = {0};
-    for (int k = 0; k < 4; ++k) {
-        d[k] = tv->current_energy()[k];
-        d[4 + k] = tv->current_AM()[k];
-        d[8 + k] = tv->current_diss()[k];
-        d[12 + k] = tv->current_reynolds()[k];
-    }
-    io_check(qlio_dat_append(dir_.c_str(), t, d, nrey, nullptr, 0, 0));
+    io_check(qlio_dat_append_values(dir_.c_str(), t, tv->current_energy(), tv->current_AM(), tv->current_diss(),
+                                    tv->current_reynolds(nrey), nrey, nullptr, 0, 0));
 }
 
 void TimeVariablesFiles::Save_Mean_Fields(GpuModel *eqs)

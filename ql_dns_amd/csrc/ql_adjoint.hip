@@ -64,7 +64,7 @@ int adj_line_mb(int NZ) { return NZ >= 128 ? 1 : 128 / NZ; }
 int check_adjoint(const Batch &b, const char *who)
 {
     if (b.runs.size() != 1) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is for single-run contexts", who);
-    if (b.model != QL_MODEL_MHD_FULLQLIN)
+    if (b.model != QL_MODEL_MHD_FULLQLIN || b.hd)
         return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin only", who);
     if (b.precision != QL_PREC_FP64) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint runs in FP64 only", who);
     if (b.vdt) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs fixed steps (dt > 0)", who);

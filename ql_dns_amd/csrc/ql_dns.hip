@@ -80,7 +80,7 @@ int sync_all(Batch &b)
 // ---------------------------------------------------------------------------
 int check_params(const ql_params *p, const char *who)
 {
-    if (p->model != QL_MODEL_MHD_FULLQLIN && p->model != QL_MODEL_MHD_BQLIN)
+    if (p->model != QL_MODEL_MHD_FULLQLIN && p->model != QL_MODEL_MHD_BQLIN && p->model != QL_MODEL_HD_FULLU)
         return fail(QL_ERR_ARG, "%s: unknown model %d", who, p->model);
     if (p->integrator != QL_INT_RK3CN && p->integrator != QL_INT_RK3CN_ELIN)
         return fail(QL_ERR_ARG, "%s: unknown integrator %d", who, p->integrator);
@@ -99,9 +99,27 @@ int check_params(const ql_params *p, const char *who)
     return QL_OK;
 }
 
-// sizes and wavenumber tables of one run (Model.h:32-36, Kdata.cpp:28-80)
-void fill_run(RunInfo &r, const ql_params &p)
+// HD_fullU (HD_fullU.cpp:12-21, 145-327) is MHD_FullQlin with bx = eta = 0,
+// Bx = By = 0, B0z = 0 and viscosity order 2 (its operator nu (-kx^2 - ky^2 +
+// kz2), :272): every magnetic term is then a product with an exact zero and
+// the magnetic fields stay zero
+ql_params kernel_params(const ql_params &p)
 {
+    ql_params k = p;
+    if (p.model == QL_MODEL_HD_FULLU) {
+        k.model = QL_MODEL_MHD_FULLQLIN;
+        k.B0z = 0.0;
+        k.viscosity_order = 2;
+        k.eta = p.nu;
+    }
+    return k;
+}
+
+// sizes and wavenumber tables of one run (Model.h:32-36, Kdata.cpp:28-80)
+void fill_run(RunInfo &r, const ql_params &pu)
+{
+    const ql_params p = kernel_params(pu);
+    r.hd = pu.model == QL_MODEL_HD_FULLU;
     r.p = p;
     r.nxy = (p.NX - 1) * (p.NY / 2);
     r.nz = 1 + 2 * (int)(p.NZ / 3.0);
@@ -159,6 +177,8 @@ qlk::RunParams run_params(const RunInfo &r)
     rp.kmax = r.kmax;
     rp.cfl = r.p.CFL;
     rp.dtmax = r.p.dt < 0 ? -r.p.dt : r.p.dt;
+    rp.Lz = r.p.L[2];
+    rp.hd = r.hd ? 1 : 0;
     return rp;
 }
 
@@ -222,17 +242,20 @@ void noise_params(qlk::NoiseParams &n, const RunInfo &r, const ql_noise &c)
     const double mnf = 1.0 / (16 * 32 * 32);
     n.nfac = r.totalN2 * (mnf * mnf);
     n.seed = c.seed;
-    const bool full = r.p.model == QL_MODEL_MHD_FULLQLIN;
+    // the flags are MHD_FullQlin's; HD_fullU drives u and zeta, ky = 0 included
+    // (HD_fullU.cpp:635-852); MHD_BQlin everything but ky = 0
+    const bool full = r.p.model == QL_MODEL_MHD_FULLQLIN && !r.hd;
     n.drive_u = full ? !c.drive_only_magnetic : 1;
-    n.drive_b = full ? !c.drive_only_velocity : 1;
-    n.drive_ky0 = full ? 1 : 0;
+    n.drive_b = r.hd ? 0 : (full ? !c.drive_only_velocity : 1);
+    n.drive_ky0 = r.p.model == QL_MODEL_MHD_FULLQLIN ? 1 : 0;
     n.nyh = r.p.NY / 2;
     n.nxh = r.p.NX / 2;
 }
 
 int create_batch(Batch &b, const ql_params *ps, int nruns)
 {
-    const ql_params &p0 = ps[0];
+    const ql_params p0 = kernel_params(ps[0]);
+    b.hd = ps[0].model == QL_MODEL_HD_FULLU;
     b.model = p0.model;
     b.integrator = p0.integrator;
     b.precision = p0.precision;
@@ -623,6 +646,28 @@ int run_steps(Batch &b, int nsteps, double *t_inout)
     return QL_OK;
 }
 
+// the caller's layout: fluctuation fields per mode, mean fields, and the
+// first internal mean-field slot the caller's mean field 0 is
+int ext_nlin(const Batch &b) { return b.hd ? 2 : 4; }
+int ext_nmf(const Batch &b) { return b.hd ? 2 : b.nMF; }
+int ext_mf0(const Batch &b) { return b.hd ? 2 : 0; }
+
+// [modes][4][nz] <-> [modes][nL][nz] (the internal slots past nL are zero)
+template <class T>
+void lin_to_int(const T *src, T *dst, int modes, int nL, int nz)
+{
+    for (int i = 0; i < modes; ++i) {
+        std::copy(src + (size_t)i * nL * nz, src + (size_t)(i + 1) * nL * nz, dst + (size_t)i * 4 * nz);
+        std::fill(dst + (size_t)i * 4 * nz + (size_t)nL * nz, dst + (size_t)(i + 1) * 4 * nz, T{});
+    }
+}
+template <class T>
+void lin_to_ext(const T *src, T *dst, int modes, int nL, int nz)
+{
+    for (int i = 0; i < modes; ++i)
+        std::copy(src + (size_t)i * 4 * nz, src + (size_t)i * 4 * nz + (size_t)nL * nz, dst + (size_t)i * nL * nz);
+}
+
 int check_run(const Batch &b, int run, const char *who)
 {
     if (run < 0 || run >= (int)b.runs.size()) return fail(QL_ERR_ARG, "%s: run %d of %d", who, run, (int)b.runs.size());
@@ -634,10 +679,22 @@ int set_state(Batch &b, int run, const double *lin, const double *mf, const doub
     const RunInfo &r = b.runs[run];
     Group &g = b.groups[r.group];
     const size_t mo = (size_t)r.gi * b.nMF * g.nz;
-    if (lin)
-        QL_HIP(hipMemcpyAsync(g.u + (size_t)r.mode_off * 4 * g.nz, lin, sizeof(double2) * (size_t)r.nxy * 4 * g.nz,
+    std::vector<double2> tl, tm;  // internal layout of an HD_fullU state
+    const double2 *ul = reinterpret_cast<const double2 *>(lin), *um = reinterpret_cast<const double2 *>(mf);
+    if (b.hd && ul) {
+        tl.resize((size_t)r.nxy * 4 * g.nz);
+        lin_to_int(ul, tl.data(), r.nxy, 2, g.nz);
+        ul = tl.data();
+    }
+    if (b.hd && um) {
+        tm.assign((size_t)b.nMF * g.nz, make_double2(0.0, 0.0));
+        std::copy(um, um + 2 * g.nz, tm.begin() + 2 * g.nz);
+        um = tm.data();
+    }
+    if (ul)
+        QL_HIP(hipMemcpyAsync(g.u + (size_t)r.mode_off * 4 * g.nz, ul, sizeof(double2) * (size_t)r.nxy * 4 * g.nz,
                               hipMemcpyHostToDevice, b.s));
-    if (mf) QL_HIP(hipMemcpyAsync(g.mf + mo, mf, sizeof(double2) * b.nMF * g.nz, hipMemcpyHostToDevice, b.s));
+    if (um) QL_HIP(hipMemcpyAsync(g.mf + mo, um, sizeof(double2) * b.nMF * g.nz, hipMemcpyHostToDevice, b.s));
     if (kx0)
         QL_HIP(hipMemcpyAsync(g.kx0 + r.mode_off, kx0, sizeof(double) * r.nxy, hipMemcpyHostToDevice, b.s));
     QL_RC(refresh_profiles(b, g, b.s));
@@ -650,13 +707,18 @@ int get_state(Batch &b, int run, double *lin, double *mf, double *kx0)
     const RunInfo &r = b.runs[run];
     Group &g = b.groups[r.group];
     const size_t mo = (size_t)r.gi * b.nMF * g.nz;
+    std::vector<double2> tl(b.hd && lin ? (size_t)r.nxy * 4 * g.nz : 0);
+    double2 *dl = b.hd ? tl.data() : reinterpret_cast<double2 *>(lin);
     if (lin)
-        QL_HIP(hipMemcpyAsync(lin, g.u + (size_t)r.mode_off * 4 * g.nz, sizeof(double2) * (size_t)r.nxy * 4 * g.nz,
+        QL_HIP(hipMemcpyAsync(dl, g.u + (size_t)r.mode_off * 4 * g.nz, sizeof(double2) * (size_t)r.nxy * 4 * g.nz,
                               hipMemcpyDeviceToHost, b.s));
-    if (mf) QL_HIP(hipMemcpyAsync(mf, g.mf + mo, sizeof(double2) * b.nMF * g.nz, hipMemcpyDeviceToHost, b.s));
+    if (mf)
+        QL_HIP(hipMemcpyAsync(mf, g.mf + mo + (size_t)ext_mf0(b) * g.nz, sizeof(double2) * ext_nmf(b) * g.nz,
+                              hipMemcpyDeviceToHost, b.s));
     if (kx0)
         QL_HIP(hipMemcpyAsync(kx0, g.kx0 + r.mode_off, sizeof(double) * r.nxy, hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    if (b.hd && lin) lin_to_ext(tl.data(), reinterpret_cast<double2 *>(lin), r.nxy, 2, g.nz);
     return QL_OK;
 }
 
@@ -700,6 +762,12 @@ int diagnostics(Batch &b, int run, double t, double out[18])
     QL_HIP(hipGetLastError());
     QL_HIP(hipMemcpyAsync(out, b.dout, sizeof(double) * 18, hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    if (r.hd) {
+        // HD_fullU.cpp:463-488: no magnetic quantities, Reynolds stresses are
+        // saved as full profiles (ql_drive_real) instead of projections
+        for (int k = 12; k < 16; ++k) out[k] = 0.0;
+        out[17] = 0.0;
+    }
     return QL_OK;
 }
 
@@ -867,6 +935,24 @@ int init_batch(Batch &b, const ql_params *ps, int nruns, const char *who)
     return rc;
 }
 
+// fac1D Re(ifft(pad(v))) of the caller's mean-field slots of v -> [nMF][NZ]
+// (TimeVariables.cpp:202-228, HD_fullU.cpp:524-535)
+int profiles_real(Batch &b, const double2 *v, const Group &g, double *out)
+{
+    const int n = ext_nmf(b);
+    const double fac = 1.0 / g.NZ;
+    switch (g.NZ) {
+    case 32: qlk::k_mf_real<32><<<1, qlk::NT, 0, b.s>>>(v, n, g.tw, fac, b.mfr_tmp); break;
+    case 64: qlk::k_mf_real<64><<<1, qlk::NT, 0, b.s>>>(v, n, g.tw, fac, b.mfr_tmp); break;
+    case 128: qlk::k_mf_real<128><<<1, qlk::NT, 0, b.s>>>(v, n, g.tw, fac, b.mfr_tmp); break;
+    default: qlk::k_mf_real<256><<<1, qlk::NT, 0, b.s>>>(v, n, g.tw, fac, b.mfr_tmp); break;
+    }
+    QL_HIP(hipGetLastError());
+    QL_HIP(hipMemcpyAsync(out, b.mfr_tmp, sizeof(double) * n * g.NZ, hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipStreamSynchronize(b.s));
+    return QL_OK;
+}
+
 }  // namespace qlh
 
 using namespace qlh;
@@ -930,7 +1016,12 @@ int ql_linear_ops_init(ql_ctx *c, double t0, double *linop, double *linop_mf)
         const RunInfo &r = b.runs[0];
         std::vector<double> k0(r.nxy);
         QL_RC(get_state(b, 0, nullptr, nullptr, k0.data()));
-        linear_ops_host(b, r, k0, r.p.q * t0, linop, linop_mf);
+        std::vector<double> L(linop ? (size_t)r.nxy * 4 * r.nz : 0), Lm(linop_mf ? (size_t)b.nMF * r.nz : 0);
+        linear_ops_host(b, r, k0, r.p.q * t0, linop ? L.data() : nullptr, linop_mf ? Lm.data() : nullptr);
+        if (linop) lin_to_ext(L.data(), linop, r.nxy, ext_nlin(b), r.nz);
+        if (linop_mf)
+            std::copy(Lm.begin() + (size_t)ext_mf0(b) * r.nz, Lm.begin() + (size_t)(ext_mf0(b) + ext_nmf(b)) * r.nz,
+                      linop_mf);
     }
     return QL_OK;
 }
@@ -954,10 +1045,19 @@ int ql_rhs(ql_ctx *c, double t, double dt_lin, double *rhs_lin, double *rhs_mf, 
     m.mode = qlk::MODE_RHS;
     m.mfn_out = g.mfN1;
     QL_RC(enqueue_sequence(b, g, a, m, b.s));
-    if (rhs_lin) QL_HIP(hipMemcpyAsync(rhs_lin, g.N1, sizeof(double2) * nl, hipMemcpyDeviceToHost, b.s));
-    if (rhs_mf) QL_HIP(hipMemcpyAsync(rhs_mf, g.mfN1, sizeof(double2) * b.nMF * g.nz, hipMemcpyDeviceToHost, b.s));
-    if (linop) QL_HIP(hipMemcpyAsync(linop, g.lin_tmp, sizeof(double) * nl, hipMemcpyDeviceToHost, b.s));
+    const int nL = ext_nlin(b);
+    std::vector<double2> tl(rhs_lin && b.hd ? nl : 0);
+    std::vector<double> tL(linop && b.hd ? nl : 0);
+    double2 *dl = b.hd ? tl.data() : reinterpret_cast<double2 *>(rhs_lin);
+    double *dL = b.hd ? tL.data() : linop;
+    if (rhs_lin) QL_HIP(hipMemcpyAsync(dl, g.N1, sizeof(double2) * nl, hipMemcpyDeviceToHost, b.s));
+    if (rhs_mf)
+        QL_HIP(hipMemcpyAsync(rhs_mf, g.mfN1 + (size_t)ext_mf0(b) * g.nz, sizeof(double2) * ext_nmf(b) * g.nz,
+                              hipMemcpyDeviceToHost, b.s));
+    if (linop) QL_HIP(hipMemcpyAsync(dL, g.lin_tmp, sizeof(double) * nl, hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    if (b.hd && rhs_lin) lin_to_ext(tl.data(), reinterpret_cast<double2 *>(rhs_lin), g.modes, nL, g.nz);
+    if (b.hd && linop) lin_to_ext(tL.data(), linop, g.modes, nL, g.nz);
     return QL_OK;
 }
 
@@ -1018,18 +1118,16 @@ int ql_mf_real(ql_ctx *c, double *out)
     Batch &b = c->bt.b;
     const RunInfo &r = b.runs[0];
     Group &g = b.groups[r.group];
-    const double2 *mf = g.mf + (size_t)r.gi * b.nMF * g.nz;
-    const double fac = 1.0 / g.NZ;
-    switch (g.NZ) {
-    case 32: qlk::k_mf_real<32><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
-    case 64: qlk::k_mf_real<64><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
-    case 128: qlk::k_mf_real<128><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
-    default: qlk::k_mf_real<256><<<1, qlk::NT, 0, b.s>>>(mf, b.nMF, g.tw, fac, b.mfr_tmp); break;
-    }
-    QL_HIP(hipGetLastError());
-    QL_HIP(hipMemcpyAsync(out, b.mfr_tmp, sizeof(double) * b.nMF * g.NZ, hipMemcpyDeviceToHost, b.s));
-    QL_HIP(hipStreamSynchronize(b.s));
-    return QL_OK;
+    return profiles_real(b, g.mf + (size_t)(r.gi * b.nMF + ext_mf0(b)) * g.nz, g, out);
+}
+
+int ql_drive_real(ql_ctx *c, double *out)
+{
+    if (!c || !out) return fail(QL_ERR_ARG, "ql_drive_real: null argument");
+    Batch &b = c->bt.b;
+    const RunInfo &r = b.runs[0];
+    Group &g = b.groups[r.group];
+    return profiles_real(b, g.drive + (size_t)(r.gi * b.nMF + ext_mf0(b)) * g.nz, g, out);
 }
 
 int ql_calculate_cfl(ql_ctx *c, double *cfl_over_dt)
@@ -1123,8 +1221,8 @@ int ql_batch_dims(const ql_batch *bt, int run, int d[6])
     const RunInfo &r = bt->b.runs[run];
     d[0] = r.nxy;
     d[1] = r.nz;
-    d[2] = bt->b.nMF;
-    d[3] = 4;
+    d[2] = ext_nmf(bt->b);
+    d[3] = ext_nlin(bt->b);
     d[4] = r.half;
     d[5] = r.p.NZ;
     return QL_OK;
@@ -1174,8 +1272,8 @@ int ql_batch_get_drive(ql_batch *bt, int run, double *drive)
     Batch &b = bt->b;
     const RunInfo &r = b.runs[run];
     Group &g = b.groups[r.group];
-    QL_HIP(hipMemcpyAsync(drive, g.drive + (size_t)r.gi * b.nMF * g.nz, sizeof(double2) * b.nMF * g.nz,
-                          hipMemcpyDeviceToHost, b.s));
+    QL_HIP(hipMemcpyAsync(drive, g.drive + (size_t)(r.gi * b.nMF + ext_mf0(b)) * g.nz,
+                          sizeof(double2) * ext_nmf(b) * g.nz, hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
     return QL_OK;
 }
@@ -1201,7 +1299,8 @@ int ql_batch_check_solution(ql_batch *bt, int run, double sums[5], int *verdict)
     Batch &b = bt->b;
     const RunInfo &r = b.runs[run];
     Group &g = b.groups[r.group];
-    const double2 *mf = g.mf + (size_t)r.gi * b.nMF * g.nz;
+    // solution.cpp:170: the caller's mean fields 1 (checked) and 0
+    const double2 *mf = g.mf + (size_t)(r.gi * b.nMF + ext_mf0(b)) * g.nz;
     switch (g.NZ) {
     case 32: qlk::k_check_sums<32><<<1, qlk::NT, 0, b.s>>>(mf, g.tw, b.dout); break;
     case 64: qlk::k_check_sums<64><<<1, qlk::NT, 0, b.s>>>(mf, g.tw, b.dout); break;

@@ -28,10 +28,14 @@ constexpr double A0 = 8.0 / 15.0, A1 = 5.0 / 12.0, A2 = 3.0 / 4.0;
 constexpr double B1 = -17.0 / 60.0, B2 = -5.0 / 12.0;
 constexpr double P1 = 8.0 / 15.0, P2 = 2.0 / 3.0;
 
-// per-run record of the mean-field profile maxima that variable steps use:
+// per-run record of the mean-field quantities that variable steps use:
 // [0, 8) max |profile line| of the current state, [8, 16) those of the
-// profiles the last right-hand side read (the reference's MBx_ ... arrays)
-constexpr int PM_CUR = 0, PM_RHS = 8, PM_STRIDE = 16;
+// profiles the last right-hand side read (the reference's MBx_ ... arrays);
+// for HD_fullU also the energy sum |MUx|^2 + |MUy|^2 of the current profiles
+// and of the last rhs's, mean |kz^2 drive| of the Ux and Uy drives of the last
+// rhs, and mean |kz^2 Ux| of the current state (HD_fullU.cpp:609-624)
+constexpr int PM_CUR = 0, PM_RHS = 8, PM_E_CUR = 16, PM_E_RHS = 17, PM_DUX = 18, PM_DUY = 19, PM_UX = 20;
+constexpr int PM_STRIDE = 24;
 
 // physical parameters of one run
 struct Phys {
@@ -51,6 +55,9 @@ struct RunParams {
     int wg_off, nwg;     // its mode-kernel workgroups = its partial rows
     double kmax;         // CFL wavenumber (MHD_FullQlin.cpp:92-97)
     double cfl, dtmax;   // CFL number and -dt of variable steps (RK3CN.cpp:20-30)
+    double Lz;
+    int hd;              // HD_fullU run (MHD_FullQlin kernels with b = B = 0)
+    int pad;
 };
 
 // one mode-kernel workgroup: its run, its mode-group range [g0, g1), and the
@@ -161,9 +168,17 @@ __host__ __device__ inline double cfl_value(const double *pm, const RunParams &r
 #pragma clang fp contract(off)
     const double *cur = pm + PM_CUR, *rhs = pm + PM_RHS;
     if (model == 0) {
+        // with B = 0 (HD_fullU) the sum is Ux + Uy exactly (HD_fullU.cpp:628)
         const double s = ((rhs[2] + rhs[0]) + rhs[4]) + rhs[6];
         const double k = rp.kmax * s;
-        return k + rp.ph.q;
+        double v = k + rp.ph.q;
+        if (rp.hd) {
+            // HD_fullU.cpp:614-624: mean-field source term above an energy cutoff
+            double mf = 0.0;
+            if (rp.ph.quasilinear && pm[PM_E_RHS] * rp.Lz > 1e-6) mf = pm[PM_DUX] / pm[PM_UX] + pm[PM_DUY] / pm[PM_UX];
+            v = v + mf;
+        }
+        return v;
     }
     const double ky = rp.kmax * rhs[0], kx = rp.kmax * cur[3];
     const double s = ky + kx;
@@ -264,6 +279,20 @@ __device__ __forceinline__ void clock_advance(Clock *c)
     }
 }
 
+// sum over the workgroup's threads in a fixed order (every thread gets it)
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    const int tid = threadIdx.x;
+    __syncthreads();
+    red[tid] = v;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    return red[0];
+}
+
 // ---------------------------------------------------------------------------
 // mean fields, one workgroup per run (blockIdx.x)
 // ---------------------------------------------------------------------------
@@ -296,6 +325,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     __shared__ double2 smf[4 * NZD];    // mean fields at the start of the substep
     __shared__ double2 sdrive[4 * NZD]; // drives
     __shared__ double2 smfn[4 * NZD];   // updated mean fields
+    __shared__ double red[NT];          // block sums
     const int tid = threadIdx.x;
     const int run = blockIdx.x;
     double *pm = a.pm + (size_t)run * PM_STRIDE;
@@ -305,6 +335,8 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     // the right-hand side of this substep read the profiles of the current
     // state (Calculate_CFL reads those, MHD_FullQlin.cpp:772)
     if (a.mode != MODE_PROFILES && tid < 8) pm[PM_RHS + tid] = pm[PM_CUR + tid];
+    if (a.mode != MODE_PROFILES && tid == 8) pm[PM_E_RHS] = pm[PM_E_CUR];
+    const bool hd = a.runs[run].hd != 0;
     // the substep's mode kernel, the only reader of the clock, has finished;
     // dt is read above, before this block can change it (the next step's dt is
     // written at the end)
@@ -349,6 +381,22 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
             a.drive[mo + k] = v;
         }
         __syncthreads();
+        if (hd) {
+            // mean |kz^2 Ux_drive|, mean |kz^2 Uy_drive| (HD_fullU.cpp:619)
+            double sx = 0.0, sy = 0.0;
+            for (int m = tid; m < NZD; m += NT) {
+                const double k2 = -(skz[m] * skz[m]);
+                const double2 dx = sdrive[2 * NZD + m], dy = sdrive[3 * NZD + m];
+                sx += hypot(dx.x * k2, dx.y * k2);
+                sy += hypot(dy.x * k2, dy.y * k2);
+            }
+            sx = block_sum(sx, red);
+            sy = block_sum(sy, red);
+            if (tid == 0) {
+                pm[PM_DUX] = sx / NZD;
+                pm[PM_DUY] = sy / NZD;
+            }
+        }
         // mean-field RHS (MHD_FullQlin.cpp:411-423, MHD_BQlin.cpp:274-280) and
         // update (RK3CN_Elin.cpp:109-112 / RK3CN.cpp:112-115)
 #pragma unroll
@@ -436,6 +484,28 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
 #pragma unroll
         for (int o = 16; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o, 32));
         if (j0 == 0) pm[PM_CUR + l] = sqrt(m2);
+    }
+    if (hd) {
+        // |MUx|^2 + |MUy|^2 summed over z (HD_fullU.cpp:618) and mean |kz^2 Ux|
+        // of the current state (:619)
+        double ex = 0.0, ey = 0.0, su = 0.0;
+        for (int j = tid; j < NZ; j += NT) {
+            const double2 x = buf[4 * NZ + j], y = buf[6 * NZ + j];
+            ex += x.x * x.x + x.y * x.y;
+            ey += y.x * y.x + y.y * y.y;
+        }
+        for (int m = tid; m < NZD; m += NT) {
+            const double k2 = -(skz[m] * skz[m]);
+            const double2 u = smfn[2 * NZD + m];
+            su += hypot(u.x * k2, u.y * k2);
+        }
+        ex = block_sum(ex, red);
+        ey = block_sum(ey, red);
+        su = block_sum(su, red);
+        if (tid == 0) {
+            pm[PM_E_CUR] = ex + ey;
+            pm[PM_UX] = su / NZD;
+        }
     }
     if (a.vdt && a.advance && run == 0) {
         __syncthreads();

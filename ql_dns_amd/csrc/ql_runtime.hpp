@@ -51,7 +51,8 @@ inline int mb_for(int NZ, int line) { return NZ >= line ? 1 : line / NZ; }
 
 // one run: parameters, sizes and its place in its group
 struct RunInfo {
-    ql_params p{};
+    ql_params p{};  // as the kernels see it (HD_fullU: MHD_FullQlin with b = B = 0)
+    bool hd = false;
     int group = 0, gi = 0, mode_off = 0;
     int nxy = 0, nz = 0, nzl = 0, half = 0;
     double omega_eff = 0, rsf = 0, remap_thr = 0, remap_shift = 0, totalN2 = 0, kzfac = 0, kmax = 0;
@@ -86,6 +87,10 @@ struct AdjState;  // adjoint checkpoints and buffers (ql_adjoint.hip)
 
 struct Batch {
     int model = 0, integrator = 0, precision = 0, device = 0, nMF = 4;
+    // HD_fullU runs on the MHD_FullQlin kernels with the magnetic fields zero;
+    // at the boundary they have 2 fluctuation fields (u, zeta) and 2 mean
+    // fields (Ux, Uy, internal slots 2 and 3)
+    bool hd = false;
     double dt = 0;   // the fixed step, or the largest variable one
     int vdt = 0;     // variable steps (RK3CN.cpp:20-30): dt and coefficients on the device
     double dt_last = 0;  // the step taken last

@@ -69,7 +69,7 @@ def mode_tables(p: QLParams):
 
 def initial_state(p: QLParams, seed: int | None = None, amp: float = 1e-2,
                   mf_amp: float = 0.1, kdecay: float = 50.0):
-    """Return (lin, mf) complex128 arrays of shapes (nxy, 4, nz) and (nMF, nz).
+    """Return (lin, mf) complex128 arrays of shapes (nxy, nLin, nz) and (nMF, nz).
 
     Fluctuations: each Cartesian component of u and b is complex N(0,1) times
     amp*exp(-|k|^2/kdecay) (zero where |k|^2/kdecay > 300) with k = (kx0, ky, kz),
@@ -78,7 +78,8 @@ def initial_state(p: QLParams, seed: int | None = None, amp: float = 1e-2,
     Hermitian across the ky = 0 plane (the (kx, 0, kz) mode is the conjugate of
     (-kx, 0, -kz)), with the (kx, ky) = (0, 0) mode zero.
     Mean fields: Bx, By = mf_amp * sum_{m=1..4} (a_m cos 2 pi m z/Lz + b_m sin 2 pi m z/Lz)
-    with a, b ~ N(0,1); Ux = Uy = 0.
+    with a, b ~ N(0,1); Ux = Uy = 0.  HD_fullU (no magnetic field) keeps (ux,
+    zeta) and puts those two profiles in its mean fields Ux, Uy.
     """
     seed = p.seed if seed is None else seed
     nxy, nz = p.nxy, p.nz
@@ -125,6 +126,7 @@ def initial_state(p: QLParams, seed: int | None = None, amp: float = 1e-2,
             partner = idx[(-int(kxi[n]), 0)]
             lin[n] = np.conj(lin[partner][:, neg_kz])
 
+    lin = lin[:, :p.nLin].copy()
     mf = np.zeros((p.nMF, nz), dtype=np.complex128)
     for f in range(2):
         coeff = normals(seed, 10 + f, np.arange(2, 10, dtype=np.uint64))  # a1,b1,...,a4,b4

@@ -13,12 +13,15 @@ import math
 
 MODEL_MHD_FULLQLIN = 0
 MODEL_MHD_BQLIN = 1
+MODEL_HD_FULLU = 2
 INT_RK3CN = 0
 INT_RK3CN_ELIN = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
 
-MODEL_NAMES = {"MHD_FullQlin": MODEL_MHD_FULLQLIN, "MHD_BQlin": MODEL_MHD_BQLIN}
+MODEL_NAMES = {"MHD_FullQlin": MODEL_MHD_FULLQLIN, "MHD_BQlin": MODEL_MHD_BQLIN, "HD_fullU": MODEL_HD_FULLU}
+# the oracle's model ids (oracle/ql_oracle.h)
+_ORACLE_MODEL = {MODEL_MHD_FULLQLIN: 0, MODEL_MHD_BQLIN: 1, MODEL_HD_FULLU: 4}
 INTEGRATOR_NAMES = {"RK3CN": INT_RK3CN, "RK3CN_Elin": INT_RK3CN_ELIN}
 
 
@@ -61,22 +64,22 @@ class QLParams:
 
     @property
     def nMF(self) -> int:
-        return 2 if self.model == MODEL_MHD_BQLIN else 4
+        return 4 if self.model == MODEL_MHD_FULLQLIN else 2
 
     @property
     def nLin(self) -> int:
-        return 4
+        return 2 if self.model == MODEL_HD_FULLU else 4
 
     @property
     def state_bytes(self) -> int:
-        """FP64 fluctuation state size S = nxy * 4 fields * nz * 16 B."""
-        return self.nxy * 4 * self.nz * 16
+        """FP64 fluctuation state size S = nxy * nLin fields * nz * 16 B."""
+        return self.nxy * self.nLin * self.nz * 16
 
     def replace(self, **kw) -> "QLParams":
         return dataclasses.replace(self, **kw)
 
     def oracle_dict(self) -> dict:
-        return dict(model=self.model, integrator=self.integrator, NX=self.NX, NY=self.NY,
+        return dict(model=_ORACLE_MODEL[self.model], integrator=self.integrator, NX=self.NX, NY=self.NY,
                     NZ=self.NZ, L=(self.Lx, self.Ly, self.Lz), q=self.q, omega=self.omega,
                     nu=self.nu, eta=self.eta, B0z=self.B0z,
                     viscosity_order=self.viscosity_order, quasilinear=self.quasilinear,

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libql_io.so")
 
 EXPORTS = ("qlio_layout_from_params", "qlio_h5_save", "qlio_h5_load_latest", "qlio_h5_count", "qlio_dat_init",
-           "qlio_dat_append", "qlio_last_error")
+           "qlio_dat_append", "qlio_dat_append_values", "qlio_last_error")
 
 
 class qlio_layout(ctypes.Structure):
@@ -49,6 +49,8 @@ def load(path: str = LIB_PATH):
         "qlio_h5_count": ([S, I], ctypes.c_int),
         "qlio_dat_init": ([S], ctypes.c_int),
         "qlio_dat_append": ([S, ctypes.c_double, D, ctypes.c_int, D, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "qlio_dat_append_values": ([S, ctypes.c_double, D, D, D, D, ctypes.c_int, D, ctypes.c_int, ctypes.c_int],
+                                   ctypes.c_int),
         "qlio_last_error": ([], ctypes.c_char_p),
     }
     for name, (args, res) in sig.items():

@@ -8,7 +8,8 @@
 // unless given (dt < 0: variable steps of at most -dt, RK3CN.cpp:20-30).
 // state.bin: lin [nxy][4][nz] then mf [nMF][nz], complex128.
 // out.bin: rhs_lin, rhs_mf, linop (from the initial state at t = 0.0, dt_lin = 0.0053),
-//          then lin, mf, kx, t, and 18 diagnostics after nsteps.
+//          then lin, mf, kx, t, 18 diagnostics after nsteps, the Reynolds-save
+//          count and values.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -41,14 +42,14 @@ int main(int argc, char **argv)
         SP.omega = 1.0;
 
         GpuModel *fluidEqs = new GpuModel(SP);
-        const int nxy = fluidEqs->Dimxy(), nz = fluidEqs->NZ(), nMF = fluidEqs->num_MFs();
+        const int nxy = fluidEqs->Dimxy(), nz = fluidEqs->NZ(), nMF = fluidEqs->num_MFs(), nLin = fluidEqs->num_Lin();
         solution *sol = new solution(fluidEqs);
         solution *sol_rhs = new solution(fluidEqs);
 
         std::FILE *fi = std::fopen(argv[4], "rb");
         if (!fi) return 3;
         for (int i = 0; i < nxy; ++i)
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < nLin; ++j)
                 if (std::fread(sol->pLin(i, j)->data(), sizeof(dcmplx), nz, fi) != (size_t)nz) return 3;
         for (int f = 0; f < nMF; ++f)
             if (std::fread(sol->pMF(f)->data(), sizeof(dcmplx), nz, fi) != (size_t)nz) return 3;
@@ -56,15 +57,15 @@ int main(int argc, char **argv)
 
         std::FILE *fo = std::fopen(argv[5], "wb");
         // Model::rhs with caller-allocated linear-operator arrays (RK3CN.cpp:41-50)
-        std::vector<doubVec> Lstore((size_t)nxy * 4, doubVec(nz));
+        std::vector<doubVec> Lstore((size_t)nxy * nLin, doubVec(nz));
         std::vector<doubVec *> Lrows(nxy);
-        for (int i = 0; i < nxy; ++i) Lrows[i] = &Lstore[(size_t)i * 4];
+        for (int i = 0; i < nxy; ++i) Lrows[i] = &Lstore[(size_t)i * nLin];
         fluidEqs->rhs(0.0, 0.0053, sol, sol_rhs, Lrows.data());
         for (int i = 0; i < nxy; ++i)
-            for (int j = 0; j < 4; ++j) std::fwrite(sol_rhs->pLin(i, j)->data(), sizeof(dcmplx), nz, fo);
+            for (int j = 0; j < nLin; ++j) std::fwrite(sol_rhs->pLin(i, j)->data(), sizeof(dcmplx), nz, fo);
         for (int f = 0; f < nMF; ++f) std::fwrite(sol_rhs->pMF(f)->data(), sizeof(dcmplx), nz, fo);
         for (int i = 0; i < nxy; ++i)
-            for (int j = 0; j < 4; ++j) std::fwrite(Lrows[i][j].data(), sizeof(double), nz, fo);
+            for (int j = 0; j < nLin; ++j) std::fwrite(Lrows[i][j].data(), sizeof(double), nz, fo);
 
         // main.cpp:90-123
         double t = SP.t_start, dt;
@@ -81,7 +82,7 @@ int main(int argc, char **argv)
         sol->Check_Solution();  // main.cpp:136; throws where the reference aborts
 
         for (int i = 0; i < nxy; ++i)
-            for (int j = 0; j < 4; ++j) std::fwrite(sol->pLin(i, j)->data(), sizeof(dcmplx), nz, fo);
+            for (int j = 0; j < nLin; ++j) std::fwrite(sol->pLin(i, j)->data(), sizeof(dcmplx), nz, fo);
         for (int f = 0; f < nMF; ++f) std::fwrite(sol->pMF(f)->data(), sizeof(dcmplx), nz, fo);
         std::vector<double> kx = fluidEqs->kx();
         std::fwrite(kx.data(), sizeof(double), nxy, fo);
@@ -97,6 +98,10 @@ int main(int argc, char **argv)
         d[16] = tv.etaK_times_kmax[0];
         d[17] = tv.etaK_times_kmax[1];
         std::fwrite(d, sizeof(double), 18, fo);
+        // the model's num_reynolds_saves values (HD_fullU: 2 NZ stress profile values)
+        const double nrey = (double)fluidEqs->num_reynolds_saves();
+        std::fwrite(&nrey, sizeof(double), 1, fo);
+        std::fwrite(r.reynolds.data(), sizeof(double), (size_t)nrey, fo);
         std::fclose(fo);
         std::printf("mean time step %.6g over %d steps\n", integrator->mean_time_step(), nsteps);
         delete integrator;

@@ -17,7 +17,7 @@ BIN = os.path.join(ROOT, "tests", "cpp", "test_host_adapter")
     ("MHD_FullQlin", "RK3CN_Elin", 0.01, 1.0), ("MHD_FullQlin", "RK3CN", 0.01, 1.0),
     ("MHD_BQlin", "RK3CN_Elin", 0.01, 1.0),
     ("MHD_FullQlin", "RK3CN_Elin", -0.05, 0.5), ("MHD_FullQlin", "RK3CN", -0.05, 0.5),
-    ("MHD_BQlin", "RK3CN_Elin", -0.05, 0.5)])
+    ("MHD_BQlin", "RK3CN_Elin", -0.05, 0.5), ("HD_fullU", "RK3CN_Elin", 0.01, 1.0), ("HD_fullU", "RK3CN", -0.05, 0.5)])
 def test_main_loop_through_cpp_adapter(tmp_path, model, integrator, dt, cfl):
     from ql_dns_amd.ic import initial_state
     from ql_dns_amd.params import INTEGRATOR_NAMES, MODEL_NAMES, workload
@@ -35,7 +35,7 @@ def test_main_loop_through_cpp_adapter(tmp_path, model, integrator, dt, cfl):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = np.fromfile(out, dtype=np.float64)
-    nl, nm = p.nxy * 4 * p.nz, p.nMF * p.nz
+    nl, nm = p.nxy * p.nLin * p.nz, p.nMF * p.nz
     o_ = 0
 
     def take(n, cplx=True):
@@ -46,6 +46,8 @@ def test_main_loop_through_cpp_adapter(tmp_path, model, integrator, dt, cfl):
         return a.view(np.complex128) if cplx else a
     rhs_lin, rhs_mf, linop = take(nl), take(nm), take(nl, False)
     fl, fm, kx, t, diag = take(nl), take(nm), take(p.nxy, False), take(1, False)[0], take(18, False)
+    nrey = int(take(1, False)[0])
+    rey = take(nrey, False)
     assert o_ == raw.size
 
     o = Oracle.from_params(p, nthreads=8)
@@ -66,5 +68,20 @@ def test_main_loop_through_cpp_adapter(tmp_path, model, integrator, dt, cfl):
     assert rel_err(fl, lo.ravel()) < 1e-10
     assert rel_err(fm, mo.ravel()) < 1e-10
     do = o.diagnostics(to)
+    if model == "HD_fullU":
+        # TimeVariables of HD_fullU (HD_fullU.cpp:466-535): (MU, u) pairs, then
+        # the Ux and Uy stress profiles fac1D Re(ifft(pad(drive)))
+        from oracle.oracle import fft_inverse
+        d = o.drive()
+        prof = np.concatenate([fft_inverse(p.NZ, d[f]).real / p.NZ for f in range(2)])
+        assert nrey == 2 * p.NZ and rel_err(rey, prof) < 1e-10
+        hd = np.zeros(18)
+        for k, src in enumerate((0, 2)):
+            hd[k], hd[4 + k], hd[8 + k] = do[src], do[4 + src], do[8 + src]
+        hd[12:16] = prof[:4]
+        hd[16] = do[16]
+        do = hd
+    else:
+        assert nrey == 4
     fin = np.isfinite(do)
     assert np.all(np.abs(diag[fin] - do[fin]) <= 1e-10 * np.abs(do[fin]) + 1e-300)
