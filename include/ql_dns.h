@@ -128,8 +128,10 @@ int ql_drive_real(ql_ctx *ctx, double *out);
  * HD_fullU.cpp:604-628 with its mean-field source term):
  * CFL/dt of the next variable step, kmax (max|By| + max|Bx| + max|Ux| +
  * max|Uy|) + q over the mean-field profiles the last rhs read (MHD_BQlin:
- * kmax max|By| + kmax max|Bx| + q, Bx of the current state).  The maxima are
- * recorded on the device by every mean-field update. */
+ * kmax max|By| + kmax max|Bx| + q, Bx of the current state).  With variable
+ * steps the maxima are recorded on the device by every mean-field update; with
+ * fixed steps (where the reference never calls it) nothing is recorded while
+ * stepping and the value is that of the current state's profiles. */
 int ql_calculate_cfl(ql_ctx *ctx, double *cfl_over_dt);
 
 /* solution::Check_Solution (solution.cpp:169-203) as a device reduction of the

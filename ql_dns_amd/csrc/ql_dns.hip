@@ -433,6 +433,7 @@ qlk::MFArgs mf_args(const Batch &b, const Group &g)
     m.clock = g.clock;
     m.pm = g.pm;
     m.vdt = b.vdt;
+    m.rec = b.vdt;
     m.model = b.model;
     m.nMF = b.nMF;
     m.fac = 1.0 / g.NZ;
@@ -488,10 +489,11 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
     return QL_OK;
 }
 
-int refresh_profiles(Batch &b, Group &g, hipStream_t st)
+int refresh_profiles(Batch &b, Group &g, hipStream_t st, bool rec)
 {
     qlk::MFArgs m = mf_args(b, g);
     m.mode = qlk::MODE_PROFILES;
+    if (rec) m.rec = 1;
     m.mfn_out = g.mfN1;
     launch_mf(g, m, st);
     QL_HIP(hipGetLastError());
@@ -1285,9 +1287,17 @@ int ql_batch_calculate_cfl(ql_batch *bt, int run, double *cfl_over_dt)
     Batch &b = bt->b;
     const RunInfo &r = b.runs[run];
     Group &g = b.groups[r.group];
+    // with fixed steps nothing is recorded while stepping (the reference calls
+    // Calculate_CFL for variable steps only): evaluate the current state's
+    // profiles and the last rhs's drives now
+    if (!b.vdt) QL_RC(refresh_profiles(b, g, b.s, true));
     double pm[qlk::PM_STRIDE];
     QL_HIP(hipMemcpyAsync(pm, g.pm + (size_t)r.gi * qlk::PM_STRIDE, sizeof(pm), hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    if (!b.vdt) {
+        std::copy(pm + qlk::PM_CUR, pm + qlk::PM_CUR + 8, pm + qlk::PM_RHS);
+        pm[qlk::PM_E_RHS] = pm[qlk::PM_E_CUR];
+    }
     *cfl_over_dt = qlk::cfl_value(pm, g.h_rp[r.gi], b.model);
     return QL_OK;
 }

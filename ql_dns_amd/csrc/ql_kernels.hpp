@@ -310,6 +310,7 @@ struct MFArgs {
     double *pm;    // profile maxima [runs][PM_STRIDE]
     int model, nMF, mode, advance;
     int s, vdt;    // substep; variable steps (coefficients from the clock)
+    int rec;       // record the CFL quantities in pm (variable steps, or on request)
     double fac, linC, nlc1, nlc2;
 };
 
@@ -334,9 +335,26 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     const double linC = cf.linC, nlc1 = cf.nlc1, nlc2 = cf.nlc2;
     // the right-hand side of this substep read the profiles of the current
     // state (Calculate_CFL reads those, MHD_FullQlin.cpp:772)
-    if (a.mode != MODE_PROFILES && tid < 8) pm[PM_RHS + tid] = pm[PM_CUR + tid];
-    if (a.mode != MODE_PROFILES && tid == 8) pm[PM_E_RHS] = pm[PM_E_CUR];
-    const bool hd = a.runs[run].hd != 0;
+    const bool rec = a.rec != 0;
+    if (rec && a.mode != MODE_PROFILES && tid < 8) pm[PM_RHS + tid] = pm[PM_CUR + tid];
+    if (rec && a.mode != MODE_PROFILES && tid == 8) pm[PM_E_RHS] = pm[PM_E_CUR];
+    const bool hd = rec && a.runs[run].hd != 0;
+    // mean |kz^2 Ux_drive|, mean |kz^2 Uy_drive| of a drive (HD_fullU.cpp:619)
+    auto drive_means = [&](const double2 *d) {
+        double sx = 0.0, sy = 0.0;
+        for (int m = tid; m < NZD; m += NT) {
+            const double k2 = -(skz[m] * skz[m]);
+            const double2 dx = d[2 * NZD + m], dy = d[3 * NZD + m];
+            sx += hypot(dx.x * k2, dx.y * k2);
+            sy += hypot(dy.x * k2, dy.y * k2);
+        }
+        sx = block_sum(sx, red);
+        sy = block_sum(sy, red);
+        if (tid == 0) {
+            pm[PM_DUX] = sx / NZD;
+            pm[PM_DUY] = sy / NZD;
+        }
+    };
     // the substep's mode kernel, the only reader of the clock, has finished;
     // dt is read above, before this block can change it (the next step's dt is
     // written at the end)
@@ -381,22 +399,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
             a.drive[mo + k] = v;
         }
         __syncthreads();
-        if (hd) {
-            // mean |kz^2 Ux_drive|, mean |kz^2 Uy_drive| (HD_fullU.cpp:619)
-            double sx = 0.0, sy = 0.0;
-            for (int m = tid; m < NZD; m += NT) {
-                const double k2 = -(skz[m] * skz[m]);
-                const double2 dx = sdrive[2 * NZD + m], dy = sdrive[3 * NZD + m];
-                sx += hypot(dx.x * k2, dx.y * k2);
-                sy += hypot(dy.x * k2, dy.y * k2);
-            }
-            sx = block_sum(sx, red);
-            sy = block_sum(sy, red);
-            if (tid == 0) {
-                pm[PM_DUX] = sx / NZD;
-                pm[PM_DUY] = sy / NZD;
-            }
-        }
+        if (hd) drive_means(sdrive);
         // mean-field RHS (MHD_FullQlin.cpp:411-423, MHD_BQlin.cpp:274-280) and
         // update (RK3CN_Elin.cpp:109-112 / RK3CN.cpp:112-115)
 #pragma unroll
@@ -444,6 +447,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     } else {
         for (int k = tid; k < nslot; k += NT) smfn[k] = a.mf[mo + k];
         __syncthreads();
+        if (hd) drive_means(a.drive + mo);  // the last rhs's drive
     }
     // real-space profiles for the next rhs (MHD_FullQlin.cpp:180-198,
     // MHD_BQlin.cpp:122-127): fac1D * ifft(pad(...)), real part; for
@@ -473,7 +477,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     for (int idx = tid; idx < nlines * NZ; idx += NT) a.mfz[(size_t)run * 8 * NZ + idx] = buf[idx].x;
     // max |profile| per line, sqrt(max |z|^2) as Eigen's abs2().maxCoeff()
     // (MHD_FullQlin.cpp:773-776): 32 threads per line, fixed order
-    {
+    if (rec) {
         static_assert(NT == 8 * 32, "one 32-lane group per profile line");
         const int l = tid / 32, j0 = tid % 32;
         double m2 = 0.0;

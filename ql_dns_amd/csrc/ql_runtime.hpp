@@ -135,7 +135,7 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
 void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st);
 int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool remap, hipStream_t st);
 qlk::MFArgs mf_args(const Batch &b, const Group &g);
-int refresh_profiles(Batch &b, Group &g, hipStream_t st);
+int refresh_profiles(Batch &b, Group &g, hipStream_t st, bool rec = false);
 int run_steps(Batch &b, int nsteps, double *t_inout);
 // defined in ql_adjoint.hip
 void adj_release(Batch &b);

@@ -185,3 +185,17 @@ def test_check_solution_verdicts_match_oracle():
             with pytest.raises(QLError) as e:
                 g.check_solution(strict=True)
             assert e.value.code == -4 and "ERROR" in str(e.value)
+
+
+@pytest.mark.parametrize("model", [0, 1, 2])
+def test_fixed_step_cfl_is_the_current_state_value(model):
+    """With fixed steps nothing is recorded while stepping; ql_calculate_cfl
+    evaluates the current state, i.e. the oracle's value right after an rhs."""
+    p = _params(model=model)
+    o, g = _pair(p)
+    to = o.steps(7, 0.0)
+    g.steps(7, 0.0)
+    o.rhs(to, 0.0)
+    g.rhs(to, 0.0)
+    vo, vg = o.calculate_cfl(), g.calculate_cfl()
+    assert abs(vg - vo) <= 1e-12 * vo
