@@ -357,12 +357,13 @@ template <int NZ, int MB, int MODE>
 void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
     const bool full = (b.model == QL_MODEL_MHD_FULLQLIN);
+    constexpr int T = qlk::mode_threads(NZ, MB);
     if (b.precision == QL_PREC_MIXED) {
-        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE><<<g.W, qlk::NT, 0, st>>>(a);
-        else qlk::k_mode32<NZ, MB, qlk::ModelBQ, MODE><<<g.W, qlk::NT, 0, st>>>(a);
+        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+        else qlk::k_mode32<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
     } else {
-        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE><<<g.W, qlk::NT, 0, st>>>(a);
-        else qlk::k_mode64<NZ, MB, qlk::ModelBQ, MODE><<<g.W, qlk::NT, 0, st>>>(a);
+        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+        else qlk::k_mode64<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
     }
 }
 

@@ -261,17 +261,17 @@ struct Plan2 {
     static constexpr int R2 = NZ / R1;
 };
 
-template <int NZ, int NL, bool INV, int R, int NS, bool SYNC_MID, class C, class Ld, class St>
+template <int NZ, int NL, bool INV, int R, int NS, bool SYNC_MID, int TPB = NT, class C, class Ld, class St>
 __device__ __forceinline__ void fft_pass_io(const C *tw, const Ld &ld, const St &st)
 {
     constexpr int M = NZ / R;
     constexpr int TOT = NL * M;
-    constexpr int IT = (TOT + NT - 1) / NT;
+    constexpr int IT = (TOT + TPB - 1) / TPB;
     C v[IT][R];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int item = threadIdx.x + it * NT;
-        if (TOT % NT == 0 || item < TOT) {
+        const int item = threadIdx.x + it * TPB;
+        if (TOT % TPB == 0 || item < TOT) {
             const int line = item / M, j = item % M;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[it][r] = ld(line, j + r * M);
@@ -280,8 +280,8 @@ __device__ __forceinline__ void fft_pass_io(const C *tw, const Ld &ld, const St 
     if constexpr (SYNC_MID) __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int item = threadIdx.x + it * NT;
-        if (TOT % NT == 0 || item < TOT) {
+        const int item = threadIdx.x + it * TPB;
+        if (TOT % TPB == 0 || item < TOT) {
             const int line = item / M, j = item % M;
             const int k = j % NS;
             if constexpr (NS > 1) {
@@ -345,30 +345,31 @@ struct Plan2Size {
 };
 
 // Inverse (exp(+i)) transform of NL lines whose first pass reads src(line, g),
-// result in buf (lines of LS slots).  The caller must have synchronised before
-// buf may be overwritten and before src may be read.  Three barriers.
-template <int NZ, int NL, int LS, class C, class Src>
+// result in buf (lines of LS slots), by a workgroup of TPB threads.  The
+// caller must have synchronised before buf may be overwritten and before src
+// may be read.  Three barriers.
+template <int NZ, int NL, int LS, int TPB = NT, class C, class Src>
 __device__ __forceinline__ void fft2_inv_from(C *buf, const C *tw, const Src &src)
 {
     using PL = Plan2<NZ>;
     const BufIO<LS, C> io{buf};
     const PadIO<NZ, C> pad{buf};
-    fft_pass_io<NZ, NL, true, PL::R1, 1, false>(tw, src, pad);
-    fft_pass_io<NZ, NL, true, PL::R2, PL::R1, true>(tw, pad, io);
+    fft_pass_io<NZ, NL, true, PL::R1, 1, false, TPB>(tw, src, pad);
+    fft_pass_io<NZ, NL, true, PL::R2, PL::R1, true, TPB>(tw, pad, io);
 }
 
 // Forward (exp(-i)) transform of NL lines held in buf (lines of LS slots); the
 // last pass writes through dst(line, index, value), which must not alias the
 // first Plan2Size<NZ, NL, LS>::SLOTS slots of buf.  The caller must have
 // synchronised after filling buf.  Three barriers.
-template <int NZ, int NL, int LS, class C, class Dst>
+template <int NZ, int NL, int LS, int TPB = NT, class C, class Dst>
 __device__ __forceinline__ void fft2_fwd_to(C *buf, const C *tw, const Dst &dst)
 {
     using PL = Plan2<NZ>;
     const BufIO<LS, C> io{buf};
     const PadIO<NZ, C> pad{buf};
-    fft_pass_io<NZ, NL, false, PL::R1, 1, true>(tw, io, pad);
-    fft_pass_io<NZ, NL, false, PL::R2, PL::R1, false>(tw, pad, dst);
+    fft_pass_io<NZ, NL, false, PL::R1, 1, true, TPB>(tw, io, pad);
+    fft_pass_io<NZ, NL, false, PL::R2, PL::R1, false, TPB>(tw, pad, dst);
 }
 
 // dealiased sizes fixed by NZ (Models/Model.h:34: nz = 1 + 2*int(NZ/3))

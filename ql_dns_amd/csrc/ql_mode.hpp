@@ -313,6 +313,9 @@ struct CompactDst {
     }
 };
 
+// threads of a mode-kernel workgroup: one per z point of a pass
+constexpr int mode_threads(int NZ, int MB) { return NZ * MB < NT ? NZ * MB : NT; }
+
 template <int NZ, int MB, class P, class C, int MODE>
 __device__ __forceinline__ void mode_body(const ModeArgs &a)
 {
@@ -322,7 +325,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     constexpr int NZD = D::nz;
     constexpr int NIT = MB * NZD;  // (mode slot, kz slot) items: bases, RHS, update
     constexpr int NZI = MB * NZ;   // (mode slot, z) items: products
-    static_assert(NIT <= NT && NZI <= NT, "one item per thread");
+    constexpr int TPB = mode_threads(NZ, MB);
+    static_assert(NIT <= TPB && NZI <= TPB, "one item per thread");
     __shared__ C buf[L::BUF];
     __shared__ C sbase[MB * P::NB * L::NZDP];
     __shared__ C tw[NZ];
@@ -350,8 +354,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     const bool do_remap = a.do_remap && rp.remap;
 
     const C *twg = reinterpret_cast<const C *>(sizeof(C) == 8 ? a.tw32 : (const void *)a.tw);
-    for (int k = tid; k < NZ; k += NT) tw[k] = twg[k];
-    for (int k = tid; k < NZD; k += NT) skzr[k] = (R)a.kzv[k];
+    for (int k = tid; k < NZ; k += TPB) tw[k] = twg[k];
+    for (int k = tid; k < NZD; k += TPB) skzr[k] = (R)a.kzv[k];
 
     // the thread's items: (imb, im) for the bases and the update, (zmb, zz)
     // for the products; both fixed for the whole kernel
@@ -366,7 +370,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 #pragma unroll
             for (int c = 0; c < P::NMFZ; ++c) mfr[c] = has_z ? (R)mfz[c * NZ + zz] : (R)0;
         } else {
-            for (int k = tid; k < NZ * P::NMFZ; k += NT) smfz[k] = (R)mfz[k];
+            for (int k = tid; k < NZ * P::NMFZ; k += TPB) smfz[k] = (R)mfz[k];
         }
     }
     double acc[P::NACC];
@@ -400,7 +404,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         // Kdata.cpp:100-120) and the base spectral fields
         const int l = local_mode(g);
         if (!AHEAD) fetch(g);
-        const double2 U0 = upre[0], U1 = upre[1], U2 = upre[2], U3 = upre[3];
+        const double2 B0 = upre[0], B1 = upre[1], B2 = upre[2], B3 = upre[3];
         const double kx0 = kpre[0], ky = kpre[1];
         if (AHEAD) fetch(g + 1);
         const size_t off = (size_t)(mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
@@ -411,7 +415,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             const double lap2 = zero ? 0.0 : (-ky * ky + kz2);
             const double ilap2 = zero ? 1.0 : 1.0 / lap2;
             if (l >= 0) ilapF = (l == 0 && im == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
-            P::bases(U0, U1, U2, U3, kx, ky, kz, ilap2, a.fac, sbase + imb * L::NZDP + im, MB * L::NZDP);
+            P::bases(B0, B1, B2, B3, kx, ky, kz, ilap2, a.fac, sbase + imb * L::NZDP + im, MB * L::NZDP);
             if (im == 0) {
                 skx[imb] = kx;
                 sky[imb] = ky;
@@ -421,7 +425,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         __syncthreads();
 
         // C: inverse transforms of the padded lines
-        fft2_inv_from<NZ, L::NL, L::LS>(buf, tw, LineSrcI<NZ, MB, P, C>{sbase, skzr});
+        fft2_inv_from<NZ, L::NL, L::LS, TPB>(buf, tw, LineSrcI<NZ, MB, P, C>{sbase, skzr});
 
         // D: products on the z grid, in place, and the EMF / stress sums
         if (has_z) {
@@ -460,11 +464,21 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 
         // E: forward transforms of the product lines, dealiased band kept
         C *ct = buf + L::CT_OFF;  // [line][nz]
-        fft2_fwd_to<NZ, L::NLO, L::LS>(buf, tw, CompactDst<NZ, C>{ct});
+        fft2_fwd_to<NZ, L::NLO, L::LS, TPB>(buf, tw, CompactDst<NZ, C>{ct});
 
         // F: RHS, dissipation operator at the substep end, update
         if (l >= 0) {
             const C *T = ct + imb * NZD + im;
+            // without prefetch (FP64, NZ = 256) the item's state is read again
+            // here instead of being kept in registers through both transforms,
+            // which removes that kernel's spills
+            double2 U0 = B0, U1 = B1, U2 = B2, U3 = B3;
+            if constexpr (!AHEAD) {
+                U0 = a.u[off];
+                U1 = a.u[off + NZD];
+                U2 = a.u[off + 2 * NZD];
+                U3 = a.u[off + 3 * NZD];
+            }
             bool hit = false;
             if (do_remap) hit = remap_hit(remap_qt, rp.remap_thr, kx0, ky);
 #pragma unroll
@@ -538,7 +552,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 #pragma unroll
         for (int c = 0; c < P::NACC; ++c) red[(zmb * P::NACC + c) * NZ + zz] = acc[c];
     __syncthreads();
-    for (int k = tid; k < P::NACC * NZ; k += NT) {
+    for (int k = tid; k < P::NACC * NZ; k += TPB) {
         double s = 0.0;
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) s += red[mb * P::NACC * NZ + k];
@@ -547,18 +561,18 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 }
 
 // FP64 path and mixed path (FP32 transforms); each with its own occupancy
-// request, by pass size
 // request, by pass size; the substep mode (MODE_RHS, MODE_ELIN, MODE_CN) is a
-// template parameter, so each instantiation carries only its own update
+// template parameter, so each instantiation carries only its own update.  A
+// workgroup has one thread per z point of a pass (MB NZ, at most NT).
 constexpr bool half_pass(int NZ, int MB) { return NZ * MB < 256; }
 template <int NZ, int MB, class P, int MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES64_HALF : QL_WAVES64)))
+__global__ __launch_bounds__(mode_threads(NZ, MB)) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES64_HALF : QL_WAVES64)))
 void k_mode64(ModeArgs a)
 {
     mode_body<NZ, MB, P, double2, MODE>(a);
 }
 template <int NZ, int MB, class P, int MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES32_HALF : QL_WAVES32)))
+__global__ __launch_bounds__(mode_threads(NZ, MB)) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES32_HALF : QL_WAVES32)))
 void k_mode32(ModeArgs a)
 {
     mode_body<NZ, MB, P, float2, MODE>(a);
