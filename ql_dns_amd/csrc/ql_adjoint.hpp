@@ -80,7 +80,7 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
     using P = ModelFull;
     using D = Dealias<NZ>;
     constexpr int NZD = D::nz;
-    constexpr int NLX = P::NLINE * MB;  // 10 lines per mode
+    constexpr int NLX = P::NLINE * MB;  // 6 lines per mode
     constexpr int NLT = P::NOUT * MB;   // 6 lines per mode
     constexpr int HI = MB * 2 * NZ;     // product items (mode, half, z)
     constexpr int HW = (HI + NT - 1) / NT;
@@ -296,8 +296,8 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
         fft_lines_from<NZ, NLT, true>(bufT, tw, CompactSrc<NZ>{sbase});
 
         // H: products and EMF / stress terms in reverse, per (mode, half, z):
-        // half 0 owns the u lines (ux uy uz dz ux dz uy), half 1 the b lines
-        double2 gX[HW][5];
+        // half 0 owns the u lines (ux uy uz), half 1 the b lines (bx by bz)
+        double2 gX[HW][3];
 #pragma unroll
         for (int p = 0; p < HW; ++p) {
             const int item = tid + p * NT;
@@ -305,11 +305,10 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
             const int mb = item / (2 * NZ), rem = item % (2 * NZ);
             const int h = rem / NZ, z = rem % NZ;
 #pragma unroll
-            for (int c = 0; c < 5; ++c) gX[p][c] = make_double2(0.0, 0.0);
+            for (int c = 0; c < 3; ++c) gX[p][c] = make_double2(0.0, 0.0);
             if (sidx[mb] < 0) continue;
             const double2 *X = bufX + mb * P::NLINE * NZ + z;
             const double2 u0 = X[0], u1 = X[NZ], u2 = X[2 * NZ], b0 = X[3 * NZ], b1 = X[4 * NZ], b2 = X[5 * NZ];
-            const double2 dzux = X[6 * NZ], dzbx = X[7 * NZ], dzuy = X[8 * NZ], dzby = X[9 * NZ];
             const double2 *GPp = bufT + mb * P::NOUT * NZ + z;
             const double2 G0 = GPp[0], G1 = GPp[NZ], G2 = GPp[2 * NZ], G3 = GPp[3 * NZ], G4 = GPp[4 * NZ],
                           G5 = GPp[5 * NZ];
@@ -323,25 +322,22 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
                          g3 = ftfac * sgs[3 * NZ + z];
             double gkx = 0.0;
             if (h == 0) {
-                // u0 (0), u1 (1), u2 (2), dz ux (3), dz uy (4)
+                // u0 (0), u1 (1), u2 (2)
                 double2 Gu0 = cadd(imul(-aU, G0), imul(aB, G3));
                 double2 Gu1 = cadd(imul(-aU, G1), imul(aB, G4));
                 double2 Gu2 = cadd(cadd(cscale(dzMUx, G0), cscale(dzMUy, G1)), imul(-aU, G2));
                 Gu2 = cadd(Gu2, cadd(cadd(cscale(dzMBx, G3), cscale(dzMBy, G4)), imul(aB, G5)));
-                // EMF x, y (slots 0, 1) and stresses (slots 2, 3)
+                // EMF x, y (slots 0, 1) and stresses (slots 2, 3:
+                // -Re(uz conj(ux)), -Re(uz conj(uy)) before their z-derivative)
                 Gu0 = cadd(Gu0, cscale(g0, b2));
                 Gu2 = csub(Gu2, cadd(cscale(g0, b0), cscale(g1, b1)));
                 Gu1 = cadd(Gu1, cscale(g1, b2));
-                Gu1 = cadd(Gu1, imul(-g2 * ky, u0));
-                Gu0 = cadd(Gu0, imul(g2 * ky, u1));
-                Gu2 = csub(Gu2, cadd(cscale(g2, dzux), cscale(g3, dzuy)));
-                Gu0 = cadd(Gu0, imul(-g3 * kx, u1));
-                Gu1 = cadd(Gu1, imul(g3 * kx, u0));
+                Gu0 = csub(Gu0, cscale(g2, u2));
+                Gu1 = csub(Gu1, cscale(g3, u2));
+                Gu2 = csub(Gu2, cadd(cscale(g2, u0), cscale(g3, u1)));
                 gX[p][0] = Gu0;
                 gX[p][1] = Gu1;
                 gX[p][2] = Gu2;
-                gX[p][3] = cscale(-g2, u2);
-                gX[p][4] = cscale(-g3, u2);
                 // mean-field profile and kx gradients from the first three products
                 const double gaU = redot(imul(1.0, u0), G0) + redot(imul(1.0, u1), G1) + redot(imul(1.0, u2), G2);
                 const double gaB = redot(imul(-1.0, b0), G0) + redot(imul(-1.0, b1), G1) + redot(imul(-1.0, b2), G2);
@@ -355,7 +351,7 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
                 accm[p][2] += ky * gaB;
                 gkx += MUx * gaU + MBx * gaB;
             } else {
-                // b0 (0), b1 (1), b2 (2), dz bx (3), dz by (4)
+                // b0 (0), b1 (1), b2 (2)
                 double2 Gb0 = cadd(imul(aB, G0), imul(-aU, G3));
                 double2 Gb1 = cadd(imul(aB, G1), imul(-aU, G4));
                 double2 Gb2 = cadd(cadd(cscale(-dzMBx, G0), cscale(-dzMBy, G1)), imul(aB, G2));
@@ -363,16 +359,13 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
                 Gb2 = cadd(Gb2, cadd(cscale(g0, u0), cscale(g1, u1)));
                 Gb0 = csub(Gb0, cscale(g0, u2));
                 Gb1 = csub(Gb1, cscale(g1, u2));
-                Gb1 = cadd(Gb1, imul(g2 * ky, b0));
-                Gb0 = cadd(Gb0, imul(-g2 * ky, b1));
-                Gb2 = cadd(Gb2, cadd(cscale(g2, dzbx), cscale(g3, dzby)));
-                Gb0 = cadd(Gb0, imul(g3 * kx, b1));
-                Gb1 = cadd(Gb1, imul(-g3 * kx, b0));
+                // stresses: +Re(bz conj(bx)), +Re(bz conj(by))
+                Gb0 = cadd(Gb0, cscale(g2, b2));
+                Gb1 = cadd(Gb1, cscale(g3, b2));
+                Gb2 = cadd(Gb2, cadd(cscale(g2, b0), cscale(g3, b1)));
                 gX[p][0] = Gb0;
                 gX[p][1] = Gb1;
                 gX[p][2] = Gb2;
-                gX[p][3] = cscale(g2, b2);
-                gX[p][4] = cscale(g3, b2);
                 const double gaU = redot(imul(1.0, b0), G3) + redot(imul(1.0, b1), G4) + redot(imul(1.0, b2), G5);
                 const double gaB = redot(imul(-1.0, u0), G3) + redot(imul(-1.0, u1), G4) + redot(imul(-1.0, u2), G5);
                 accm[p][1] += redot(u2, G3);   // dz MBx
@@ -384,8 +377,6 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
                 accm[p][0] += kx * gaB;
                 accm[p][2] += ky * gaB;
                 gkx += MUx * gaU + MBx * gaB;
-                // d(stress y)/dkx: -Re(u0 conj(i u1)) + Re(b0 conj(i b1))
-                gkx += g3 * (-redot(u0, imul(1.0, u1)) + redot(b0, imul(1.0, b1)));
             }
             gq += gkx * (a.ts * ky);
         }
@@ -397,13 +388,11 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
             const int mb = item / (2 * NZ), rem = item % (2 * NZ);
             const int h = rem / NZ, z = rem % NZ;
             double2 *Y = bufX + mb * P::NLINE * NZ + z;
-            // lines: u0 u1 u2 b0 b1 b2 dzux dzbx dzuy dzby
-            const int l0 = h ? 3 : 0, l3 = h ? 7 : 6, l4 = h ? 9 : 8;
+            // lines: u0 u1 u2 b0 b1 b2
+            const int l0 = h ? 3 : 0;
             Y[l0 * NZ] = gX[p][0];
             Y[(l0 + 1) * NZ] = gX[p][1];
             Y[(l0 + 2) * NZ] = gX[p][2];
-            Y[l3 * NZ] = gX[p][3];
-            Y[l4 * NZ] = gX[p][4];
         }
         __syncthreads();
 
@@ -419,12 +408,8 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
             const bool zero = (ky == 0.0 && m == 0);
             const double ilap2 = zero ? 1.0 : 1.0 / (-ky * ky + -(kz * kz));
             const double2 *GL = bufX + mb * P::NLINE * NZ + D::pad(m);
-            const double2 GB0 = cadd(GL[0], imul(-kz, GL[6 * NZ]));
-            const double2 GB1 = cadd(GL[NZ], imul(-kz, GL[8 * NZ]));
-            const double2 GB2 = GL[2 * NZ];
-            const double2 GB3 = cadd(GL[3 * NZ], imul(-kz, GL[7 * NZ]));
-            const double2 GB4 = cadd(GL[4 * NZ], imul(-kz, GL[9 * NZ]));
-            const double2 GB5 = GL[5 * NZ];
+            const double2 GB0 = GL[0], GB1 = GL[NZ], GB2 = GL[2 * NZ];
+            const double2 GB3 = GL[3 * NZ], GB4 = GL[4 * NZ], GB5 = GL[5 * NZ];
             const double2 *U = uin + mb * 4 * NZD + m;
             const double2 ux = U[0], bx = U[2 * NZD];
             const double f = a.fac;
@@ -561,7 +546,8 @@ __global__ __launch_bounds__(NT) void k_adj_mf_a(AdjMFArgs a)
                    redot(imul(kz, M1), N3);
         }
     }
-    // drive in reverse: Hermitian fill and x i kz (MHD_FullQlin.cpp:386-399),
+    // drive in reverse: Hermitian fill and x i kz of every line (the EMF,
+    // MHD_FullQlin.cpp:326-336, and the stresses before their z-derivative),
     // then the forward transform's adjoint (an inverse transform) of the
     // real stress profiles
     for (int k = tid; k < 4 * NZ; k += NT) {
@@ -570,12 +556,12 @@ __global__ __launch_bounds__(NT) void k_adj_mf_a(AdjMFArgs a)
         if (n < HALF) {
             const double c = skz[n];
             const double2 d0 = sGN[f * NZD + n];
-            v = (f < 2) ? imul(-c, d0) : d0;
+            v = imul(-c, d0);
             const int mm = 2 * HALF - 1 - n;  // slot filled with conj of this one
             if (n >= 1 && mm < NZD) {
                 double2 d1 = sGN[f * NZD + mm];
                 d1.y = -d1.y;
-                v = cadd(v, (f < 2) ? imul(-c, d1) : d1);
+                v = cadd(v, imul(-c, d1));
             }
         }
         buf[k] = v;

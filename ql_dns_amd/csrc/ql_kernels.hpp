@@ -13,6 +13,9 @@
 //                 profiles, x i kz, Hermitian fill (the reference's MPI
 //                 Allreduce + fill, MHD_FullQlin.cpp:386-399), mean-field RHS
 //                 and update, and the real-space profiles the next substep reads.
+// (A single launch in which the last reducing workgroup of a run goes on with
+// the mean fields was measured slower: the device-scope fences and the atomic
+// count cost more than the launch boundary of a replayed graph.)
 // Step times live on the device (one Clock per group), advanced by the
 // group's last mean-field kernel of each step, so a whole step can be captured
 // once in a HIP graph and replayed.
@@ -233,22 +236,20 @@ __device__ __forceinline__ void clock_advance(Clock *c)
 
 // ---------------------------------------------------------------------------
 // stress / EMF reduction: column c of run r's partial rows summed with
-// Neumaier compensation.  A block (blockIdx.y = run) owns 16 columns; its 16
-// row chunks are summed in row order by separate threads and then added in
-// chunk order, so the result does not depend on scheduling.  No cross-block
-// communication (no fences, no atomics).
+// Neumaier compensation.  A block owns 16 columns of one run; its 16 row
+// chunks are summed in row order by separate threads and then added in chunk
+// order, so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
-[[maybe_unused]] static __global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, const RunParams *__restrict__ runs,
-                                                    int ncols, double *__restrict__ sums)
+__device__ __forceinline__ void reduce_cols_block(const double *__restrict__ part, const RunParams &rp, int ncols,
+                                                  double *__restrict__ sums, int cblk, double (*red)[17],
+                                                  double (*red2)[17])
 {
-    __shared__ double red[16][17], red2[16][17];
-    const RunParams &rp = runs[blockIdx.y];
     const int cl = threadIdx.x % 16, chunk = threadIdx.x / 16;
-    const int c = blockIdx.x * 16 + cl;
+    const int c = cblk * 16 + cl;
     const int nrows = rp.nwg;
     const int per = (nrows + 15) / 16;
     const int r0 = rp.wg_off + chunk * per, r1 = rp.wg_off + min(nrows, chunk * per + per);
-    constexpr int B = 16;
+    constexpr int B = 32;  // rows in flight per thread (one batch up to 512 rows)
     double s = 0.0, e = 0.0;
     if (c < ncols) {
         for (int r = r0; r < r1; r += B) {
@@ -275,8 +276,16 @@ __device__ __forceinline__ void clock_advance(Clock *c)
             te += red2[k][cl];
             t = u;
         }
-        sums[(size_t)blockIdx.y * ncols + c] = t + te;
+        sums[c] = t + te;
     }
+}
+
+// grid (column blocks, runs)
+[[maybe_unused]] static __global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, const RunParams *__restrict__ runs,
+                                                    int ncols, double *__restrict__ sums)
+{
+    __shared__ double red[16][17], red2[16][17];
+    reduce_cols_block(part, runs[blockIdx.y], ncols, sums + (size_t)blockIdx.y * ncols, blockIdx.x, red, red2);
 }
 
 // sum over the workgroup's threads in a fixed order (every thread gets it)
@@ -294,7 +303,7 @@ __device__ __forceinline__ double block_sum(double v, double *red)
 }
 
 // ---------------------------------------------------------------------------
-// mean fields, one workgroup per run (blockIdx.x)
+// mean fields of one run, by one workgroup
 // ---------------------------------------------------------------------------
 struct MFArgs {
     const double *sums;       // reduced stress / EMF z-profiles [runs][nMF][NZ]
@@ -315,20 +324,52 @@ struct MFArgs {
 };
 
 template <int NZ>
-__global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
+struct MFShared {
+    static constexpr int NZD = Dealias<NZ>::nz;
+    double2 pad[8 * PadIO<NZ, double2>::LP];  // between the two passes of a transform
+    double2 line[8 * NZ];                     // stress spectra, then the profile lines
+    double2 tw[NZ];
+    double skz[NZD];
+    double2 smf[4 * NZD];     // mean fields at the start of the substep
+    double2 sprev[4 * NZD];   // previous substep's mean-field RHS
+    double2 sdrive[4 * NZD];  // drives (HD_fullU records)
+    double2 smfn[4 * NZD];    // updated mean fields
+    double red[NT];           // block sums
+};
+
+// Inputs of the mean-field update other than the summed profiles (twiddles,
+// wavenumbers, mean fields, previous RHS), stored to LDS without a barrier:
+// meanfield_body reads them only after its first one.
+template <int NZ>
+__device__ __forceinline__ void meanfield_load(const MFArgs &a, int run, MFShared<NZ> &sh)
+{
+    constexpr int NZD = Dealias<NZ>::nz;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < NZ; k += NT) sh.tw[k] = a.tw[k];
+    for (int k = tid; k < NZD; k += NT) sh.skz[k] = a.kzv[k];
+    const int nslot = a.nMF * NZD;
+    const size_t mo = (size_t)run * nslot;
+    if (a.mode != MODE_PROFILES) {
+        for (int k = tid; k < nslot; k += NT) {
+            sh.smf[k] = a.mf[mo + k];
+            sh.sprev[k] = a.mfn_prev ? a.mfn_prev[mo + k] : make_double2(0.0, 0.0);
+        }
+    }
+}
+
+// Both transforms are two passes (radix 16 x 8 at NZ = 128) between separate
+// LDS regions: the forward one reads the summed profiles from global memory
+// in its first pass, the inverse one builds its padded input lines from the
+// updated spectra in its first pass and stores the real profiles in its
+// second, so a substep takes five barriers.  meanfield_load must have run.
+template <int NZ>
+__device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShared<NZ> &sh)
 {
     using D = Dealias<NZ>;
+    using PL = Plan2<NZ>;
     constexpr int NZD = D::nz, HALF = D::half;
     constexpr int MW = (4 * NZD + NT - 1) / NT;  // mean-field slots per thread
-    __shared__ double2 buf[8 * NZ];
-    __shared__ double2 tw[NZ];
-    __shared__ double skz[NZD];
-    __shared__ double2 smf[4 * NZD];    // mean fields at the start of the substep
-    __shared__ double2 sdrive[4 * NZD]; // drives
-    __shared__ double2 smfn[4 * NZD];   // updated mean fields
-    __shared__ double red[NT];          // block sums
     const int tid = threadIdx.x;
-    const int run = blockIdx.x;
     double *pm = a.pm + (size_t)run * PM_STRIDE;
     StepCoefs cf{a.linC, a.nlc1, a.nlc2};
     if (a.vdt && a.mode != MODE_PROFILES && a.s > 0) cf = step_coefs(a.clock->dt, a.s, a.mode == MODE_CN);
@@ -343,13 +384,13 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     auto drive_means = [&](const double2 *d) {
         double sx = 0.0, sy = 0.0;
         for (int m = tid; m < NZD; m += NT) {
-            const double k2 = -(skz[m] * skz[m]);
+            const double k2 = -(sh.skz[m] * sh.skz[m]);
             const double2 dx = d[2 * NZD + m], dy = d[3 * NZD + m];
             sx += hypot(dx.x * k2, dx.y * k2);
             sy += hypot(dy.x * k2, dy.y * k2);
         }
-        sx = block_sum(sx, red);
-        sy = block_sum(sy, red);
+        sx = block_sum(sx, sh.red);
+        sy = block_sum(sy, sh.red);
         if (tid == 0) {
             pm[PM_DUX] = sx / NZD;
             pm[PM_DUY] = sy / NZD;
@@ -362,58 +403,43 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     const Phys ph = a.runs[run].ph;
     const int nMF = a.nMF, nslot = nMF * NZD;
     const size_t mo = (size_t)run * nslot;
-    for (int k = tid; k < NZ; k += NT) tw[k] = a.tw[k];
-    for (int k = tid; k < NZD; k += NT) skz[k] = a.kzv[k];
+    const PadIO<NZ, double2> pad{sh.pad};
+    const BufIO<NZ, double2> lines{sh.line};
 
     if (a.mode != MODE_PROFILES) {
-        double2 pmf[MW], pprev[MW];
-#pragma unroll
-        for (int w = 0; w < MW; ++w) {
-            const int k = tid + w * NT;
-            pmf[w] = (k < nslot) ? a.mf[mo + k] : make_double2(0.0, 0.0);
-            pprev[w] = (k < nslot && a.mfn_prev) ? a.mfn_prev[mo + k] : make_double2(0.0, 0.0);
+        // forward transform of the summed z-profiles (real lines)
+        const double *S = a.sums + (size_t)run * nMF * NZ;
+        const auto ld = [S](int l, int j) { return make_double2(S[l * NZ + j], 0.0); };
+        if (nMF == 4) {
+            fft_pass_io<NZ, 4, false, PL::R1, 1, false>(sh.tw, ld, pad);
+            fft_pass_io<NZ, 4, false, PL::R2, PL::R1, false>(sh.tw, pad, lines);
+        } else {
+            fft_pass_io<NZ, 2, false, PL::R1, 1, false>(sh.tw, ld, pad);
+            fft_pass_io<NZ, 2, false, PL::R2, PL::R1, false>(sh.tw, pad, lines);
         }
-        for (int c = tid; c < nMF * NZ; c += NT) buf[c] = make_double2(a.sums[(size_t)run * nMF * NZ + c], 0.0);
-#pragma unroll
-        for (int w = 0; w < MW; ++w) {
-            const int k = tid + w * NT;
-            if (k < nslot) smf[k] = pmf[w];
-        }
-        __syncthreads();
-        if (nMF == 4) fft_lines<NZ, 4, false>(buf, tw);
-        else fft_lines<NZ, 2, false>(buf, tw);
-        // EMF lines get d/dz = i kz; keep the non-negative half and fill the
-        // negative half by conjugate symmetry (MHD_FullQlin.cpp:326-399)
-        for (int k = tid; k < nslot; k += NT) {
-            const int f = k / NZD, m = k % NZD;
-            int src = m;
-            bool cj = false;
-            if (m >= HALF) {
-                src = HALF - 1 - (m - HALF);
-                cj = true;
-            }
-            double2 v = buf[f * NZ + src];
-            if (f < 2) v = imul(skz[src], v);
-            if (cj) v.y = -v.y;
-            sdrive[k] = v;
-            a.drive[mo + k] = v;
-        }
-        __syncthreads();
-        if (hd) drive_means(sdrive);
-        // mean-field RHS (MHD_FullQlin.cpp:411-423, MHD_BQlin.cpp:274-280) and
-        // update (RK3CN_Elin.cpp:109-112 / RK3CN.cpp:112-115)
 #pragma unroll
         for (int w = 0; w < MW; ++w) {
             const int k = tid + w * NT;
             if (k >= nslot) continue;
             const int f = k / NZD, m = k % NZD;
-            const double kz = skz[m], kz2 = -(kz * kz);
+            // d/dz = i kz of every line (the EMF, MHD_FullQlin.cpp:326-336, and
+            // the stresses, accumulated before their z-derivative by the mode
+            // kernel); the non-negative half is kept and the negative half
+            // filled by conjugate symmetry (:386-399)
+            const bool cj = m >= HALF;
+            const int src = cj ? NZD - m : m;
+            const double2 Dv0 = imul(sh.skz[src], sh.line[f * NZ + src]);
+            const double2 Dv = cj ? make_double2(Dv0.x, -Dv0.y) : Dv0;
+            a.drive[mo + k] = Dv;
+            if (hd) sh.sdrive[k] = Dv;
+            // mean-field RHS (MHD_FullQlin.cpp:411-423, MHD_BQlin.cpp:274-280)
+            // and update (RK3CN_Elin.cpp:109-112 / RK3CN.cpp:112-115)
+            const double kz = sh.skz[m], kz2 = -(kz * kz);
             double2 N = make_double2(0.0, 0.0);
             double Lmf = 0.0;
             if (ph.quasilinear) {
-                const double2 Dv = sdrive[k];
                 if (a.model == 0) {
-                    const double2 M0 = smf[m], M1 = smf[NZD + m], M2 = smf[2 * NZD + m], M3 = smf[3 * NZD + m];
+                    const double2 M0 = sh.smf[m], M1 = sh.smf[NZD + m], M2 = sh.smf[2 * NZD + m], M3 = sh.smf[3 * NZD + m];
                     if (f == 0) N = cadd(imul(ph.B0z * kz, M2), Dv);
                     else if (f == 1) N = cadd(cadd(cscale(-ph.q, M0), imul(ph.B0z * kz, M3)), Dv);
                     else if (f == 2) N = cadd(cadd(cscale(2 * ph.omega, M3), imul(ph.B0z * kz, M0)), Dv);
@@ -421,31 +447,32 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
                     Lmf = (f < 2 ? ph.eta : ph.nu) * kz2;
                 } else {
                     if (f == 0) N = Dv;
-                    else N = cadd(cscale(-ph.q, smf[m]), Dv);
+                    else N = cadd(cscale(-ph.q, sh.smf[m]), Dv);
                     Lmf = ph.eta * kz2;
                 }
             }
             a.mfn_out[mo + k] = N;
             if (a.mode == MODE_RHS) continue;
-            const double2 M = smf[k];
+            const double2 M = sh.smf[k];
             double2 Mn;
             if (a.mode == MODE_ELIN) {
                 double2 sacc = cscale(nlc1, N);
-                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2, pprev[w]));
+                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2, sh.sprev[k]));
                 Mn = cscale(exp(linC * Lmf), cadd(sacc, M));
             } else {
                 const double denom = 1.0 / (1.0 - linC * Lmf);
                 double2 sacc = cscale(nlc1 * denom, N);
-                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2 * denom, pprev[w]));
+                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2 * denom, sh.sprev[k]));
                 Mn = cadd(sacc, cscale((1 + linC * Lmf) * denom, M));
             }
             a.mf[mo + k] = Mn;
-            smfn[k] = Mn;
+            sh.smfn[k] = Mn;
         }
         __syncthreads();
+        if (hd) drive_means(sh.sdrive);
         if (a.mode == MODE_RHS) return;
     } else {
-        for (int k = tid; k < nslot; k += NT) smfn[k] = a.mf[mo + k];
+        for (int k = tid; k < nslot; k += NT) sh.smfn[k] = a.mf[mo + k];
         __syncthreads();
         if (hd) drive_means(a.drive + mo);  // the last rhs's drive
     }
@@ -454,27 +481,32 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     // MHD_BQlin line 3 is Bx, which only the CFL condition reads
     // (MHD_BQlin.cpp:546-548)
     const int nlines = (a.model == 0) ? 8 : 3;
-    for (int idx = tid; idx < 8 * NZ; idx += NT) {
-        const int l = idx / NZ, j = idx % NZ;
+    const double fac = a.fac;
+    const int model = a.model;
+    const auto padded = [&](int l, int j) {
         double2 v = make_double2(0.0, 0.0);
         if (D::kept(j)) {
             const int m = D::slot(j);
-            const double kz = skz[m];
-            if (a.model == 0) {
-                const double2 M = smfn[(l >> 1) * NZD + m];
-                v = (l & 1) ? imul(a.fac * kz, M) : cscale(a.fac, M);
+            const double kz = sh.skz[m];
+            if (model == 0) {
+                const double2 M = sh.smfn[(l >> 1) * NZD + m];
+                v = (l & 1) ? imul(fac * kz, M) : cscale(fac, M);
             } else if (l < 4) {
-                const double2 M = smfn[(l == 3 ? 0 : NZD) + m];  // By, or Bx for line 3
-                if (l == 0 || l == 3) v = cscale(a.fac, M);
-                else if (l == 1) v = imul(a.fac * kz, M);
-                else v = cscale(a.fac * (-(kz * kz)), M);
+                const double2 M = sh.smfn[(l == 3 ? 0 : NZD) + m];  // By, or Bx for line 3
+                if (l == 0 || l == 3) v = cscale(fac, M);
+                else if (l == 1) v = imul(fac * kz, M);
+                else v = cscale(fac * (-(kz * kz)), M);
             }
         }
-        buf[idx] = v;
-    }
-    __syncthreads();
-    fft_lines<NZ, 8, true>(buf, tw);
-    for (int idx = tid; idx < nlines * NZ; idx += NT) a.mfz[(size_t)run * 8 * NZ + idx] = buf[idx].x;
+        return v;
+    };
+    double *Z = a.mfz + (size_t)run * 8 * NZ;
+    const auto out = [&](int l, int i, double2 v) {
+        sh.line[l * NZ + i] = v;
+        if (l < nlines) Z[l * NZ + i] = v.x;
+    };
+    fft_pass_io<NZ, 8, true, PL::R1, 1, false>(sh.tw, padded, pad);
+    fft_pass_io<NZ, 8, true, PL::R2, PL::R1, false>(sh.tw, pad, out);
     // max |profile| per line, sqrt(max |z|^2) as Eigen's abs2().maxCoeff()
     // (MHD_FullQlin.cpp:773-776): 32 threads per line, fixed order
     if (rec) {
@@ -482,7 +514,7 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
         const int l = tid / 32, j0 = tid % 32;
         double m2 = 0.0;
         for (int j = j0; j < NZ; j += 32) {
-            const double2 v = buf[l * NZ + j];
+            const double2 v = sh.line[l * NZ + j];
             m2 = fmax(m2, rn_add(rn_mul(v.x, v.x), rn_mul(v.y, v.y)));
         }
 #pragma unroll
@@ -494,18 +526,18 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
         // of the current state (:619)
         double ex = 0.0, ey = 0.0, su = 0.0;
         for (int j = tid; j < NZ; j += NT) {
-            const double2 x = buf[4 * NZ + j], y = buf[6 * NZ + j];
+            const double2 x = sh.line[4 * NZ + j], y = sh.line[6 * NZ + j];
             ex += x.x * x.x + x.y * x.y;
             ey += y.x * y.x + y.y * y.y;
         }
         for (int m = tid; m < NZD; m += NT) {
-            const double k2 = -(skz[m] * skz[m]);
-            const double2 u = smfn[2 * NZD + m];
+            const double k2 = -(sh.skz[m] * sh.skz[m]);
+            const double2 u = sh.smfn[2 * NZD + m];
             su += hypot(u.x * k2, u.y * k2);
         }
-        ex = block_sum(ex, red);
-        ey = block_sum(ey, red);
-        su = block_sum(su, red);
+        ex = block_sum(ex, sh.red);
+        ey = block_sum(ey, sh.red);
+        su = block_sum(su, sh.red);
         if (tid == 0) {
             pm[PM_E_CUR] = ex + ey;
             pm[PM_UX] = su / NZD;
@@ -516,6 +548,15 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
         // the next step's dt (RK3CN_Elin.cpp:76-80)
         if (tid == 0) a.clock->dt = cfl_dt(pm, a.runs[0], a.model);
     }
+}
+
+// one workgroup per run (blockIdx.x)
+template <int NZ>
+__global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
+{
+    __shared__ MFShared<NZ> sh;
+    meanfield_load<NZ>(a, blockIdx.x, sh);
+    meanfield_body<NZ>(a, blockIdx.x, sh);
 }
 
 // ---------------------------------------------------------------------------

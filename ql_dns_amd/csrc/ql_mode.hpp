@@ -13,7 +13,7 @@
 //   B  per (mode, kz slot): 1/lap2, 1/lapF and the base spectral fields
 //      (fac1D folded in; y, z components rebuilt from (ux, zeta), (bx, eta));
 //   C  inverse transforms of the padded lines, the first pass reading the
-//      base table directly (lines = bases or i kz * bases);
+//      base table directly (lines = bases, for MHD_BQlin also i kz uz);
 //   D  mean-field x fluctuation products on the z grid, x and y derivatives
 //      being i kx and i ky times the z profile; EMF / stress z-profiles summed
 //      in registers over the workgroup's modes;
@@ -75,14 +75,25 @@ __device__ __forceinline__ void yz_from(double2 f, double2 zt, double kx, double
     fz = make_double2((sz * f.x + ky * zt.y) * ilap2, (sz * f.y - ky * zt.x) * ilap2);
 }
 
+// The stress terms of MHD_FullQlin (MHD_FullQlin.cpp:339-362) are
+// -Re(u . grad conj(ux)) + Re(b . grad conj(bx)) and likewise for y, with
+// x and y derivatives i kx, i ky.  The reconstructed fields are divergence
+// free slot by slot (i kx ux + i ky uy + i kz uz = 0 wherever lap2 != 0, by
+// MHD_FullQlin.cpp:220-221), so Re(u . grad conj(ux)) = d/dz Re(uz conj(ux)):
+// the stresses are z-derivatives, accumulated here as
+// Re(bz conj(bx)) - Re(uz conj(ux)) and differentiated by the mean-field
+// kernel (x i kz) exactly as the EMF is, which takes the four dz lines out of
+// the inverse transforms.  The one slot the reconstruction leaves alone,
+// kz = 0 of a ky = 0 mode, holds ux = bx = 0 in a divergence-free state and
+// keeps it (its RHS is ilapF (-kx^2 T1) - T1 = 0).
 struct ModelFull {
     static constexpr int NB = 6;     // bases: ux uy uz bx by bz
-    static constexpr int NLINE = 10; // + dz ux, dz bx, dz uy, dz by
+    static constexpr int NLINE = 6;  // inverse-transformed lines: the bases
     static constexpr int NOUT = 6;   // forward lines (3 momentum, 3 induction)
     static constexpr int NMFZ = 8;   // MBx dzMBx MBy dzMBy MUx dzMUx MUy dzMUy
-    static constexpr int NACC = 4;   // EMF x, EMF y, stress x, stress y
-    __device__ static int base_of(int l) { return l < 6 ? l : (((l & 1) ? 3 : 0) + (l >= 8 ? 1 : 0)); }
-    __device__ static bool dz_of(int l) { return l >= 6; }
+    static constexpr int NACC = 4;   // EMF x, EMF y, stress x, stress y (as above)
+    __device__ static int base_of(int l) { return l; }
+    __device__ static bool dz_of(int) { return false; }
     template <class C>
     __device__ static void bases(double2 ux, double2 ze, double2 bx, double2 et, double kx, double ky, double kz,
                                  double ilap2, double fac, C *out, int stride)
@@ -100,7 +111,7 @@ struct ModelFull {
     __device__ static int nprod(int) { return 3; }
     __device__ static int out_line(int part, int c) { return part * 3 + c; }
     __device__ static int acc_slot(int part, int c) { return part * 2 + c; }
-    // MHD_FullQlin.cpp:271-300 (products) and :322-358 (EMF, stresses)
+    // MHD_FullQlin.cpp:271-300 (products) and :322-362 (EMF, stresses)
     template <class C>
     __device__ static void products(int part, const C *X, int ld, const RealOf<C> *mf, RealOf<C> kx, RealOf<C> ky,
                                     C *prod, double *racc)
@@ -123,17 +134,9 @@ struct ModelFull {
             prod[0] = csub(cadd(imul(aU, b0), cscale(dzMBx, u2)), cadd(imul(aB, u0), cscale(dzMUx, b2)));
             prod[1] = csub(cadd(imul(aU, b1), cscale(dzMBy, u2)), cadd(imul(aB, u1), cscale(dzMUy, b2)));
             prod[2] = csub(imul(aU, b2), imul(aB, u2));
-            const C dzux = X[6 * ld], dzbx = X[7 * ld], dzuy = X[8 * ld], dzby = X[9 * ld];
-            // x and y derivatives i kx f, i ky f formed in FP64
-            const double kxd = kx, kyd = ky;
-            const double2 U0 = to_d(u0), U1 = to_d(u1), B0 = to_d(b0), B1 = to_d(b1);
-            const double2 u3 = imul(kxd, U0), u6 = imul(kyd, U0), b3 = imul(kxd, B0), b6 = imul(kyd, B0);
-            const double2 u4 = imul(kxd, U1), u7 = imul(kyd, U1), b4 = imul(kxd, B1), b7 = imul(kyd, B1);
-            const double2 U2 = to_d(u2), B2 = to_d(b2);
-            racc[0] = -(redot(U0, u3) + redot(U1, u6) + redot64(u2, dzux)) + (redot(B0, b3) + redot(B1, b6) + redot64(b2, dzbx));
-            racc[1] = -(redot(U0, u4) + redot(U1, u7) + redot64(u2, dzuy)) + (redot(B0, b4) + redot(B1, b7) + redot64(b2, dzby));
-            (void)U2;
-            (void)B2;
+            // stresses before the z-derivative (see above)
+            racc[0] = redot64(b2, b0) - redot64(u2, u0);
+            racc[1] = redot64(b2, b1) - redot64(u2, u1);
         }
     }
     // MHD_FullQlin.cpp:283-285 (pair 0: ux, zeta) and :303-305 (pair 1: bx, eta)
