@@ -143,7 +143,7 @@ void launch_adj_nz(const qlk::AdjModeArgs &am, const qlk::AdjMFArgs &aa, const q
     } else if (stage == 1) {
         qlk::k_adj_mode<NZ, MB><<<W, qlk::NT, 0, st>>>(am);
         const int ncols = 8 * NZ + qlk::NGP;
-        qlk::k_reduce_cols<<<dim3((ncols + 15) / 16, 1), qlk::NT, 0, st>>>(gpart, rp1, ncols, grows);
+        qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, 1), qlk::NT, 0, st>>>(gpart, rp1, ncols, grows);
     } else {
         qlk::k_adj_mf_b<NZ><<<1, qlk::NT, 0, st>>>(ab);
     }

@@ -472,7 +472,7 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
     QL_HIP(hipGetLastError());
     if (b.timing) QL_HIP(hipEventRecord(b.ev[1], st));
     const int ncols = b.nMF * g.NZ;
-    qlk::k_reduce_cols<<<dim3((ncols + 15) / 16, g.nruns), qlk::NT, 0, st>>>(g.part, g.rp, ncols, g.sums);
+    qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(g.part, g.rp, ncols, g.sums);
     QL_HIP(hipGetLastError());
     launch_mf(g, m, st);
     QL_HIP(hipGetLastError());

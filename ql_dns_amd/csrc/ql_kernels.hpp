@@ -236,20 +236,25 @@ __device__ __forceinline__ void clock_advance(Clock *c)
 
 // ---------------------------------------------------------------------------
 // stress / EMF reduction: column c of run r's partial rows summed with
-// Neumaier compensation.  A block owns 16 columns of one run; its 16 row
-// chunks are summed in row order by separate threads and then added in chunk
-// order, so the result does not depend on scheduling.
+// Neumaier compensation.  A block (blockIdx.y = run) owns RED_CW columns; its
+// NT / RED_CW row chunks are summed in row order by separate threads and then
+// added in chunk order, so the result does not depend on scheduling.  No
+// cross-block communication (no fences, no atomics).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void reduce_cols_block(const double *__restrict__ part, const RunParams &rp, int ncols,
-                                                  double *__restrict__ sums, int cblk, double (*red)[17],
-                                                  double (*red2)[17])
+constexpr int RED_CW = 16;  // columns per block (4, 8 and 32 measured slower)
+
+[[maybe_unused]] static __global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, const RunParams *__restrict__ runs,
+                                                    int ncols, double *__restrict__ sums)
 {
-    const int cl = threadIdx.x % 16, chunk = threadIdx.x / 16;
-    const int c = cblk * 16 + cl;
+    constexpr int CW = RED_CW, CH = NT / CW;
+    __shared__ double red[CH][CW + 1], red2[CH][CW + 1];
+    const RunParams &rp = runs[blockIdx.y];
+    const int cl = threadIdx.x % CW, chunk = threadIdx.x / CW;
+    const int c = blockIdx.x * CW + cl;
     const int nrows = rp.nwg;
-    const int per = (nrows + 15) / 16;
+    const int per = (nrows + CH - 1) / CH;
     const int r0 = rp.wg_off + chunk * per, r1 = rp.wg_off + min(nrows, chunk * per + per);
-    constexpr int B = 32;  // rows in flight per thread (one batch up to 512 rows)
+    constexpr int B = 512 / CH;  // rows in flight per thread (one batch up to 512 rows)
     double s = 0.0, e = 0.0;
     if (c < ncols) {
         for (int r = r0; r < r1; r += B) {
@@ -269,23 +274,15 @@ __device__ __forceinline__ void reduce_cols_block(const double *__restrict__ par
     __syncthreads();
     if (chunk == 0 && c < ncols) {
         double t = 0.0, te = 0.0;
-        for (int k = 0; k < 16; ++k) {
+        for (int k = 0; k < CH; ++k) {
             const double v = red[k][cl];
             const double u = t + v;
             te += (fabs(t) >= fabs(v)) ? ((t - u) + v) : ((v - u) + t);
             te += red2[k][cl];
             t = u;
         }
-        sums[c] = t + te;
+        sums[(size_t)blockIdx.y * ncols + c] = t + te;
     }
-}
-
-// grid (column blocks, runs)
-[[maybe_unused]] static __global__ __launch_bounds__(NT) void k_reduce_cols(const double *__restrict__ part, const RunParams *__restrict__ runs,
-                                                    int ncols, double *__restrict__ sums)
-{
-    __shared__ double red[16][17], red2[16][17];
-    reduce_cols_block(part, runs[blockIdx.y], ncols, sums + (size_t)blockIdx.y * ncols, blockIdx.x, red, red2);
 }
 
 // sum over the workgroup's threads in a fixed order (every thread gets it)
@@ -330,6 +327,7 @@ struct MFShared {
     double2 line[8 * NZ];                     // stress spectra, then the profile lines
     double2 tw[NZ];
     double skz[NZD];
+    double ssum[4 * NZ];      // summed stress / EMF z-profiles
     double2 smf[4 * NZD];     // mean fields at the start of the substep
     double2 sprev[4 * NZD];   // previous substep's mean-field RHS
     double2 sdrive[4 * NZD];  // drives (HD_fullU records)
@@ -337,31 +335,57 @@ struct MFShared {
     double red[NT];           // block sums
 };
 
-// Inputs of the mean-field update other than the summed profiles (twiddles,
-// wavenumbers, mean fields, previous RHS), stored to LDS without a barrier:
-// meanfield_body reads them only after its first one.
+// Inputs of the mean-field kernel (twiddles, wavenumbers, mean fields,
+// previous RHS, summed profiles) into LDS, every load issued before the first
+// store so that they are in flight together; meanfield_body starts with the
+// barrier that makes them visible.
 template <int NZ>
 __device__ __forceinline__ void meanfield_load(const MFArgs &a, int run, MFShared<NZ> &sh)
 {
     constexpr int NZD = Dealias<NZ>::nz;
+    constexpr int TW = (NZ + NT - 1) / NT, MW = (4 * NZD + NT - 1) / NT, SW = (4 * NZ + NT - 1) / NT;
+    static_assert(NZD <= NT, "one wavenumber per thread");
     const int tid = threadIdx.x;
-    for (int k = tid; k < NZ; k += NT) sh.tw[k] = a.tw[k];
-    for (int k = tid; k < NZD; k += NT) sh.skz[k] = a.kzv[k];
-    const int nslot = a.nMF * NZD;
+    const int nslot = a.nMF * NZD, nsum = a.nMF * NZ;
     const size_t mo = (size_t)run * nslot;
-    if (a.mode != MODE_PROFILES) {
-        for (int k = tid; k < nslot; k += NT) {
-            sh.smf[k] = a.mf[mo + k];
-            sh.sprev[k] = a.mfn_prev ? a.mfn_prev[mo + k] : make_double2(0.0, 0.0);
+    const bool upd = a.mode != MODE_PROFILES;
+    double2 t[TW], m[MW], p[MW];
+    double su[SW];
+#pragma unroll
+    for (int w = 0; w < TW; ++w) t[w] = (tid + w * NT < NZ) ? a.tw[tid + w * NT] : make_double2(0.0, 0.0);
+    const double kz = tid < NZD ? a.kzv[tid] : 0.0;
+#pragma unroll
+    for (int w = 0; w < MW; ++w) {
+        const int k = tid + w * NT;
+        m[w] = (k < nslot) ? a.mf[mo + k] : make_double2(0.0, 0.0);
+        p[w] = (upd && k < nslot && a.mfn_prev) ? a.mfn_prev[mo + k] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int w = 0; w < SW; ++w) {
+        const int k = tid + w * NT;
+        su[w] = (upd && k < nsum) ? a.sums[(size_t)run * nsum + k] : 0.0;
+    }
+#pragma unroll
+    for (int w = 0; w < TW; ++w)
+        if (tid + w * NT < NZ) sh.tw[tid + w * NT] = t[w];
+    if (tid < NZD) sh.skz[tid] = kz;
+#pragma unroll
+    for (int w = 0; w < MW; ++w) {
+        const int k = tid + w * NT;
+        if (k < nslot) {
+            sh.smf[k] = m[w];
+            sh.sprev[k] = p[w];
         }
     }
+#pragma unroll
+    for (int w = 0; w < SW; ++w)
+        if (tid + w * NT < nsum) sh.ssum[tid + w * NT] = su[w];
 }
 
 // Both transforms are two passes (radix 16 x 8 at NZ = 128) between separate
-// LDS regions: the forward one reads the summed profiles from global memory
-// in its first pass, the inverse one builds its padded input lines from the
-// updated spectra in its first pass and stores the real profiles in its
-// second, so a substep takes five barriers.  meanfield_load must have run.
+// LDS regions; the inverse one builds its padded input lines from the updated
+// spectra in its first pass and stores the real profiles in its second, so a
+// substep takes six barriers.  meanfield_load must have run.
 template <int NZ>
 __device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShared<NZ> &sh)
 {
@@ -405,10 +429,11 @@ __device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShare
     const size_t mo = (size_t)run * nslot;
     const PadIO<NZ, double2> pad{sh.pad};
     const BufIO<NZ, double2> lines{sh.line};
+    __syncthreads();  // the inputs meanfield_load stored
 
     if (a.mode != MODE_PROFILES) {
         // forward transform of the summed z-profiles (real lines)
-        const double *S = a.sums + (size_t)run * nMF * NZ;
+        const double *S = sh.ssum;
         const auto ld = [S](int l, int j) { return make_double2(S[l * NZ + j], 0.0); };
         if (nMF == 4) {
             fft_pass_io<NZ, 4, false, PL::R1, 1, false>(sh.tw, ld, pad);
@@ -472,7 +497,7 @@ __device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShare
         if (hd) drive_means(sh.sdrive);
         if (a.mode == MODE_RHS) return;
     } else {
-        for (int k = tid; k < nslot; k += NT) sh.smfn[k] = a.mf[mo + k];
+        for (int k = tid; k < nslot; k += NT) sh.smfn[k] = sh.smf[k];
         __syncthreads();
         if (hd) drive_means(a.drive + mo);  // the last rhs's drive
     }
