@@ -356,29 +356,32 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     const double remap_qt = to_sgpr(rn_mul(ph.q, st.tnew));
     const bool do_remap = a.do_remap && rp.remap;
 
-    const C *twg = reinterpret_cast<const C *>(sizeof(C) == 8 ? a.tw32 : (const void *)a.tw);
-    for (int k = tid; k < NZ; k += TPB) tw[k] = twg[k];
-    for (int k = tid; k < NZD; k += TPB) skzr[k] = (R)a.kzv[k];
-
     // the thread's items: (imb, im) for the bases and the update, (zmb, zz)
     // for the products; both fixed for the whole kernel
     const bool has_it = tid < NIT, has_z = tid < NZI;
     const int imb = tid / NZD, im = tid - imb * NZD;
     const int zmb = tid / NZ, zz = tid - zmb * NZ;
+
+    // tables: every load is issued before the first LDS store, so that they
+    // are in flight together
+    const C *twg = reinterpret_cast<const C *>(sizeof(C) == 8 ? a.tw32 : (const void *)a.tw);
+    const double *mfz = a.mfz + (size_t)run * 8 * NZ;
+    constexpr int TWN = (NZ + TPB - 1) / TPB, MFN = MF_REG ? 1 : (NZ * P::NMFZ + TPB - 1) / TPB;
+    static_assert(NZD <= TPB, "one wavenumber per thread");
+    C twv[TWN];
+    double mfv[MFN];
+#pragma unroll
+    for (int w = 0; w < TWN; ++w) twv[w] = (tid + w * TPB < NZ) ? twg[tid + w * TPB] : Cx<C>::make(0, 0);
+    const double kzt = tid < NZD ? a.kzv[tid] : 0.0;
     const double kz = has_it ? a.kzv[im] : 0.0, kz2 = -(kz * kz);
     R mfr[MF_REG ? P::NMFZ : 1];
-    {
-        const double *mfz = a.mfz + (size_t)run * 8 * NZ;
-        if constexpr (MF_REG) {
+    if constexpr (MF_REG) {
 #pragma unroll
-            for (int c = 0; c < P::NMFZ; ++c) mfr[c] = has_z ? (R)mfz[c * NZ + zz] : (R)0;
-        } else {
-            for (int k = tid; k < NZ * P::NMFZ; k += TPB) smfz[k] = (R)mfz[k];
-        }
+        for (int c = 0; c < P::NMFZ; ++c) mfr[c] = has_z ? (R)mfz[c * NZ + zz] : (R)0;
+    } else {
+#pragma unroll
+        for (int w = 0; w < MFN; ++w) mfv[w] = (tid + w * TPB < NZ * P::NMFZ) ? mfz[tid + w * TPB] : 0.0;
     }
-    double acc[P::NACC];
-#pragma unroll
-    for (int c = 0; c < P::NACC; ++c) acc[c] = 0.0;
 
     // run-local mode index of this thread's item in group g, or -1
     auto local_mode = [&](int g) {
@@ -400,6 +403,18 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         kpre[1] = (l >= 0) ? a.ky[i] : 0.0;
     };
     if (AHEAD) fetch(g0);
+#pragma unroll
+    for (int w = 0; w < TWN; ++w)
+        if (tid + w * TPB < NZ) tw[tid + w * TPB] = twv[w];
+    if (tid < NZD) skzr[tid] = (R)kzt;
+    if constexpr (!MF_REG) {
+#pragma unroll
+        for (int w = 0; w < MFN; ++w)
+            if (tid + w * TPB < NZ * P::NMFZ) smfz[tid + w * TPB] = (R)mfv[w];
+    }
+    double acc[P::NACC];
+#pragma unroll
+    for (int c = 0; c < P::NACC; ++c) acc[c] = 0.0;
 
     for (int g = g0; g < g1; ++g) {
         // B: state of the item (prefetched), next group's state, previous RHS;
