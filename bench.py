@@ -297,6 +297,10 @@ def main():
     finite = bool(abs(lin2).max() < 1e300 and abs(mf2).max() < 1e300)
     ctx.close()
 
+    # the mode kernel the tuned shape launches (line 64: one wavefront per mode)
+    elin = 1 if args.integrator == "RK3CN_Elin" else 2
+    kname = (f"k_wave64<{p.NZ},ModelFull,{elin}>" if shape.get("line") == 64
+             else f"k_mode64<{p.NZ},{shape.get('modes_per_pass')},ModelFull,{elin}>")
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -316,9 +320,9 @@ def main():
                    "remap": bool(p.remap), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_mode_full", "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "kernel": kname, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_ms[0]},
-        "kernels_ms": {"k_mode_full": avg_ms[0], "k_reduce_cols+k_meanfield": avg_ms[1],
+        "kernels_ms": {kname: avg_ms[0], "k_reduce_cols+k_meanfield": avg_ms[1],
                        "launches_profiled": counts},
         "state_finite": finite,
         "launch_shape": shape,

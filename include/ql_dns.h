@@ -211,7 +211,8 @@ int ql_batch_linear_ops_init(ql_batch *b, double t0);
 /* nsteps of main.cpp:108-123 for every run (as ql_steps), one clock */
 int ql_batch_steps(ql_batch *b, int nsteps, double *t_inout);
 /* Launch-shape tuner: for each NZ group, times every candidate shape (z
- * points per workgroup pass 128 or 256, target workgroup count) over
+ * points per workgroup pass 128 or 256, or one wavefront per mode (64), and
+ * a target workgroup count) over
  * trial_steps steps and keeps the fastest; the state is restored afterwards.
  * *step_ms (may be NULL) receives the sum over groups of the best step times.
  * The shape sets the summation order of the stress reduction: results of two
@@ -220,7 +221,9 @@ int ql_batch_tune(ql_batch *b, int trial_steps, double *step_ms);
 /* current shape per group: cfg[5k..5k+4] = NZ, line, target, modes per
  * workgroup pass, workgroups; *ngroups receives the group count */
 int ql_batch_get_config(const ql_batch *b, int max_groups, int *ngroups, int *cfg);
-/* sets the shape of the NZ group: line in {128, 256}, target >= 1 */
+/* sets the shape of the NZ group: line in {64, 128, 256}, target >= 1.
+ * line 64 selects the kernel with one wavefront per mode (modes per pass 1;
+ * MHD_FullQlin and HD_fullU at NZ <= 128, else QL_ERR_ARG) */
 int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
 /* ------------------------------------------------------------------------

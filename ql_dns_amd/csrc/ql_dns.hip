@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "ql_mode.hpp"
+#include "ql_mode_wave.hpp"
 #include "ql_runtime.hpp"
 
 namespace qlh {
@@ -188,6 +189,9 @@ qlk::RunParams run_params(const RunInfo &r)
 // near-equal contiguous ranges
 int configure(Batch &b, Group &g, int line, int target)
 {
+    if (line == LINE_WAVE && !wave_ok(b.model, g.NZ))
+        return fail(QL_ERR_ARG, "line %d (one wavefront per mode) needs MHD_FullQlin or HD_fullU at NZ <= 128",
+                    LINE_WAVE);
     // the tables are rewritten below: nothing queued may still read them
     QL_RC(sync_all(b));
     const int MB = mb_for(g.NZ, line);
@@ -345,7 +349,7 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_HIP(hipMemcpy(g.kzv, r0.h_kz.data(), sizeof(double) * g.nz, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw, htw.data(), sizeof(double2) * g.NZ, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw32, htw32.data(), sizeof(float2) * g.NZ, hipMemcpyHostToDevice));
-        QL_RC(configure(b, g, DEFAULT_LINE, DEFAULT_TARGET));
+        QL_RC(configure(b, g, default_line(b.model, g.NZ), DEFAULT_TARGET));
     }
     return QL_OK;
 }
@@ -367,6 +371,22 @@ void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
     }
 }
 
+template <int NZ, int MODE>
+void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    constexpr int T = qlk::MODE_WAVES * qlk::WAVE;
+    if (b.precision == QL_PREC_MIXED) qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+    else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+}
+
+template <int NZ>
+void launch_wave_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    if (a.mode == qlk::MODE_ELIN) launch_wave_m<NZ, qlk::MODE_ELIN>(b, g, a, st);
+    else if (a.mode == qlk::MODE_CN) launch_wave_m<NZ, qlk::MODE_CN>(b, g, a, st);
+    else launch_wave_m<NZ, qlk::MODE_RHS>(b, g, a, st);
+}
+
 template <int NZ, int MB>
 void launch_mode_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
@@ -377,6 +397,15 @@ void launch_mode_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
 
 void launch_mode(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
+    if (g.line == LINE_WAVE) {
+        // configure admits it only where wave_ok
+        switch (g.NZ) {
+        case 32: launch_wave_t<32>(b, g, a, st); break;
+        case 64: launch_wave_t<64>(b, g, a, st); break;
+        default: launch_wave_t<128>(b, g, a, st); break;
+        }
+        return;
+    }
     switch (g.NZ) {
     case 32: g.MB == 4 ? launch_mode_t<32, 4>(b, g, a, st) : launch_mode_t<32, 8>(b, g, a, st); break;
     case 64: g.MB == 2 ? launch_mode_t<64, 2>(b, g, a, st) : launch_mode_t<64, 4>(b, g, a, st); break;
@@ -854,14 +883,16 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
         Group &g = b.groups[gi];
         double best = 1e300;
         int best_line = g.line, best_target = g.target;
-        std::vector<std::pair<int, int>> seen;  // (MB, W) already timed
+        std::vector<std::pair<int, int>> seen;  // (kernel and MB, W) already timed
         for (int line : LINES) {
+            if (line == LINE_WAVE && !wave_ok(b.model, g.NZ)) continue;
             for (int target : TARGETS) {
                 if ((rc = configure(b, g, line, target))) break;
+                const int kind = line == LINE_WAVE ? -1 : g.MB;
                 bool dup = false;
-                for (auto &sw : seen) dup = dup || (sw.first == g.MB && sw.second == g.W);
+                for (auto &sw : seen) dup = dup || (sw.first == kind && sw.second == g.W);
                 if (dup) continue;
-                seen.emplace_back(g.MB, g.W);
+                seen.emplace_back(kind, g.W);
                 for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, false, b.s);
                 if (rc) break;
                 if (hipEventRecord(b.ev[0], b.s) != hipSuccess) {
@@ -1391,7 +1422,8 @@ int ql_batch_get_config(const ql_batch *bt, int max_groups, int *ngroups, int *c
 int ql_batch_set_config(ql_batch *bt, int NZ, int line, int target)
 {
     if (!bt) return fail(QL_ERR_ARG, "ql_batch_set_config: null handle");
-    if (line != 128 && line != 256) return fail(QL_ERR_ARG, "ql_batch_set_config: line=%d not in {128, 256}", line);
+    if (line != LINE_WAVE && line != 128 && line != 256)
+        return fail(QL_ERR_ARG, "ql_batch_set_config: line=%d not in {64, 128, 256}", line);
     if (target < 1) return fail(QL_ERR_ARG, "ql_batch_set_config: target=%d < 1", target);
     Batch &b = bt->b;
     for (Group &g : b.groups)

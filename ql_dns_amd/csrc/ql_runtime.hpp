@@ -41,13 +41,19 @@ constexpr double QL_PI = 3.14159265358979323846;
 // steps captured in the long replay graph
 constexpr int GRAPH_STEPS = 16;
 // launch-shape candidates of the tuner: z points per workgroup pass
-// (modes per workgroup MB = line / NZ) and the target workgroup count
-constexpr int LINES[] = {128, 256};
+// (modes per workgroup MB = line / NZ) and the target workgroup count; line
+// LINE_WAVE selects the kernel with one wavefront per mode (MB = 1,
+// ql_mode_wave.hpp; MHD_FullQlin and HD_fullU at NZ <= 128)
+constexpr int LINE_WAVE = 64;
+constexpr int LINES[] = {LINE_WAVE, 128, 256};
 constexpr int TARGETS[] = {256, 512, 1024, 2048, 4096, 1 << 30};
 constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
 
 inline bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
-inline int mb_for(int NZ, int line) { return NZ >= line ? 1 : line / NZ; }
+inline int mb_for(int NZ, int line) { return (line == LINE_WAVE || NZ >= line) ? 1 : line / NZ; }
+inline bool wave_ok(int model, int NZ) { return model != 1 && NZ <= 128; }  // model 1: MHD_BQlin
+// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128)
+inline int default_line(int model, int NZ) { return (NZ == 128 && wave_ok(model, NZ)) ? LINE_WAVE : DEFAULT_LINE; }
 
 // one run: parameters, sizes and its place in its group
 struct RunInfo {
