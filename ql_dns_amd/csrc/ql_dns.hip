@@ -349,7 +349,8 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_HIP(hipMemcpy(g.kzv, r0.h_kz.data(), sizeof(double) * g.nz, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw, htw.data(), sizeof(double2) * g.NZ, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw32, htw32.data(), sizeof(float2) * g.NZ, hipMemcpyHostToDevice));
-        QL_RC(configure(b, g, default_line(b.model, g.NZ), DEFAULT_TARGET));
+        const int line = default_line(b.model, g.NZ);
+        QL_RC(configure(b, g, line, default_target(line)));
     }
     return QL_OK;
 }

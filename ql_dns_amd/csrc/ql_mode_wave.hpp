@@ -356,8 +356,11 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     }
 }
 
-// workgroups of MODE_WAVES wavefronts; FP64 (k_wave64) and mixed (k_wave32)
-constexpr int MODE_WAVES = 4;
+// workgroups of MODE_WAVES wavefronts (one workgroup per CU at NZ = 128 in
+// FP64: 115 KB of LDS); FP64 (k_wave64) and mixed (k_wave32).  Measured at
+// 64x128x128: 8 wavefronts and 252 workgroups 31.1 us, 4 and 504 31.9 us, 2
+// and 1008 31.6 us (and a slower column reduction of twice the rows)
+constexpr int MODE_WAVES = 8;
 template <int NZ, class P, int MODE>
 __global__ __launch_bounds__(MODE_WAVES * WAVE) __attribute__((amdgpu_waves_per_eu(2)))
 void k_wave64(ModeArgs a)

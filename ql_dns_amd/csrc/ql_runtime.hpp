@@ -52,8 +52,10 @@ constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
 inline bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
 inline int mb_for(int NZ, int line) { return (line == LINE_WAVE || NZ >= line) ? 1 : line / NZ; }
 inline bool wave_ok(int model, int NZ) { return model != 1 && NZ <= 128; }  // model 1: MHD_BQlin
-// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128)
+// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128),
+// with about one workgroup (8 wavefronts) per CU
 inline int default_line(int model, int NZ) { return (NZ == 128 && wave_ok(model, NZ)) ? LINE_WAVE : DEFAULT_LINE; }
+inline int default_target(int line) { return line == LINE_WAVE ? 256 : DEFAULT_TARGET; }
 
 // one run: parameters, sizes and its place in its group
 struct RunInfo {
