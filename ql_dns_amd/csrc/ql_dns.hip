@@ -376,8 +376,14 @@ template <int NZ, int MODE>
 void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
     constexpr int T = qlk::MODE_WAVES * qlk::WAVE;
-    if (b.precision == QL_PREC_MIXED) qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
-    else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+    // HD_fullU: the velocity-only policy (its magnetic fields are zero)
+    if (b.precision == QL_PREC_MIXED) {
+        if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
+        else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+    } else {
+        if (b.hd) qlk::k_wave64<NZ, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
+        else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+    }
 }
 
 template <int NZ>

@@ -87,6 +87,7 @@ __device__ __forceinline__ void yz_from(double2 f, double2 zt, double kx, double
 // kz = 0 of a ky = 0 mode, holds ux = bx = 0 in a divergence-free state and
 // keeps it (its RHS is ilapF (-kx^2 T1) - T1 = 0).
 struct ModelFull {
+    static constexpr int NPAIR = 2;  // field pairs updated: (ux, zeta), (bx, eta)
     static constexpr int NB = 6;     // bases: ux uy uz bx by bz
     static constexpr int NLINE = 6;  // inverse-transformed lines: the bases
     static constexpr int NOUT = 6;   // forward lines (3 momentum, 3 induction)
@@ -174,7 +175,80 @@ struct ModelFull {
     }
 };
 
+// HD_fullU (Models/HD_fullU.cpp:145-309) on the MHD_FullQlin state layout,
+// whose magnetic fields it holds at zero (b = eta = 0, B0z = 0): the velocity
+// bases and the momentum products only, each exactly as ModelFull computes
+// them with b = 0 (the terms dropped are products with zero), and the
+// Reynolds stresses -Re(uz conj(ux)), -Re(uz conj(uy)) before their
+// z-derivative in the stress slots 2, 3 (the EMF slots stay zero).  Only the
+// (ux, zeta) pair is read, updated and written.
+struct ModelHD {
+    static constexpr int NPAIR = 1;
+    static constexpr int NB = 3;     // bases: ux uy uz
+    static constexpr int NLINE = 3;
+    static constexpr int NOUT = 3;   // momentum lines
+    static constexpr int NMFZ = 8;   // the MHD_FullQlin profile table (Ux, Uy at 4-7)
+    static constexpr int NACC = 4;
+    __device__ static int base_of(int l) { return l; }
+    __device__ static bool dz_of(int) { return false; }
+    template <class C>
+    __device__ static void bases(double2 ux, double2 ze, double2, double2, double kx, double ky, double kz,
+                                 double ilap2, double fac, C *out, int stride)
+    {
+        double2 uy, uz;
+        yz_from(ux, ze, kx, ky, kz, ilap2, uy, uz);
+        out[0] = from_d<C>(cscale(fac, ux));
+        out[stride] = from_d<C>(cscale(fac, uy));
+        out[2 * stride] = from_d<C>(cscale(fac, uz));
+    }
+    __device__ static int nprod(int part) { return part == 0 ? 3 : 0; }
+    __device__ static int out_line(int, int c) { return c; }
+    __device__ static int acc_slot(int, int c) { return 2 + c; }
+    template <class C>
+    __device__ static void products(int part, const C *X, int ld, const RealOf<C> *mf, RealOf<C> kx, RealOf<C> ky,
+                                    C *prod, double *racc)
+    {
+        typedef RealOf<C> R;
+        racc[0] = racc[1] = 0.0;
+        if (part != 0) return;
+        const C u0 = X[0], u1 = X[ld], u2 = X[2 * ld];
+        const R MUx = mf[4], dzMUx = mf[5], MUy = mf[6], dzMUy = mf[7];
+        const R aU = kx * MUx + ky * MUy;
+        // U.grad(u) + uz dzU (MHD_FullQlin.cpp:271-280 with B = b = 0)
+        prod[0] = cadd(imul(aU, u0), cscale(dzMUx, u2));
+        prod[1] = cadd(imul(aU, u1), cscale(dzMUy, u2));
+        prod[2] = imul(aU, u2);
+        racc[0] = -redot64(u2, u0);
+        racc[1] = -redot64(u2, u1);
+    }
+    template <class C>
+    __device__ static void rhs_pair(int, const C *T, int ld, double2 ux, double2 ze, double2, double2, double kx,
+                                    double ky, double kz, double ilapF, const Phys &a, double2 &N0, double2 &N1)
+    {
+        // HD_fullU.cpp:283-286 (ModelFull's pair 0 with b = 0)
+        const double q = a.q, Om = a.omega;
+        const double2 T1 = to_d(T[0]), T2 = to_d(T[ld]), T3 = to_d(T[2 * ld]);
+        double2 s = cscale(2.0 * kx * ky * q, ux);
+        s = cadd(s, imul(2.0 * Om * kz, ze));
+        s = csub(s, cscale(kx * kx, T1));
+        s = csub(s, cscale(kx * ky, T2));
+        s = csub(s, cscale(kx * kz, T3));
+        N0 = csub(cscale(ilapF, s), T1);
+        N1 = imul((q - 2.0 * Om) * kz, ux);
+        N1 = csub(N1, csub(imul(kz, T2), imul(ky, T3)));
+    }
+    __device__ static double lin(int pr, double kxn, double ky, double kz2, const Phys &a)
+    {
+        return ModelFull::lin(pr, kxn, ky, kz2, a);
+    }
+    __device__ static double lin_init(int pr, double kxo, double lap2, const Phys &a)
+    {
+        return ModelFull::lin_init(pr, kxo, lap2, a);
+    }
+};
+
 struct ModelBQ {
+    static constexpr int NPAIR = 2;
     static constexpr int NB = 8;    // bases: ux zeta bx eta uy uz by bz
     static constexpr int NLINE = 9; // + dz uz
     static constexpr int NOUT = 5;  // u (two parts), b, zeta, eta

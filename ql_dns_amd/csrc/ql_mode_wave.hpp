@@ -84,7 +84,8 @@ __device__ __forceinline__ void wave_pass(int lane, const C *tw, const Ld &ld, c
 
 // LDS layout of one wavefront's buffer (one mode): base table [base][NZDP],
 // z lines [line][LS], transform intermediate [line][LP], transformed products
-// [line][nz], all from slot 0 (each stage's inputs are dead once it stores)
+// [line][nz], all from slot 0 (each stage's inputs are dead once it stores);
+// at the end the wavefront's EMF / stress sums [NACC][NZ] (FP64)
 template <int NZ, class P, class C>
 struct WaveLayout {
     static constexpr int NZD = Dealias<NZ>::nz;
@@ -92,7 +93,8 @@ struct WaveLayout {
     static constexpr int LS = bank_stride(NZ);
     static constexpr int LP = PadIO<NZ, C>::LP;
     static constexpr int cmax(int x, int y) { return x > y ? x : y; }
-    static constexpr int BUF = cmax(cmax(P::NB * NZDP, P::NLINE * cmax(LS, LP)), P::NOUT * NZD);
+    static constexpr int RED = (P::NACC * NZ * (int)sizeof(double) + (int)sizeof(C) - 1) / (int)sizeof(C);
+    static constexpr int BUF = cmax(cmax(cmax(P::NB * NZDP, P::NLINE * cmax(LS, LP)), P::NOUT * NZD), RED);
 };
 
 template <int NZ, class P, class C>
@@ -198,7 +200,11 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
             const int m = lane + k * WAVE;
             if (KI * WAVE == NZD || m < NZD) {
                 const double2 u0 = a.u[row + m], u1 = a.u[row + NZD + m];
-                const double2 u2 = a.u[row + 2 * NZD + m], u3 = a.u[row + 3 * NZD + m];
+                double2 u2 = make_double2(0.0, 0.0), u3 = u2;
+                if constexpr (P::NPAIR == 2) {
+                    u2 = a.u[row + 2 * NZD + m];
+                    u3 = a.u[row + 3 * NZD + m];
+                }
                 const double kzk = skz[m], kz2 = -(kzk * kzk);
                 const bool zero = (ky == 0.0 && m == 0);
                 const double lap2 = zero ? 0.0 : (-ky * ky + kz2);
@@ -268,13 +274,18 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
             // 1/lapF as in B (recomputed rather than held through the transforms)
             const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
             const double ilapF = (l == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
-            const double2 U0 = a.u[off], U1 = a.u[off + NZD], U2 = a.u[off + 2 * NZD], U3 = a.u[off + 3 * NZD];
+            const double2 U0 = a.u[off], U1 = a.u[off + NZD];
+            double2 U2 = make_double2(0.0, 0.0), U3 = U2;
+            if constexpr (P::NPAIR == 2) {
+                U2 = a.u[off + 2 * NZD];
+                U3 = a.u[off + 3 * NZD];
+            }
             double2 np[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f) np[f] = a.n_prev ? a.n_prev[off + f * NZD] : make_double2(0.0, 0.0);
+            for (int f = 0; f < 2 * P::NPAIR; ++f) np[f] = a.n_prev ? a.n_prev[off + f * NZD] : make_double2(0.0, 0.0);
             const C *T = buf + m;
 #pragma unroll
-            for (int pr = 0; pr < 2; ++pr) {
+            for (int pr = 0; pr < P::NPAIR; ++pr) {
                 double2 N0, N1;
                 P::rhs_pair(pr, T, NZD, U0, U1, U2, U3, kx, ky, kzk, ilapF, ph, N0, N1);
                 const double Ln = P::lin(pr, kx + qdtl * ky, ky, kz2, ph);
@@ -356,7 +367,7 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     }
 }
 
-// workgroups of MODE_WAVES wavefronts (one workgroup per CU at NZ = 128 in
+// MHD_FullQlin (ModelFull) and HD_fullU (ModelHD); workgroups of MODE_WAVES wavefronts (one workgroup per CU at NZ = 128 in
 // FP64: 115 KB of LDS); FP64 (k_wave64) and mixed (k_wave32).  Measured at
 // 64x128x128: 8 wavefronts and 252 workgroups 31.1 us, 4 and 504 31.9 us, 2
 // and 1008 31.6 us (and a slower column reduction of twice the rows)

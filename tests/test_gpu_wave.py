@@ -5,8 +5,8 @@ Bars as for the other shapes: states, drives and diagnostics within 1e-10
 relative of the oracle (kx0 after remaps and fixed-step times exactly equal);
 the same run under line 64 and under (256, 512), which sum the stress
 partials in different orders, within 1e-12; one shape bitwise reproducible;
-the mixed path within 1e-6 of FP64.  MHD_BQlin has no wavefront kernel and
-refuses the shape.
+the mixed path within 1e-6 of FP64.  HD_fullU runs its velocity-only policy
+(ModelHD) there.  MHD_BQlin has no wavefront kernel and refuses the shape.
 """
 import numpy as np
 import pytest
@@ -69,7 +69,8 @@ def test_rhs_matches_oracle(NZ):
 
 def test_hd_and_variable_steps_match_oracle():
     from ql_dns_amd.params import MODEL_HD_FULLU
-    for p in (_params(model=MODEL_HD_FULLU), _params(dt=-0.05, CFL=0.5)):
+    for p in (_params(model=MODEL_HD_FULLU), _params(model=MODEL_HD_FULLU, integrator=0, NX=8, NY=8, NZ=64),
+              _params(model=MODEL_HD_FULLU, NX=6, NY=4, NZ=128, dt=-0.05, CFL=0.6), _params(dt=-0.05, CFL=0.5)):
         o, g = _pair(p)
         o.rhs(0.0, 0.0)
         g.rhs(0.0, 0.0)
