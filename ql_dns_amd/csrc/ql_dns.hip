@@ -421,13 +421,37 @@ void launch_mode(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStre
     }
 }
 
+template <int NZ, int MODE>
+void launch_mf_m(const Group &g, const qlk::MFArgs &m, hipStream_t st)
+{
+    const bool rec = m.rec || m.vdt;
+    if (m.model == QL_MODEL_MHD_BQLIN) {
+        if (rec) qlk::k_meanfield<NZ, MODE, 1, true><<<g.nruns, qlk::NT, 0, st>>>(m);
+        else qlk::k_meanfield<NZ, MODE, 1, false><<<g.nruns, qlk::NT, 0, st>>>(m);
+    } else {
+        if (rec) qlk::k_meanfield<NZ, MODE, 0, true><<<g.nruns, qlk::NT, 0, st>>>(m);
+        else qlk::k_meanfield<NZ, MODE, 0, false><<<g.nruns, qlk::NT, 0, st>>>(m);
+    }
+}
+
+template <int NZ>
+void launch_mf_t(const Group &g, const qlk::MFArgs &m, hipStream_t st)
+{
+    switch (m.mode) {
+    case qlk::MODE_RHS: launch_mf_m<NZ, qlk::MODE_RHS>(g, m, st); break;
+    case qlk::MODE_ELIN: launch_mf_m<NZ, qlk::MODE_ELIN>(g, m, st); break;
+    case qlk::MODE_CN: launch_mf_m<NZ, qlk::MODE_CN>(g, m, st); break;
+    default: launch_mf_m<NZ, qlk::MODE_PROFILES>(g, m, st); break;
+    }
+}
+
 void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st)
 {
     switch (g.NZ) {
-    case 32: qlk::k_meanfield<32><<<g.nruns, qlk::NT, 0, st>>>(m); break;
-    case 64: qlk::k_meanfield<64><<<g.nruns, qlk::NT, 0, st>>>(m); break;
-    case 128: qlk::k_meanfield<128><<<g.nruns, qlk::NT, 0, st>>>(m); break;
-    default: qlk::k_meanfield<256><<<g.nruns, qlk::NT, 0, st>>>(m); break;
+    case 32: launch_mf_t<32>(g, m, st); break;
+    case 64: launch_mf_t<64>(g, m, st); break;
+    case 128: launch_mf_t<128>(g, m, st); break;
+    default: launch_mf_t<256>(g, m, st); break;
     }
 }
 
