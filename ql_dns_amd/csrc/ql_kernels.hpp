@@ -254,7 +254,7 @@ constexpr int RED_CW = 16;  // columns per block (4, 8 and 32 measured slower)
     const int nrows = rp.nwg;
     const int per = (nrows + CH - 1) / CH;
     const int r0 = rp.wg_off + chunk * per, r1 = rp.wg_off + min(nrows, chunk * per + per);
-    constexpr int B = 512 / CH;  // rows in flight per thread (one batch up to 512 rows)
+    constexpr int B = 256 / CH;  // rows in flight per thread (one batch up to 256 rows)
     double s = 0.0, e = 0.0;
     if (c < ncols) {
         for (int r = r0; r < r1; r += B) {

@@ -149,3 +149,19 @@ def test_bqlin_refuses_the_wave_shape():
         c.set_config(32, WAVE, 512)
     assert e.value.code == -1
     c.set_config(32, 128, 512)
+
+
+def test_noise_and_remap_match_oracle():
+    """Driven steps (the noise kernel remaps after the step) with the wavefront
+    kernel, remaps inside the window."""
+    p = _params(NX=8, NY=8, NZ=128, q=1.5, Ly=1.0)
+    o, g = _pair(p)
+    k_init = g.get_state()[2]
+    for x in (o, g):
+        x.set_noise(0.8, seed=5)
+    to, tg = o.steps(30, 0.0), g.steps(30, 0.0)
+    assert tg == to
+    lo, mo, ko = o.get_state()
+    lg, mg, kg = g.get_state()
+    assert np.array_equal(kg, ko) and not np.array_equal(kg, k_init)
+    assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL
