@@ -45,12 +45,13 @@ def _run(p, lin, mf, n):
 PARAMS = ("q", "omega", "nu", "eta", "B0z")
 
 
-@pytest.mark.parametrize("integrator", [1, 0])
-def test_adjoint_matches_finite_differences(integrator):
+@pytest.mark.parametrize("integrator,NZ", [(1, 32), (0, 32), (1, 128), (0, 128)])
+def test_adjoint_matches_finite_differences(integrator, NZ):
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state, mode_tables
-    # remaps happen within these steps (Ly = 1); B0z != 0 exercises its terms
-    p = _p(NX=8, NY=8, NZ=32, Ly=1.0, B0z=0.03, integrator=integrator, nu=1 / 1500, eta=1 / 1200)
+    # remaps happen within these steps (Ly = 1); B0z != 0 exercises its terms;
+    # at NZ = 128 the forward runs the wavefront kernel (the default shape there)
+    p = _p(NX=8, NY=8, NZ=NZ, Ly=1.0, B0z=0.03, integrator=integrator, nu=1 / 1500, eta=1 / 1200)
     n, every = 14, 5
     lin0, mf0 = initial_state(p)
     alpha = 0.01 / float((np.abs(lin0) ** 2).sum())
