@@ -189,8 +189,9 @@ qlk::RunParams run_params(const RunInfo &r)
 // near-equal contiguous ranges
 int configure(Batch &b, Group &g, int line, int target)
 {
-    if (line == LINE_WAVE && !wave_ok(b.model, g.NZ))
-        return fail(QL_ERR_ARG, "line %d (one wavefront per mode) needs MHD_FullQlin or HD_fullU at NZ <= 128",
+    if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision))
+        return fail(QL_ERR_ARG,
+                    "line %d (one wavefront per mode) needs MHD_FullQlin or HD_fullU, and FP64 at NZ = 256",
                     LINE_WAVE);
     // the tables are rewritten below: nothing queued may still read them
     QL_RC(sync_all(b));
@@ -349,7 +350,7 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_HIP(hipMemcpy(g.kzv, r0.h_kz.data(), sizeof(double) * g.nz, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw, htw.data(), sizeof(double2) * g.NZ, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw32, htw32.data(), sizeof(float2) * g.NZ, hipMemcpyHostToDevice));
-        const int line = default_line(b.model, g.NZ);
+        const int line = default_line(b.model, g.NZ, b.precision);
         QL_RC(configure(b, g, line, default_target(line)));
     }
     return QL_OK;
@@ -375,11 +376,13 @@ void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
 template <int NZ, int MODE>
 void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
-    constexpr int T = qlk::MODE_WAVES * qlk::WAVE;
+    constexpr int T = qlk::mode_waves(NZ) * qlk::WAVE;
     // HD_fullU: the velocity-only policy (its magnetic fields are zero)
     if (b.precision == QL_PREC_MIXED) {
-        if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
-        else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+        if constexpr (NZ <= 128) {  // wave_ok: no FP32 wavefront kernel at NZ = 256
+            if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
+            else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+        }
     } else {
         if (b.hd) qlk::k_wave64<NZ, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
         else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
@@ -409,7 +412,8 @@ void launch_mode(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStre
         switch (g.NZ) {
         case 32: launch_wave_t<32>(b, g, a, st); break;
         case 64: launch_wave_t<64>(b, g, a, st); break;
-        default: launch_wave_t<128>(b, g, a, st); break;
+        case 128: launch_wave_t<128>(b, g, a, st); break;
+        default: launch_wave_t<256>(b, g, a, st); break;
         }
         return;
     }
@@ -916,7 +920,7 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
         int best_line = g.line, best_target = g.target;
         std::vector<std::pair<int, int>> seen;  // (kernel and MB, W) already timed
         for (int line : LINES) {
-            if (line == LINE_WAVE && !wave_ok(b.model, g.NZ)) continue;
+            if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision)) continue;
             for (int target : TARGETS) {
                 if ((rc = configure(b, g, line, target))) break;
                 const int kind = line == LINE_WAVE ? -1 : g.MB;

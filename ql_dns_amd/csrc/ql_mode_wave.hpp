@@ -367,22 +367,25 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     }
 }
 
-// MHD_FullQlin (ModelFull) and HD_fullU (ModelHD); workgroups of MODE_WAVES wavefronts (one workgroup per CU at NZ = 128 in
-// FP64: 115 KB of LDS); FP64 (k_wave64) and mixed (k_wave32).  Measured at
-// 64x128x128: 8 wavefronts and 252 workgroups 31.1 us, 4 and 504 31.9 us, 2
-// and 1008 31.6 us (and a slower column reduction of twice the rows)
-constexpr int MODE_WAVES = 8;
+// MHD_FullQlin (ModelFull) and HD_fullU (ModelHD); workgroups of
+// mode_waves(NZ) wavefronts: 8 at NZ <= 128 (one workgroup per CU at NZ = 128
+// in FP64: 115 KB of LDS; measured at 64x128x128: 8 wavefronts and 252
+// workgroups 31.1 us, 4 and 504 31.9 us, 2 and 1008 31.6 us and twice the
+// partial rows for the column reduction), 4 at NZ = 256 (26 KB per wavefront
+// in FP64, one wavefront per SIMD with the whole register file)
+constexpr int mode_waves(int NZ) { return NZ >= 256 ? 4 : 8; }
+constexpr int wave_occ(int NZ) { return NZ >= 256 ? 1 : 2; }
 template <int NZ, class P, int MODE>
-__global__ __launch_bounds__(MODE_WAVES * WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(mode_waves(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ(NZ))))
 void k_wave64(ModeArgs a)
 {
-    mode_wave_body<NZ, P, double2, MODE, MODE_WAVES>(a);
+    mode_wave_body<NZ, P, double2, MODE, mode_waves(NZ)>(a);
 }
 template <int NZ, class P, int MODE>
-__global__ __launch_bounds__(MODE_WAVES * WAVE) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(mode_waves(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ(NZ))))
 void k_wave32(ModeArgs a)
 {
-    mode_wave_body<NZ, P, float2, MODE, MODE_WAVES>(a);
+    mode_wave_body<NZ, P, float2, MODE, mode_waves(NZ)>(a);
 }
 
 }  // namespace qlk

@@ -43,7 +43,7 @@ constexpr int GRAPH_STEPS = 16;
 // launch-shape candidates of the tuner: z points per workgroup pass
 // (modes per workgroup MB = line / NZ) and the target workgroup count; line
 // LINE_WAVE selects the kernel with one wavefront per mode (MB = 1,
-// ql_mode_wave.hpp; MHD_FullQlin and HD_fullU at NZ <= 128)
+// ql_mode_wave.hpp; where wave_ok)
 constexpr int LINE_WAVE = 64;
 constexpr int LINES[] = {LINE_WAVE, 128, 256};
 constexpr int TARGETS[] = {256, 512, 1024, 2048, 4096, 1 << 30};
@@ -51,10 +51,17 @@ constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
 
 inline bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
 inline int mb_for(int NZ, int line) { return (line == LINE_WAVE || NZ >= line) ? 1 : line / NZ; }
-inline bool wave_ok(int model, int NZ) { return model != 1 && NZ <= 128; }  // model 1: MHD_BQlin
-// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128),
-// with about one workgroup (8 wavefronts) per CU
-inline int default_line(int model, int NZ) { return (NZ == 128 && wave_ok(model, NZ)) ? LINE_WAVE : DEFAULT_LINE; }
+// where the wavefront kernel exists and is used: not for MHD_BQlin (model 1),
+// and at NZ = 256 only in FP64 (there it runs one wavefront per SIMD; the
+// FP32 version measured slower than the workgroup kernel: 109 vs 101 us at
+// 128x128x256)
+inline bool wave_ok(int model, int NZ, int precision) { return model != 1 && (NZ <= 128 || precision == 0); }
+// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128,
+// and NZ = 256 in FP64), with about one workgroup per CU
+inline int default_line(int model, int NZ, int precision)
+{
+    return (NZ >= 128 && wave_ok(model, NZ, precision)) ? LINE_WAVE : DEFAULT_LINE;
+}
 inline int default_target(int line) { return line == LINE_WAVE ? 256 : DEFAULT_TARGET; }
 
 // one run: parameters, sizes and its place in its group

@@ -6,7 +6,8 @@ relative of the oracle (kx0 after remaps and fixed-step times exactly equal);
 the same run under line 64 and under (256, 512), which sum the stress
 partials in different orders, within 1e-12; one shape bitwise reproducible;
 the mixed path within 1e-6 of FP64.  HD_fullU runs its velocity-only policy
-(ModelHD) there.  MHD_BQlin has no wavefront kernel and refuses the shape.
+(ModelHD) there.  MHD_BQlin has no wavefront kernel, nor has the FP32 path at
+NZ = 256; both refuse the shape.
 """
 import numpy as np
 import pytest
@@ -40,7 +41,8 @@ def _pair(p, target=512, seed=None):
 
 
 @pytest.mark.parametrize("integrator", [1, 0])
-@pytest.mark.parametrize("NX,NY,NZ,target", [(16, 16, 32, 512), (8, 8, 64, 7), (6, 6, 128, 3), (4, 4, 128, 1 << 30)])
+@pytest.mark.parametrize("NX,NY,NZ,target", [(16, 16, 32, 512), (8, 8, 64, 7), (6, 6, 128, 3), (4, 4, 128, 1 << 30),
+                                               (6, 4, 256, 5)])
 def test_steps_match_oracle(integrator, NX, NY, NZ, target):
     p = _params(integrator=integrator, NX=NX, NY=NY, NZ=NZ)
     o, g = _pair(p, target)
@@ -142,13 +144,15 @@ def test_run_set_matches_oracle_per_run():
         assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL
 
 
-def test_bqlin_refuses_the_wave_shape():
+def test_bqlin_and_fp32_at_256_refuse_the_wave_shape():
     from ql_dns_amd.capi import QLContext, QLError
-    c = QLContext(_params(model=1))
-    with pytest.raises(QLError) as e:
-        c.set_config(32, WAVE, 512)
-    assert e.value.code == -1
-    c.set_config(32, 128, 512)
+    from ql_dns_amd.params import PREC_MIXED
+    for p, NZ in ((_params(model=1), 32), (_params(NX=4, NY=4, NZ=256, precision=PREC_MIXED), 256)):
+        c = QLContext(p)
+        with pytest.raises(QLError) as e:
+            c.set_config(NZ, WAVE, 512)
+        assert e.value.code == -1
+        c.set_config(NZ, 128 if NZ == 32 else 256, 512)
 
 
 def test_noise_and_remap_match_oracle():
