@@ -1,6 +1,8 @@
-// ql_mode_wave.hpp -- the per-(kx,ky) substep kernel of MHD_FullQlin with one
-// wavefront per mode (Models/MHD_FullQlin.cpp:171-427 and the substep updates
-// of Integrators/RK3CN_Elin.cpp:290-378, Integrators/RK3CN.cpp:92-193).
+// ql_mode_wave.hpp -- the per-(kx,ky) substep kernel of MHD_FullQlin (and of
+// HD_fullU through its velocity-only policy, ModelHD) with one wavefront per
+// mode (Models/MHD_FullQlin.cpp:171-427, Models/HD_fullU.cpp:145-309 and the
+// substep updates of Integrators/RK3CN_Elin.cpp:290-378,
+// Integrators/RK3CN.cpp:92-193).
 //
 // Same stages as mode_body (ql_mode.hpp) -- bases, inverse transforms,
 // products and EMF / stress sums, forward transforms, RHS and update -- but a
