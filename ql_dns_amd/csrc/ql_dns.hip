@@ -364,11 +364,14 @@ void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
 {
     const bool full = (b.model == QL_MODEL_MHD_FULLQLIN);
     constexpr int T = qlk::mode_threads(NZ, MB);
+    // HD_fullU: the velocity-only policy (its magnetic fields are zero)
     if (b.precision == QL_PREC_MIXED) {
-        if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+        if (b.hd) qlk::k_mode32<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
+        else if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
         else qlk::k_mode32<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
     } else {
-        if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+        if (b.hd) qlk::k_mode64<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
+        else if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
         else qlk::k_mode64<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
     }
 }

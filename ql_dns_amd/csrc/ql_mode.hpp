@@ -181,7 +181,7 @@ struct ModelFull {
 // them with b = 0 (the terms dropped are products with zero), and the
 // Reynolds stresses -Re(uz conj(ux)), -Re(uz conj(uy)) before their
 // z-derivative in the stress slots 2, 3 (the EMF slots stay zero).  Only the
-// (ux, zeta) pair is read, updated and written.
+// (ux, zeta) pair is read, updated and written.  Both mode kernels use it.
 struct ModelHD {
     static constexpr int NPAIR = 1;
     static constexpr int NB = 3;     // bases: ux uy uz
@@ -472,7 +472,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         const int i = mode_off + (l < 0 ? 0 : l);
         const size_t off = (size_t)i * 4 * NZD + im;
 #pragma unroll
-        for (int f = 0; f < 4; ++f) upre[f] = (l >= 0) ? a.u[off + f * NZD] : make_double2(0.0, 0.0);
+        for (int f = 0; f < 4; ++f)
+            upre[f] = (l >= 0 && f < 2 * P::NPAIR) ? a.u[off + f * NZD] : make_double2(0.0, 0.0);
         kpre[0] = (l >= 0) ? a.kx0[i] : 0.0;
         kpre[1] = (l >= 0) ? a.ky[i] : 0.0;
     };
@@ -552,7 +553,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         double2 np[4];
 #pragma unroll
         for (int f = 0; f < 4; ++f)
-            np[f] = (l >= 0 && a.n_prev) ? a.n_prev[off + f * NZD] : make_double2(0.0, 0.0);
+            np[f] = (l >= 0 && a.n_prev && f < 2 * P::NPAIR) ? a.n_prev[off + f * NZD] : make_double2(0.0, 0.0);
 
         // E: forward transforms of the product lines, dealiased band kept
         C *ct = buf + L::CT_OFF;  // [line][nz]
@@ -568,13 +569,15 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             if constexpr (!AHEAD) {
                 U0 = a.u[off];
                 U1 = a.u[off + NZD];
-                U2 = a.u[off + 2 * NZD];
-                U3 = a.u[off + 3 * NZD];
+                if constexpr (P::NPAIR == 2) {
+                    U2 = a.u[off + 2 * NZD];
+                    U3 = a.u[off + 3 * NZD];
+                }
             }
             bool hit = false;
             if (do_remap) hit = remap_hit(remap_qt, rp.remap_thr, kx0, ky);
 #pragma unroll
-            for (int pr = 0; pr < 2; ++pr) {
+            for (int pr = 0; pr < P::NPAIR; ++pr) {
                 double2 N0, N1;
                 P::rhs_pair(pr, T, MB * NZD, U0, U1, U2, U3, kx, ky, kz, ilapF, ph, N0, N1);
                 const double Ln = P::lin(pr, kx + qdtl * ky, ky, kz2, ph);

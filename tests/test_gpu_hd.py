@@ -71,6 +71,48 @@ def test_steps_match_oracle(integrator, NX, NY, NZ):
     _compare(o, g, to)
 
 
+@pytest.mark.parametrize("integrator,line", [(1, 256), (0, 256), (1, 128)])
+def test_workgroup_shapes_match_oracle(integrator, line):
+    """The velocity-only policy in the workgroup kernel at NZ = 128, where the
+    default shape is the wavefront kernel."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import MODEL_HD_FULLU, QLParams
+    from oracle.oracle import Oracle
+    p = QLParams(model=MODEL_HD_FULLU, integrator=integrator, NX=6, NY=4, NZ=128)
+    lin, mf = initial_state(p)
+    o = Oracle.from_params(p, nthreads=8)
+    g = QLContext(p)
+    g.set_config(128, line, 5)
+    assert g.config()[0]["line"] == line
+    for x in (o, g):
+        x.set_state(lin, mf)
+        x.linear_ops_init(0.0)
+    to, tg = o.steps(30, 0.0), g.steps(30, 0.0)
+    assert tg == to
+    _compare(o, g, to)
+
+
+def test_mixed_path_close_to_fp64():
+    """NZ = 256 in the mixed path runs the workgroup kernel's velocity-only
+    policy in FP32 transforms; bar 1e-6 relative of the FP64 run."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import MODEL_HD_FULLU, PREC_MIXED, QLParams
+    p = QLParams(model=MODEL_HD_FULLU, NX=8, NY=8, NZ=256, dt=0.005, nu=1 / 8000)
+    lin, mf = initial_state(p)
+    outs = []
+    for q in (p, p.replace(precision=PREC_MIXED)):
+        g = QLContext(q)
+        g.set_state(lin, mf)
+        g.linear_ops_init(0.0)
+        t = g.steps(50, 0.0)
+        outs.append((t, g.get_state(), g.drive()))
+    (ta, sa, da), (tb, sb, db) = outs
+    assert ta == tb and np.array_equal(sa[2], sb[2])
+    assert rel_err(sb[1], sa[1]) < 1e-6 and rel_err(sb[0], sa[0]) < 1e-6 and rel_err(db, da) < 1e-6
+
+
 def test_stress_profiles_match_oracle():
     from oracle.oracle import fft_inverse
     p, o, g = _pair()
