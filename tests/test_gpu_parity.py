@@ -139,18 +139,24 @@ def test_single_step_and_remap_entry_points():
     assert np.sum(np.all(lg == 0, axis=(1, 2))) > 1
 
 
-def test_full_size_two_steps():
-    """64x128x128 (4032 modes): two steps against the oracle."""
+FULL = ["box64x128x128", "box128x128x256"]
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_full_size_two_steps(name):
+    """64x128x128 (4032 modes) and 128x128x256 (8128 modes): two steps
+    against the oracle."""
     from ql_dns_amd.params import workload
-    p = workload("box64x128x128")
+    p = workload(name)
     o, g = _pair(p, nthreads=16)
     to = o.steps(2, 0.0)
     tg = g.steps(2, 0.0)
     _compare_state(o, g, to, tg)
 
 
-def test_full_size_properties_and_determinism():
-    """64x128x128 over 200 steps: size-independent properties.
+@pytest.mark.parametrize("name", FULL)
+def test_full_size_properties_and_determinism(name):
+    """The full workloads over 200 steps: size-independent properties.
 
     * bitwise determinism of the fixed-order reduction (two fresh runs);
     * the ky = 0 plane stays Hermitian: f(-kx, 0, -kz) = conj f(kx, 0, kz);
@@ -160,7 +166,7 @@ def test_full_size_properties_and_determinism():
     """
     from ql_dns_amd.ic import initial_state, mode_tables
     from ql_dns_amd.params import workload
-    p = workload("box64x128x128")
+    p = workload(name)
     lin, mf = initial_state(p)
     runs = []
     for _ in range(2):
