@@ -1,8 +1,10 @@
 // ql_mode.hpp -- the fused per-(kx,ky) kernel of one RK3 substep (FP64).
 //
-// One kernel skeleton, two model policies:
+// One kernel skeleton, three model policies:
 //   ModelFull  MHD_FullQlin (Models/MHD_FullQlin.cpp:171-427): fluctuations
 //              coupled to Bx, By, Ux, Uy(z); EMF and both stresses reduced.
+//   ModelHD    HD_fullU (Models/HD_fullU.cpp:145-309): ModelFull with the
+//              magnetic fields at zero; only (ux, zeta) read and written.
 //   ModelBQ    MHD_BQlin (Models/MHD_BQlin.cpp:117-284): fluctuations see
 //              By(z), dBy/dz, d2By/dz2 only; the two EMF components reduced.
 //
