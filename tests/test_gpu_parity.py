@@ -120,6 +120,22 @@ def test_steps_physics_variants(case, model):
     _compare_state(o, g, to, tg)
 
 
+@pytest.mark.parametrize("model,integrator", [(0, 1), (0, 0), (1, 1), (2, 1)])
+@pytest.mark.parametrize("NZ", [32, 256])
+def test_single_zero_mode_box(model, integrator, NZ):
+    """NX = NY = 2: the only mode is (kx, ky) = (0, 0), which stays zero, so
+    the drive is zero and the mean fields evolve under their linear terms
+    alone (one workgroup, one mode)."""
+    p = _params(model=model, integrator=integrator, NX=2, NY=2, NZ=NZ)
+    o, g = _pair(p)
+    assert g.nxy == 1
+    to, tg = o.steps(30, 0.0), g.steps(30, 0.0)
+    _compare_state(o, g, to, tg)
+    lg, mg, _ = g.get_state()
+    assert np.all(lg == 0) and np.all(g.drive() == 0)
+    assert np.abs(mg).max() > 0
+
+
 def test_single_step_and_remap_entry_points():
     from ql_dns_amd.params import workload
     p = workload("box16x16x32")
