@@ -379,16 +379,17 @@ void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
 template <int NZ, int MODE>
 void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
-    constexpr int T = qlk::mode_waves(NZ) * qlk::WAVE;
+    constexpr int TF = qlk::mode_waves<qlk::ModelFull>(NZ) * qlk::WAVE;
+    constexpr int TH = qlk::mode_waves<qlk::ModelHD>(NZ) * qlk::WAVE;
     // HD_fullU: the velocity-only policy (its magnetic fields are zero)
     if (b.precision == QL_PREC_MIXED) {
         if constexpr (NZ <= 128) {  // wave_ok: no FP32 wavefront kernel at NZ = 256
-            if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
-            else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+            if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
+            else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
         }
     } else {
-        if (b.hd) qlk::k_wave64<NZ, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
-        else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+        if (b.hd) qlk::k_wave64<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
+        else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
     }
 }
 

@@ -375,19 +375,22 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
 // workgroups 31.1 us, 4 and 504 31.9 us, 2 and 1008 31.6 us and twice the
 // partial rows for the column reduction), 4 at NZ = 256 (26 KB per wavefront
 // in FP64, one wavefront per SIMD with the whole register file)
-constexpr int mode_waves(int NZ) { return NZ >= 256 ? 4 : 8; }
-constexpr int wave_occ(int NZ) { return NZ >= 256 ? 1 : 2; }
+// (the velocity-only policy, with 3 lines per transform, keeps 8 at NZ = 256)
+template <class P>
+constexpr int mode_waves(int NZ) { return NZ >= 256 && P::NPAIR == 2 ? 4 : 8; }
+template <class P>
+constexpr int wave_occ(int NZ) { return NZ >= 256 && P::NPAIR == 2 ? 1 : 2; }
 template <int NZ, class P, int MODE>
-__global__ __launch_bounds__(mode_waves(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ(NZ))))
+__global__ __launch_bounds__(mode_waves<P>(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ<P>(NZ))))
 void k_wave64(ModeArgs a)
 {
-    mode_wave_body<NZ, P, double2, MODE, mode_waves(NZ)>(a);
+    mode_wave_body<NZ, P, double2, MODE, mode_waves<P>(NZ)>(a);
 }
 template <int NZ, class P, int MODE>
-__global__ __launch_bounds__(mode_waves(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ(NZ))))
+__global__ __launch_bounds__(mode_waves<P>(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ<P>(NZ))))
 void k_wave32(ModeArgs a)
 {
-    mode_wave_body<NZ, P, float2, MODE, mode_waves(NZ)>(a);
+    mode_wave_body<NZ, P, float2, MODE, mode_waves<P>(NZ)>(a);
 }
 
 }  // namespace qlk
