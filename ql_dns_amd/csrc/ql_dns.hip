@@ -189,9 +189,10 @@ qlk::RunParams run_params(const RunInfo &r)
 // near-equal contiguous ranges
 int configure(Batch &b, Group &g, int line, int target)
 {
-    if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision))
+    if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision, b.hd))
         return fail(QL_ERR_ARG,
-                    "line %d (one wavefront per mode) needs MHD_FullQlin or HD_fullU, and FP64 at NZ = 256",
+                    "line %d (one wavefront per mode) needs MHD_FullQlin or HD_fullU, and for MHD_FullQlin at "
+                    "NZ = 256 FP64",
                     LINE_WAVE);
     // the tables are rewritten below: nothing queued may still read them
     QL_RC(sync_all(b));
@@ -350,7 +351,7 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_HIP(hipMemcpy(g.kzv, r0.h_kz.data(), sizeof(double) * g.nz, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw, htw.data(), sizeof(double2) * g.NZ, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw32, htw32.data(), sizeof(float2) * g.NZ, hipMemcpyHostToDevice));
-        const int line = default_line(b.model, g.NZ, b.precision);
+        const int line = default_line(b.model, g.NZ, b.precision, b.hd);
         QL_RC(configure(b, g, line, default_target(line)));
     }
     return QL_OK;
@@ -383,10 +384,9 @@ void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
     constexpr int TH = qlk::mode_waves<qlk::ModelHD>(NZ) * qlk::WAVE;
     // HD_fullU: the velocity-only policy (its magnetic fields are zero)
     if (b.precision == QL_PREC_MIXED) {
-        if constexpr (NZ <= 128) {  // wave_ok: no FP32 wavefront kernel at NZ = 256
-            if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
-            else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
-        }
+        if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
+        else if constexpr (NZ <= 128)  // wave_ok: no FP32 MHD_FullQlin wavefront kernel at NZ = 256
+            qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
     } else {
         if (b.hd) qlk::k_wave64<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
         else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
@@ -924,7 +924,7 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
         int best_line = g.line, best_target = g.target;
         std::vector<std::pair<int, int>> seen;  // (kernel and MB, W) already timed
         for (int line : LINES) {
-            if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision)) continue;
+            if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision, b.hd)) continue;
             for (int target : TARGETS) {
                 if ((rc = configure(b, g, line, target))) break;
                 const int kind = line == LINE_WAVE ? -1 : g.MB;

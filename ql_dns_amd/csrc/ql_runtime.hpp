@@ -52,15 +52,19 @@ constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
 inline bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
 inline int mb_for(int NZ, int line) { return (line == LINE_WAVE || NZ >= line) ? 1 : line / NZ; }
 // where the wavefront kernel exists and is used: not for MHD_BQlin (model 1),
-// and at NZ = 256 only in FP64 (there it runs one wavefront per SIMD; the
-// FP32 version measured slower than the workgroup kernel: 109 vs 101 us at
+// and at NZ = 256 in FP32 only for HD_fullU's velocity-only policy (the
+// MHD_FullQlin kernel runs one wavefront per SIMD there in FP64; its FP32
+// version measured slower than the workgroup kernel: 109 vs 101 us at
 // 128x128x256)
-inline bool wave_ok(int model, int NZ, int precision) { return model != 1 && (NZ <= 128 || precision == 0); }
-// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128,
-// and NZ = 256 in FP64), with about one workgroup per CU
-inline int default_line(int model, int NZ, int precision)
+inline bool wave_ok(int model, int NZ, int precision, bool hd)
 {
-    return (NZ >= 128 && wave_ok(model, NZ, precision)) ? LINE_WAVE : DEFAULT_LINE;
+    return model != 1 && (NZ <= 128 || precision == 0 || hd);
+}
+// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128,
+// and NZ = 256 where it exists), with about one workgroup per CU
+inline int default_line(int model, int NZ, int precision, bool hd)
+{
+    return (NZ >= 128 && wave_ok(model, NZ, precision, hd)) ? LINE_WAVE : DEFAULT_LINE;
 }
 inline int default_target(int line) { return line == LINE_WAVE ? 256 : DEFAULT_TARGET; }
 

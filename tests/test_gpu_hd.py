@@ -93,9 +93,11 @@ def test_workgroup_shapes_match_oracle(integrator, line):
     _compare(o, g, to)
 
 
-def test_mixed_path_close_to_fp64():
-    """NZ = 256 in the mixed path runs the workgroup kernel's velocity-only
-    policy in FP32 transforms; bar 1e-6 relative of the FP64 run."""
+@pytest.mark.parametrize("line", [64, 256])
+def test_mixed_path_close_to_fp64(line):
+    """NZ = 256 in the mixed path: the velocity-only policy in FP32 transforms,
+    in the wavefront kernel (the default shape there) and in the workgroup
+    kernel; bar 1e-6 relative of the FP64 run."""
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state
     from ql_dns_amd.params import MODEL_HD_FULLU, PREC_MIXED, QLParams
@@ -104,6 +106,9 @@ def test_mixed_path_close_to_fp64():
     outs = []
     for q in (p, p.replace(precision=PREC_MIXED)):
         g = QLContext(q)
+        if q.precision == PREC_MIXED:
+            assert g.config()[0]["line"] == 64
+            g.set_config(256, line, 256 if line == 64 else 512)
         g.set_state(lin, mf)
         g.linear_ops_init(0.0)
         t = g.steps(50, 0.0)

@@ -6,8 +6,8 @@ relative of the oracle (kx0 after remaps and fixed-step times exactly equal);
 the same run under line 64 and under (256, 512), which sum the stress
 partials in different orders, within 1e-12; one shape bitwise reproducible;
 the mixed path within 1e-6 of FP64.  HD_fullU runs its velocity-only policy
-(ModelHD) there.  MHD_BQlin has no wavefront kernel, nor has the FP32 path at
-NZ = 256; both refuse the shape.
+(ModelHD) there.  MHD_BQlin has no wavefront kernel, nor has MHD_FullQlin's
+FP32 path at NZ = 256; both refuse the shape.
 """
 import numpy as np
 import pytest
