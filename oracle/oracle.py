@@ -71,6 +71,7 @@ def lib(native: bool = False):
     L.orc_check_solution.argtypes = [P]
     L.orc_step.restype = ctypes.c_double
     L.orc_step.argtypes = [P, ctypes.c_double]
+    L.orc_set_dt.argtypes = [P, ctypes.c_double]
     L.orc_steps.restype = ctypes.c_double
     L.orc_steps.argtypes = [P, ctypes.c_int, ctypes.c_double]
     L.orc_remap.argtypes = [P, ctypes.c_double]
@@ -190,6 +191,10 @@ class Oracle:
 
     def step(self, t):
         return self._lib.orc_step(self._h, t)
+
+    def set_dt(self, dt):
+        """The time step of the following steps: > 0 fixed, < 0 variable."""
+        self._lib.orc_set_dt(self._h, dt)
 
     def set_noise(self, f_noise, range_low=0.0, range_high=1e16, drive_only_velocity=0, drive_only_magnetic=0,
                   seed=0x5EED0004):

@@ -1024,6 +1024,10 @@ static double step_dt(orc *o)
     return dt;
 }
 
+/* the time step of the following steps: dt > 0 fixed, dt < 0 variable of at
+ * most -dt (a step of a chosen size, e.g. to replay another run's steps) */
+void orc_set_dt(orc *o, double dt) { o->p.dt = dt; }
+
 void orc_check_sums(const orc *o, double *out)
 {
     const int nz = o->nz, NZ = o->NZ;

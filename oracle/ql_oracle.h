@@ -102,6 +102,8 @@ void orc_noise_normals(const orc *o, double t, int i, int jj, double *z8);
 void orc_philox4x64(const uint64_t *ctr, const uint64_t *key, uint64_t *out);
 /* Integrator::Step(t, sol); returns dt */
 double orc_step(orc *o, double t);
+/* the time step of the following steps (orc_params.dt convention) */
+void orc_set_dt(orc *o, double dt);
 /* main.cpp:108-123 without output: Step, DrivingNoise (if f_noise != 0),
  * t += dt, remap(q t); returns t */
 double orc_steps(orc *o, int nsteps, double t);

@@ -78,6 +78,45 @@ def test_steps_with_noise_match_oracle(model, integrator):
     assert rel_err(lg, lo) < 1e-10
 
 
+@pytest.mark.parametrize("model,integrator,NZ", [(0, 1, 32), (0, 0, 128), (2, 1, 64), (1, 1, 256), (2, 0, 256)])
+def test_variable_steps_with_noise_match_oracle(model, integrator, NZ):
+    """Driven variable steps.  The noise is keyed on the bits of the step's
+    start time, and a variable step size agrees with the oracle's only to
+    rounding (1e-12), so after a step whose size differs in the last bit the
+    two draw different noise.  The oracle therefore replays the GPU's step
+    sizes (each checked against the oracle's own CFL step to 1e-12); states
+    then agree to 1e-10 at every step, and the GPU's replayed graph equals
+    its step loop bit for bit."""
+    p = _params(model=model, integrator=integrator, NX=8, NY=8, NZ=NZ, dt=-0.05, CFL=0.5)
+    o, g, lin0 = _pair(p, dict(f_noise=0.5, seed=41))
+    mf0 = o.get_state()[1]
+    t = 0.0
+    for n in range(12):
+        v = o.calculate_cfl()
+        dt = g.step(t)
+        assert abs(dt - min(0.5 / v, 0.05)) <= 1e-12 * dt
+        o.set_dt(dt)
+        assert o.step(t) == dt
+        for x in (o, g):
+            x.driving_noise(t, dt)
+        t += dt
+        for x in (o, g):
+            x.remap(p.q * t)
+        lo, mo, ko = o.get_state()
+        lg, mg, kg = g.get_state()
+        assert np.array_equal(kg, ko)
+        assert rel_err(mg, mo) < 1e-10 and rel_err(lg, lo) < 1e-10, n
+    # the same 12 driven steps from the replayed graph
+    from ql_dns_amd.capi import QLContext
+    h = QLContext(p)
+    h.set_state(lin0, mf0)
+    h.linear_ops_init(0.0)
+    h.set_noise(f_noise=0.5, seed=41)
+    assert h.steps(12, 0.0) == t
+    for a, b in zip(h.get_state(), g.get_state()):
+        assert np.array_equal(a, b)
+
+
 def test_noise_graph_replay_equals_eager_and_step_loop():
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state
