@@ -38,8 +38,11 @@ METRIC = "QL RK3-CN steps/s at 64x128x128; mixed/batched vs FP64 speedup at EMF 
 HBM_PEAK_GBS = 8000.0
 
 
-def traffic_from_profiles(workload: str):
-    """HBM bytes per mode-kernel launch from the committed rocprofv3 PMC summary."""
+def traffic_from_profiles(workload: str, equations: str = "MHD_FullQlin", precision: str = "fp64"):
+    """HBM bytes per mode-kernel launch from the committed rocprofv3 PMC summary
+    (measured for MHD_FullQlin in FP64; None for other models and precisions)."""
+    if equations != "MHD_FullQlin" or precision != "fp64":
+        return None
     path = os.path.join(ROOT, "profiles", f"k_mode_full_{workload}_traffic.txt")
     if not os.path.exists(path):
         return None
@@ -243,6 +246,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="box64x128x128")
     ap.add_argument("--integrator", choices=["RK3CN_Elin", "RK3CN"], default="RK3CN_Elin")
+    ap.add_argument("--equations", choices=["MHD_FullQlin", "HD_fullU"], default="MHD_FullQlin",
+                    help="model of the timed run (HD_fullU runs the velocity-only policy)")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-run ensemble section")
@@ -268,10 +273,12 @@ def main():
 
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state
-    from ql_dns_amd.params import INTEGRATOR_NAMES, PREC_FP64, PREC_MIXED, workload
+    from ql_dns_amd.params import INTEGRATOR_NAMES, MODEL_HD_FULLU, MODEL_MHD_FULLQLIN, PREC_FP64, PREC_MIXED, workload
 
+    hd = args.equations == "HD_fullU"
     p = workload(args.workload, integrator=INTEGRATOR_NAMES[args.integrator],
-                 precision=PREC_MIXED if args.precision == "mixed" else PREC_FP64)
+                 precision=PREC_MIXED if args.precision == "mixed" else PREC_FP64,
+                 model=MODEL_HD_FULLU if hd else MODEL_MHD_FULLQLIN)
     lin, mf = initial_state(p)
     ctx = QLContext(p)
     ctx.set_state(lin, mf)
@@ -291,7 +298,7 @@ def main():
     S = p.state_bytes
     bytes_per_launch = 10.0 * S / 3.0  # u read+write, RHS write (s1), RHS read+write (s2), RHS read (s3)
     achieved = bytes_per_launch / (avg_ms[0] * 1e-3) / 1e9
-    traffic = traffic_from_profiles(args.workload)
+    traffic = traffic_from_profiles(args.workload, args.equations, args.precision)
     shape = ctx.config()[0]
     lin2, mf2, _ = ctx.get_state()
     finite = bool(abs(lin2).max() < 1e300 and abs(mf2).max() < 1e300)
@@ -299,8 +306,10 @@ def main():
 
     # the mode kernel the tuned shape launches (line 64: one wavefront per mode)
     elin = 1 if args.integrator == "RK3CN_Elin" else 2
-    kname = (f"k_wave64<{p.NZ},ModelFull,{elin}>" if shape.get("line") == 64
-             else f"k_mode64<{p.NZ},{shape.get('modes_per_pass')},ModelFull,{elin}>")
+    bits = 64 if args.precision == "fp64" else 32
+    pol = "ModelHD" if hd else "ModelFull"
+    kname = (f"k_wave{bits}<{p.NZ},{pol},{elin}>" if shape.get("line") == 64
+             else f"k_mode{bits}<{p.NZ},{shape.get('modes_per_pass')},{pol},{elin}>")
     out = {
         "metric": METRIC,
         "value": steps_per_s,
@@ -316,7 +325,7 @@ def main():
         "data": "synthetic: seeded divergence-free fluctuations and 4-harmonic mean fields",
         "config": {"workload": args.workload, "NX": p.NX, "NY": p.NY, "NZ": p.NZ,
                    "modes": p.nxy, "nz_dealiased": p.nz, "Re": 1.0 / p.nu, "Rm": 1.0 / p.eta,
-                   "dt": p.dt, "equations": "MHD_FullQlin", "integrator": args.integrator,
+                   "dt": p.dt, "equations": args.equations, "integrator": args.integrator,
                    "remap": bool(p.remap), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
