@@ -246,7 +246,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="box64x128x128")
     ap.add_argument("--integrator", choices=["RK3CN_Elin", "RK3CN"], default="RK3CN_Elin")
-    ap.add_argument("--equations", choices=["MHD_FullQlin", "HD_fullU"], default="MHD_FullQlin",
+    ap.add_argument("--equations", choices=["MHD_FullQlin", "MHD_BQlin", "HD_fullU"], default="MHD_FullQlin",
                     help="model of the timed run (HD_fullU runs the velocity-only policy)")
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -273,12 +273,11 @@ def main():
 
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state
-    from ql_dns_amd.params import INTEGRATOR_NAMES, MODEL_HD_FULLU, MODEL_MHD_FULLQLIN, PREC_FP64, PREC_MIXED, workload
+    from ql_dns_amd.params import INTEGRATOR_NAMES, MODEL_NAMES, PREC_FP64, PREC_MIXED, workload
 
-    hd = args.equations == "HD_fullU"
     p = workload(args.workload, integrator=INTEGRATOR_NAMES[args.integrator],
                  precision=PREC_MIXED if args.precision == "mixed" else PREC_FP64,
-                 model=MODEL_HD_FULLU if hd else MODEL_MHD_FULLQLIN)
+                 model=MODEL_NAMES[args.equations])
     lin, mf = initial_state(p)
     ctx = QLContext(p)
     ctx.set_state(lin, mf)
@@ -307,7 +306,7 @@ def main():
     # the mode kernel the tuned shape launches (line 64: one wavefront per mode)
     elin = 1 if args.integrator == "RK3CN_Elin" else 2
     bits = 64 if args.precision == "fp64" else 32
-    pol = "ModelHD" if hd else "ModelFull"
+    pol = {"MHD_FullQlin": "ModelFull", "MHD_BQlin": "ModelBQ", "HD_fullU": "ModelHD"}[args.equations]
     kname = (f"k_wave{bits}<{p.NZ},{pol},{elin}>" if shape.get("line") == 64
              else f"k_mode{bits}<{p.NZ},{shape.get('modes_per_pass')},{pol},{elin}>")
     out = {
