@@ -223,7 +223,7 @@ int ql_batch_tune(ql_batch *b, int trial_steps, double *step_ms);
 int ql_batch_get_config(const ql_batch *b, int max_groups, int *ngroups, int *cfg);
 /* sets the shape of the NZ group: line in {64, 128, 256}, target >= 1.
  * line 64 selects the kernel with one wavefront per mode (modes per pass 1;
- * MHD_FullQlin and HD_fullU; at NZ = 256 MHD_FullQlin in FP64 only, else QL_ERR_ARG) */
+ * every model; at NZ = 256 FP64 only except for HD_fullU, else QL_ERR_ARG) */
 int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
 /* ------------------------------------------------------------------------

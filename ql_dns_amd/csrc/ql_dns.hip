@@ -190,10 +190,7 @@ qlk::RunParams run_params(const RunInfo &r)
 int configure(Batch &b, Group &g, int line, int target)
 {
     if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision, b.hd))
-        return fail(QL_ERR_ARG,
-                    "line %d (one wavefront per mode) needs MHD_FullQlin or HD_fullU, and for MHD_FullQlin at "
-                    "NZ = 256 FP64",
-                    LINE_WAVE);
+        return fail(QL_ERR_ARG, "line %d (one wavefront per mode) at NZ = 256 needs FP64 (or HD_fullU)", LINE_WAVE);
     // the tables are rewritten below: nothing queued may still read them
     QL_RC(sync_all(b));
     const int MB = mb_for(g.NZ, line);
@@ -382,13 +379,18 @@ void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
 {
     constexpr int TF = qlk::mode_waves<qlk::ModelFull>(NZ) * qlk::WAVE;
     constexpr int TH = qlk::mode_waves<qlk::ModelHD>(NZ) * qlk::WAVE;
+    constexpr int TB = qlk::mode_waves<qlk::ModelBQ>(NZ) * qlk::WAVE;
+    const bool bq = b.model == QL_MODEL_MHD_BQLIN;
     // HD_fullU: the velocity-only policy (its magnetic fields are zero)
     if (b.precision == QL_PREC_MIXED) {
         if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
-        else if constexpr (NZ <= 128)  // wave_ok: no FP32 MHD_FullQlin wavefront kernel at NZ = 256
-            qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
+        else if constexpr (NZ <= 128) {  // wave_ok: no other FP32 wavefront kernel at NZ = 256
+            if (bq) qlk::k_wave32<NZ, qlk::ModelBQ, MODE><<<g.W, TB, 0, st>>>(a);
+            else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
+        }
     } else {
         if (b.hd) qlk::k_wave64<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
+        else if (bq) qlk::k_wave64<NZ, qlk::ModelBQ, MODE><<<g.W, TB, 0, st>>>(a);
         else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
     }
 }

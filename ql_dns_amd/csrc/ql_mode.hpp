@@ -249,15 +249,18 @@ struct ModelHD {
     }
 };
 
+// The reference transforms dz uz as a line of its own (MHD_BQlin.cpp:195);
+// the reconstruction is divergence free slot by slot (as for ModelFull), so
+// on the z grid dz uz = -i (kx ux + ky uy) from the ux and uy lines.
 struct ModelBQ {
     static constexpr int NPAIR = 2;
     static constexpr int NB = 8;    // bases: ux zeta bx eta uy uz by bz
-    static constexpr int NLINE = 9; // + dz uz
+    static constexpr int NLINE = 8; // the bases
     static constexpr int NOUT = 5;  // u (two parts), b, zeta, eta
     static constexpr int NMFZ = 3;  // By dzBy dzdzBy
     static constexpr int NACC = 2;  // EMF x, EMF y
-    __device__ static int base_of(int l) { return l < 8 ? l : 5; }
-    __device__ static bool dz_of(int l) { return l == 8; }
+    __device__ static int base_of(int l) { return l; }
+    __device__ static bool dz_of(int) { return false; }
     template <class C>
     __device__ static void bases(double2 ux, double2 ze, double2 bx, double2 et, double kx, double ky, double kz,
                                  double ilap2, double fac, C *out, int stride)
@@ -284,7 +287,8 @@ struct ModelBQ {
     {
         typedef RealOf<C> R;
         const C ux = X[0], ze = X[ld], bx = X[2 * ld], et = X[3 * ld];
-        const C uy = X[4 * ld], uz = X[5 * ld], by = X[6 * ld], bz = X[7 * ld], dzuz = X[8 * ld];
+        const C uy = X[4 * ld], uz = X[5 * ld], by = X[6 * ld], bz = X[7 * ld];
+        const C dzuz = cadd(imul(-kx, ux), imul(-ky, uy));
         const R By = mf[0], dzBy = mf[1], dzdzBy = mf[2];
         if (part == 0) {
             prod[0] = cscale(By, imul(ky, bx));                 // u, first part

@@ -1,6 +1,7 @@
-// ql_mode_wave.hpp -- the per-(kx,ky) substep kernel of MHD_FullQlin (and of
-// HD_fullU through its velocity-only policy, ModelHD) with one wavefront per
-// mode (Models/MHD_FullQlin.cpp:171-427, Models/HD_fullU.cpp:145-309 and the
+// ql_mode_wave.hpp -- the per-(kx,ky) substep kernel of MHD_FullQlin, MHD_BQlin
+// and HD_fullU (through its velocity-only policy, ModelHD) with one wavefront
+// per mode (Models/MHD_FullQlin.cpp:171-427, Models/MHD_BQlin.cpp:117-284,
+// Models/HD_fullU.cpp:145-309 and the
 // substep updates of Integrators/RK3CN_Elin.cpp:290-378,
 // Integrators/RK3CN.cpp:92-193).
 //
@@ -369,7 +370,7 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     }
 }
 
-// MHD_FullQlin (ModelFull) and HD_fullU (ModelHD); workgroups of
+// MHD_FullQlin (ModelFull), MHD_BQlin (ModelBQ) and HD_fullU (ModelHD); workgroups of
 // mode_waves(NZ) wavefronts: 8 at NZ <= 128 (one workgroup per CU at NZ = 128
 // in FP64: 115 KB of LDS; measured at 64x128x128: 8 wavefronts and 252
 // workgroups 31.1 us, 4 and 504 31.9 us, 2 and 1008 31.6 us and twice the

@@ -51,15 +51,12 @@ constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
 
 inline bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
 inline int mb_for(int NZ, int line) { return (line == LINE_WAVE || NZ >= line) ? 1 : line / NZ; }
-// where the wavefront kernel exists and is used: not for MHD_BQlin (model 1),
-// and at NZ = 256 in FP32 only for HD_fullU's velocity-only policy (the
-// MHD_FullQlin kernel runs one wavefront per SIMD there in FP64; its FP32
-// version measured slower than the workgroup kernel: 109 vs 101 us at
-// 128x128x256)
-inline bool wave_ok(int model, int NZ, int precision, bool hd)
-{
-    return model != 1 && (NZ <= 128 || precision == 0 || hd);
-}
+// where the wavefront kernel exists and is used: every model, at NZ = 256 in
+// FP32 only for HD_fullU's velocity-only policy (the MHD_FullQlin and
+// MHD_BQlin kernels run one wavefront per SIMD there in FP64; the FP32
+// MHD_FullQlin version measured slower than the workgroup kernel: 109 vs
+// 101 us at 128x128x256)
+inline bool wave_ok(int, int NZ, int precision, bool hd) { return NZ <= 128 || precision == 0 || hd; }
 // shape before tuning: the wavefront kernel where it measured fastest (NZ = 128,
 // and NZ = 256 where it exists), with about one workgroup per CU
 inline int default_line(int model, int NZ, int precision, bool hd)

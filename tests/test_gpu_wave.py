@@ -6,8 +6,8 @@ relative of the oracle (kx0 after remaps and fixed-step times exactly equal);
 the same run under line 64 and under (256, 512), which sum the stress
 partials in different orders, within 1e-12; one shape bitwise reproducible;
 the mixed path within 1e-6 of FP64.  HD_fullU runs its velocity-only policy
-(ModelHD) there.  MHD_BQlin has no wavefront kernel, nor has MHD_FullQlin's
-FP32 path at NZ = 256; both refuse the shape.
+(ModelHD) there, MHD_BQlin its own (ModelBQ).  MHD_FullQlin's FP32 path has
+no wavefront kernel at NZ = 256 and refuses the shape.
 """
 import numpy as np
 import pytest
@@ -144,15 +144,33 @@ def test_run_set_matches_oracle_per_run():
         assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL
 
 
-def test_bqlin_and_fp32_at_256_refuse_the_wave_shape():
+def test_fp32_full_at_256_refuses_the_wave_shape():
     from ql_dns_amd.capi import QLContext, QLError
     from ql_dns_amd.params import PREC_MIXED
-    for p, NZ in ((_params(model=1), 32), (_params(NX=4, NY=4, NZ=256, precision=PREC_MIXED), 256)):
-        c = QLContext(p)
-        with pytest.raises(QLError) as e:
-            c.set_config(NZ, WAVE, 512)
-        assert e.value.code == -1
-        c.set_config(NZ, 128 if NZ == 32 else 256, 512)
+    c = QLContext(_params(NX=4, NY=4, NZ=256, precision=PREC_MIXED))
+    with pytest.raises(QLError) as e:
+        c.set_config(256, WAVE, 512)
+    assert e.value.code == -1
+    c.set_config(256, 256, 512)
+
+
+@pytest.mark.parametrize("integrator", [1, 0])
+@pytest.mark.parametrize("NX,NY,NZ,target", [(16, 16, 32, 512), (8, 8, 64, 7), (6, 6, 128, 3), (6, 4, 256, 5)])
+def test_bqlin_steps_match_oracle(integrator, NX, NY, NZ, target):
+    """MHD_BQlin in the wavefront kernel (its dz uz from the ux and uy lines)."""
+    p = _params(model=1, integrator=integrator, NX=NX, NY=NY, NZ=NZ)
+    o, g = _pair(p, target)
+    assert g.config()[0]["line"] == WAVE
+    to, tg = o.steps(40, 0.0), g.steps(40, 0.0)
+    assert tg == to
+    lo, mo, ko = o.get_state()
+    lg, mg, kg = g.get_state()
+    assert np.array_equal(kg, ko)
+    assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL
+    assert rel_err(g.drive(), o.drive()) < TOL
+    ro, rg = o.rhs(0.37, 0.011), g.rhs(0.37, 0.011)
+    for a, b in zip(rg, ro):
+        assert rel_err(a, b) < TOL
 
 
 def test_noise_and_remap_match_oracle():
