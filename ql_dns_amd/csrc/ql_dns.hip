@@ -189,7 +189,7 @@ qlk::RunParams run_params(const RunInfo &r)
 // near-equal contiguous ranges
 int configure(Batch &b, Group &g, int line, int target)
 {
-    if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision, b.hd))
+    if (line == LINE_WAVE && !wave_ok(g.NZ, b.precision, b.hd))
         return fail(QL_ERR_ARG, "line %d (one wavefront per mode) at NZ = 256 needs FP64 (or HD_fullU)", LINE_WAVE);
     // the tables are rewritten below: nothing queued may still read them
     QL_RC(sync_all(b));
@@ -348,7 +348,7 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_HIP(hipMemcpy(g.kzv, r0.h_kz.data(), sizeof(double) * g.nz, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw, htw.data(), sizeof(double2) * g.NZ, hipMemcpyHostToDevice));
         QL_HIP(hipMemcpy(g.tw32, htw32.data(), sizeof(float2) * g.NZ, hipMemcpyHostToDevice));
-        const int line = default_line(b.model, g.NZ, b.precision, b.hd);
+        const int line = default_line(g.NZ, b.precision, b.hd);
         QL_RC(configure(b, g, line, default_target(line)));
     }
     return QL_OK;
@@ -926,7 +926,7 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
         int best_line = g.line, best_target = g.target;
         std::vector<std::pair<int, int>> seen;  // (kernel and MB, W) already timed
         for (int line : LINES) {
-            if (line == LINE_WAVE && !wave_ok(b.model, g.NZ, b.precision, b.hd)) continue;
+            if (line == LINE_WAVE && !wave_ok(g.NZ, b.precision, b.hd)) continue;
             for (int target : TARGETS) {
                 if ((rc = configure(b, g, line, target))) break;
                 const int kind = line == LINE_WAVE ? -1 : g.MB;

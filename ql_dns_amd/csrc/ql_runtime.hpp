@@ -56,12 +56,12 @@ inline int mb_for(int NZ, int line) { return (line == LINE_WAVE || NZ >= line) ?
 // MHD_BQlin kernels run one wavefront per SIMD there in FP64; the FP32
 // MHD_FullQlin version measured slower than the workgroup kernel: 109 vs
 // 101 us at 128x128x256)
-inline bool wave_ok(int, int NZ, int precision, bool hd) { return NZ <= 128 || precision == 0 || hd; }
+inline bool wave_ok(int NZ, int precision, bool hd) { return NZ <= 128 || precision == 0 || hd; }
 // shape before tuning: the wavefront kernel where it measured fastest (NZ = 128,
 // and NZ = 256 where it exists), with about one workgroup per CU
-inline int default_line(int model, int NZ, int precision, bool hd)
+inline int default_line(int NZ, int precision, bool hd)
 {
-    return (NZ >= 128 && wave_ok(model, NZ, precision, hd)) ? LINE_WAVE : DEFAULT_LINE;
+    return (NZ >= 128 && wave_ok(NZ, precision, hd)) ? LINE_WAVE : DEFAULT_LINE;
 }
 inline int default_target(int line) { return line == LINE_WAVE ? 256 : DEFAULT_TARGET; }
 
