@@ -48,7 +48,16 @@ def lib(native: bool = False):
         return _LIB
     if not os.path.exists(path):
         build(native)
-    L = ctypes.CDLL(path)
+    L = bind(ctypes.CDLL(path))
+    if not native:
+        _LIB = L
+    return L
+
+
+def bind(L):
+    """Declares the argument and result types of the oracle's C entry points
+    on a loaded library (the default build, the timing build, or a variant
+    compiled by a test)."""
     P, D, I = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
     L.orc_create.restype = P
     L.orc_create.argtypes = [ctypes.POINTER(_Params)]
@@ -81,8 +90,6 @@ def lib(native: bool = False):
     L.orc_fft_inverse.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
     L.orc_fft_forward.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
     L.orc_fft_plain.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
-    if not native:
-        _LIB = L
     return L
 
 
@@ -124,8 +131,8 @@ def fft_plain(x: np.ndarray, sign: int) -> np.ndarray:
 class Oracle:
     def __init__(self, model=0, integrator=1, NX=16, NY=16, NZ=32, L=(1.0, 4.0, 1.0), q=1.5,
                  omega=1.0, nu=5e-4, eta=5e-4, B0z=0.0, viscosity_order=2, quasilinear=1,
-                 remap=1, dt=0.01, CFL=1.0, nthreads=1, native=False):
-        self._lib = lib(native)
+                 remap=1, dt=0.01, CFL=1.0, nthreads=1, native=False, library=None):
+        self._lib = library if library is not None else lib(native)
         p = _Params(model, integrator, NX, NY, NZ, (ctypes.c_double * 3)(*L), q, omega, nu, eta,
                     B0z, viscosity_order, quasilinear, remap, dt, nthreads, CFL)
         self._h = self._lib.orc_create(ctypes.byref(p))
@@ -134,8 +141,8 @@ class Oracle:
         self.nxy, self.nz, self.nMF, self.nLin, self.half, self.NZ = list(d)
 
     @classmethod
-    def from_params(cls, p, nthreads=1, native=False):
-        return cls(nthreads=nthreads, native=native, **p.oracle_dict())
+    def from_params(cls, p, nthreads=1, native=False, library=None):
+        return cls(nthreads=nthreads, native=native, library=library, **p.oracle_dict())
 
     def close(self):
         if self._h:
