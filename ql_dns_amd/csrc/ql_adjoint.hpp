@@ -161,7 +161,8 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
             const int part = rem / NZ, z = rem % NZ;
             double2 prod[3];
             double racc[2];
-            P::products(part, bufX + mb * P::NLINE * NZ + z, NZ, smfz + z * 8, skx[mb], sky[mb], prod, racc);
+            P::products(part, bufX + mb * P::NLINE * NZ + z, NZ, smfz + z * 8, skx[mb], sky[mb],
+                        make_double2(0.0, 0.0), prod, racc);
 #pragma unroll
             for (int c = 0; c < 3; ++c) bufT[(mb * P::NOUT + part * 3 + c) * NZ + z] = prod[c];
         }

@@ -680,23 +680,21 @@ int capture(Batch &b, int nsteps, hipGraphExec_t *out)
 }
 
 // nsteps of the driver loop with remap as each run asks; graph replay unless
-// per-kernel timing is on
+// per-kernel timing is on.  Both replay graphs are captured by the first call
+// that steps, whatever its step count, so that later calls (a timed loop after
+// a warm-up) only launch.
 int run_steps(Batch &b, int nsteps, double *t_inout)
 {
     double t = *t_inout;
     QL_RC(set_clock(b, t));
     if (b.timing) {
         for (int n = 0; n < nsteps; ++n) QL_RC(enqueue_step(b, true));
-    } else {
+    } else if (nsteps > 0) {
+        if (!b.gK) QL_RC(capture(b, GRAPH_STEPS, &b.gK));
+        if (!b.g1) QL_RC(capture(b, 1, &b.g1));
         int left = nsteps;
-        if (left >= GRAPH_STEPS) {
-            if (!b.gK) QL_RC(capture(b, GRAPH_STEPS, &b.gK));
-            for (; left >= GRAPH_STEPS; left -= GRAPH_STEPS) QL_HIP(hipGraphLaunch(b.gK, b.s));
-        }
-        if (left > 0) {
-            if (!b.g1) QL_RC(capture(b, 1, &b.g1));
-            for (; left > 0; --left) QL_HIP(hipGraphLaunch(b.g1, b.s));
-        }
+        for (; left >= GRAPH_STEPS; left -= GRAPH_STEPS) QL_HIP(hipGraphLaunch(b.gK, b.s));
+        for (; left > 0; --left) QL_HIP(hipGraphLaunch(b.g1, b.s));
     }
     QL_HIP(hipStreamSynchronize(b.s));
     if (b.vdt) {
@@ -747,6 +745,36 @@ int check_run(const Batch &b, int run, const char *who)
     return QL_OK;
 }
 
+// MHD_FullQlin and HD_fullU form their stresses as z-derivatives, which holds
+// for a divergence-free state (see ModelFull in ql_mode.hpp).  The one slot
+// the (ux, zeta) representation leaves unconstrained, kz = 0 of a ky = 0 mode,
+// is then ux = bx = 0 (u parallel to k there is a compressive component).  A
+// state uploaded with more than rounding there is refused: the reference keeps
+// such a component frozen and would give different stresses.
+int check_solenoidal(const Batch &b, const RunInfo &r, const double2 *ul)
+{
+    if (b.model == QL_MODEL_MHD_BQLIN) return QL_OK;  // exact for any state
+    const int ny = r.p.NY / 2, nz = r.nz;
+    for (int pr = 0; pr < 2; ++pr) {
+        double big = 0.0;
+        for (int i = 0; i < r.nxy; ++i)
+            for (int f = 2 * pr; f < 2 * pr + 2; ++f)
+                for (int m = 0; m < nz; ++m) {
+                    const double2 v = ul[((size_t)i * 4 + f) * nz + m];
+                    big = std::max(big, std::max(std::fabs(v.x), std::fabs(v.y)));
+                }
+        for (int i = 0; i < r.nxy; i += ny) {
+            const double2 v = ul[((size_t)i * 4 + 2 * pr) * nz];
+            if (std::max(std::fabs(v.x), std::fabs(v.y)) > 1e-12 * big)
+                return fail(QL_ERR_ARG,
+                            "set_state: %s of mode %d (ky = 0) is %g + %gi at kz = 0, the compressive slot a "
+                            "divergence-free state holds at zero (largest field value %g); zero it before upload",
+                            pr == 0 ? "ux" : "bx", i, v.x, v.y, big);
+        }
+    }
+    return QL_OK;
+}
+
 int set_state(Batch &b, int run, const double *lin, const double *mf, const double *kx0)
 {
     const RunInfo &r = b.runs[run];
@@ -759,6 +787,7 @@ int set_state(Batch &b, int run, const double *lin, const double *mf, const doub
         lin_to_int(ul, tl.data(), r.nxy, 2, g.nz);
         ul = tl.data();
     }
+    if (ul) QL_RC(check_solenoidal(b, r, ul));
     if (b.hd && um) {
         tm.assign((size_t)b.nMF * g.nz, make_double2(0.0, 0.0));
         std::copy(um, um + 2 * g.nz, tm.begin() + 2 * g.nz);

@@ -117,7 +117,7 @@ struct ModelFull {
     // MHD_FullQlin.cpp:271-300 (products) and :322-362 (EMF, stresses)
     template <class C>
     __device__ static void products(int part, const C *X, int ld, const RealOf<C> *mf, RealOf<C> kx, RealOf<C> ky,
-                                    C *prod, double *racc)
+                                    C dzc, C *prod, double *racc)
     {
         typedef RealOf<C> R;
         const C u0 = X[0], u1 = X[ld], u2 = X[2 * ld];
@@ -208,7 +208,7 @@ struct ModelHD {
     __device__ static int acc_slot(int, int c) { return 2 + c; }
     template <class C>
     __device__ static void products(int part, const C *X, int ld, const RealOf<C> *mf, RealOf<C> kx, RealOf<C> ky,
-                                    C *prod, double *racc)
+                                    C dzc, C *prod, double *racc)
     {
         typedef RealOf<C> R;
         racc[0] = racc[1] = 0.0;
@@ -251,7 +251,11 @@ struct ModelHD {
 
 // The reference transforms dz uz as a line of its own (MHD_BQlin.cpp:195);
 // the reconstruction is divergence free slot by slot (as for ModelFull), so
-// on the z grid dz uz = -i (kx ux + ky uy) from the ux and uy lines.
+// on the z grid dz uz = -i (kx ux + ky uy) from the ux and uy lines, except
+// for the kz = 0 slot of a ky = 0 mode, where uz = 0 and ux is free: there
+// -i kx ux is a z-constant that the reference's line does not hold.  The
+// kernels pass it back as dzc = i kx (fac1D ux at kz = 0) for ky = 0 modes
+// (zero otherwise), so the result is the reference's for any state.
 struct ModelBQ {
     static constexpr int NPAIR = 2;
     static constexpr int NB = 8;    // bases: ux zeta bx eta uy uz by bz
@@ -283,12 +287,12 @@ struct ModelBQ {
     // MHD_BQlin.cpp:153-201 (products) and :219-240 (EMF)
     template <class C>
     __device__ static void products(int part, const C *X, int ld, const RealOf<C> *mf, RealOf<C> kx, RealOf<C> ky,
-                                    C *prod, double *racc)
+                                    C dzc, C *prod, double *racc)
     {
         typedef RealOf<C> R;
         const C ux = X[0], ze = X[ld], bx = X[2 * ld], et = X[3 * ld];
         const C uy = X[4 * ld], uz = X[5 * ld], by = X[6 * ld], bz = X[7 * ld];
-        const C dzuz = cadd(imul(-kx, ux), imul(-ky, uy));
+        const C dzuz = cadd(cadd(imul(-kx, ux), imul(-ky, uy)), dzc);
         const R By = mf[0], dzBy = mf[1], dzdzBy = mf[2];
         if (part == 0) {
             prod[0] = cscale(By, imul(ky, bx));                 // u, first part
@@ -416,6 +420,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     __shared__ R skzr[NZD];
     __shared__ double skx[MB], sky[MB];
     __shared__ int sok[MB];
+    __shared__ double2 sdz[MB];  // fac1D ux at kz = 0 of a ky = 0 mode (ModelBQ's dz uz)
     // mean-field z-profiles [profile][z]: in registers of the product threads
     // for the FP32 path and at NZ = 256 (where LDS is short), in LDS otherwise
     // (FP64 registers are the scarcer resource)
@@ -519,6 +524,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
                 skx[imb] = kx;
                 sky[imb] = ky;
                 sok[imb] = l >= 0;
+                sdz[imb] = ky == 0.0 ? cscale(a.fac, B0) : make_double2(0.0, 0.0);
             }
         }
         __syncthreads();
@@ -538,8 +544,9 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             const R *mfp = MF_REG ? mfr : mfl;
             C prod[2][3];
             double racc[2][2];
-            P::products(0, X, MB * L::LS, mfp, kxr, kyr, prod[0], racc[0]);
-            P::products(1, X, MB * L::LS, mfp, kxr, kyr, prod[1], racc[1]);
+            const C dzc = from_d<C>(imul(skx[zmb], sdz[zmb]));
+            P::products(0, X, MB * L::LS, mfp, kxr, kyr, dzc, prod[0], racc[0]);
+            P::products(1, X, MB * L::LS, mfp, kxr, kyr, dzc, prod[1], racc[1]);
             if (sok[zmb]) {
                 const double ftfac = (sky[zmb] == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
 #pragma unroll

@@ -227,6 +227,8 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
         // D: products on the z grid, in place, and the EMF / stress sums
         {
             const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
+            // ModelBQ's dz uz constant of a ky = 0 mode: i kx fac1D ux at kz = 0
+            const C dzc = from_d<C>(ky == 0.0 ? imul(kx, cscale(a.fac, a.u[row])) : make_double2(0.0, 0.0));
 #pragma unroll
             for (int k = 0; k < ZI; ++k) {
                 if (k > 0) __builtin_amdgcn_sched_barrier(0);  // one z point at a time (registers)
@@ -238,8 +240,8 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
                     for (int c = 0; c < P::NMFZ; ++c) mfl[c] = smfz[c * NZ + zz];
                     C prod[2][3];
                     double racc[2][2];
-                    P::products(0, X, L::LS, mfl, (R)kx, (R)ky, prod[0], racc[0]);
-                    P::products(1, X, L::LS, mfl, (R)kx, (R)ky, prod[1], racc[1]);
+                    P::products(0, X, L::LS, mfl, (R)kx, (R)ky, dzc, prod[0], racc[0]);
+                    P::products(1, X, L::LS, mfl, (R)kx, (R)ky, dzc, prod[1], racc[1]);
 #pragma unroll
                     for (int part = 0; part < 2; ++part) {
                         acc[k][P::acc_slot(part, 0)] += ftfac * racc[part][0];

@@ -119,7 +119,7 @@ struct Batch {
     int old_init = 1;
     // diagnostics scratch
     double *dpart = nullptr, *dout = nullptr, *mfr_tmp = nullptr;
-    // replay graphs of 1 and GRAPH_STEPS steps (null until first use)
+    // replay graphs of 1 and GRAPH_STEPS steps (both captured by the first stepping call)
     hipGraphExec_t g1 = nullptr, gK = nullptr;
     // per-kernel timing (ql_profile_kernels)
     bool timing = false;

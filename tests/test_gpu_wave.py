@@ -144,14 +144,26 @@ def test_run_set_matches_oracle_per_run():
         assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL
 
 
-def test_fp32_full_at_256_refuses_the_wave_shape():
+@pytest.mark.parametrize("model", [0, 1])
+def test_fp32_at_256_refuses_the_wave_shape(model):
     from ql_dns_amd.capi import QLContext, QLError
     from ql_dns_amd.params import PREC_MIXED
-    c = QLContext(_params(NX=4, NY=4, NZ=256, precision=PREC_MIXED))
+    c = QLContext(_params(model=model, NX=4, NY=4, NZ=256, precision=PREC_MIXED))
+    assert c.config()[0]["line"] != WAVE
     with pytest.raises(QLError) as e:
         c.set_config(256, WAVE, 512)
     assert e.value.code == -1
     c.set_config(256, 256, 512)
+
+
+@pytest.mark.parametrize("model", [0, 1, 2])
+@pytest.mark.parametrize("NZ", [128, 256])
+def test_default_line_fp64(model, NZ):
+    """Without set_config, every model takes the wavefront kernel at NZ >= 128 in FP64."""
+    from ql_dns_amd.capi import QLContext
+    c = QLContext(_params(model=model, NX=4, NY=4, NZ=NZ))
+    assert c.config()[0]["line"] == WAVE
+    c.close()
 
 
 @pytest.mark.parametrize("integrator", [1, 0])
