@@ -89,3 +89,45 @@ def test_gpu_matches_fixture(name):
     if "lin100" in g:
         assert rel_err(l_, g["lin100"]) < 1e-10
         assert rel_err(c.mf_real(), g["mfreal100"]) < 1e-10
+
+
+# ----------------------------------------------------------------------------
+# the 64x128x128 production box over its configured 1000 steps
+# ----------------------------------------------------------------------------
+BOX64 = "box64x128x128_fullqlin_elin_1000.npz"
+
+
+def test_box64_fixture_initial_state():
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import workload
+    g = np.load(os.path.join(GOLD, BOX64), allow_pickle=False)
+    lin, mf = initial_state(workload("box64x128x128"))
+    assert np.array_equal(mf, g["ic_mf"])
+    assert rel_err(np.abs(lin).sum(axis=2), g["ic_linnorm"]) < 1e-15
+
+
+@pytest.mark.gpu
+def test_gpu_box64_1000_steps_matches_fixture():
+    """The HIP path (graph replay, default launch shape) over the full 1000
+    steps of the 64x128x128 box: mean fields, drives, diagnostics, per-mode
+    norms and the full state of 16 modes within 1e-10 of the oracle; kx0 after
+    the remaps and the accumulated time bit for bit."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import workload
+    g = np.load(os.path.join(GOLD, BOX64), allow_pickle=False)
+    p = workload("box64x128x128")
+    lin, mf = initial_state(p)
+    c = QLContext(p)
+    c.set_state(lin, mf)
+    c.linear_ops_init(0.0)
+    t, done = 0.0, 0
+    for n in (1, 100, 1000):
+        t = c.steps(n - done, t)
+        done = n
+        assert t == float(g[f"t{n}"])
+        l_, m_, k_ = c.get_state()
+        _check(g, m_, c.drive(), c.diagnostics(t), k_, n, 1e-10)
+        assert rel_err(np.abs(l_).sum(axis=2), g[f"linnorm{n}"]) < 1e-10
+    assert rel_err(l_[g["keep"]], g["lin_keep"]) < 1e-10
+    c.close()
