@@ -39,6 +39,55 @@ void TimeVariables::Save_Data(double t)
     records_.push_back(r);
 }
 
+// the tables of Kdata.cpp:28-125 for one device holding every mode
+Kdata::Kdata(const Model &mod)
+{
+    kxLfac = 2 * M_PI / mod.L(0);
+    kyLfac = 2 * M_PI / mod.L(1);
+    kzLfac = 2 * M_PI / mod.L(2);
+    const int ny = mod.N(1) / 2, nx = mod.N(0) / 2, nxy = mod.Dimxy_full();
+    kx_.resize(nxy);
+    ky_.resize(nxy);
+    kxi_.resize(nxy);
+    kyi_.resize(nxy);
+    for (int i = 0; i < 2 * nx; ++i) {
+        if (i == nx) continue;  // the Nyquist kx is left out
+        const int ix = i < nx ? i : -2 * nx + i, row = i < nx ? i : i - 1;
+        for (int j = 0; j < ny; ++j) {
+            kx_[j + ny * row] = dcmplx(0, ix * kxLfac);
+            ky_[j + ny * row] = dcmplx(0, j * kyLfac);
+            kxi_[j + ny * row] = ix;
+            kyi_[j + ny * row] = j;
+        }
+    }
+    kx = kx_.data();
+    ky = ky_.data();
+    kx_index = kx_index_full = kxi_.data();
+    ky_index = ky_index_full = kyi_.data();
+    const int nz = mod.NZ(), nzl = (nz - 1) / 2;
+    kz = dcmplxVec(nz);
+    kz2 = doubVec(nz);
+    for (int i = 0; i < nz; ++i) {
+        kz(i) = dcmplx(0, (i <= nzl ? i : -2 * nzl + i - 1) * kzLfac);
+        kz2(i) = (kz(i) * kz(i)).real();
+    }
+    lap2_.assign(ny, doubVec(nz));
+    ilap2_.assign(ny, doubVec(nz));
+    for (int j = 0; j < ny; ++j) {
+        const double kyr = ky_[j].imag();
+        for (int m = 0; m < nz; ++m) {
+            lap2_[j](m) = -kyr * kyr + kz2(m);
+            ilap2_[j](m) = 1 / lap2_[j](m);
+        }
+        if (kyr == 0) {
+            lap2_[j](0) = 0;
+            ilap2_[j](0) = 1;
+        }
+    }
+    lap2 = lap2_.data();
+    ilap2 = ilap2_.data();
+}
+
 GpuModel::GpuModel(const Inputs &sp) : Model(sp.NZ, sp.NXY, sp.L), sp_(sp), f_noise_(sp.f_noise)
 {
     ql_params p{};
@@ -67,11 +116,37 @@ GpuModel::GpuModel(const Inputs &sp) : Model(sp.NZ, sp.NXY, sp.L), sp_(sp), f_no
     noise_ = ql_noise{sp.f_noise, sp.noise_range_low, sp.noise_range_high, sp.drive_only_velocityQ ? 1 : 0,
                       sp.drive_only_magneticQ ? 1 : 0, sp.noise_seed};
     if (sp.f_noise != 0.0) check(ql_set_noise(ctx_, &noise_));
+    k_ = new Kdata(*this);
+    K = k_;
+    kx_device_.resize(Dimxy());
+    pull_kx();
 }
 
 GpuModel::~GpuModel()
 {
+    delete k_;
     if (ctx_) ql_destroy(ctx_);
+}
+
+void GpuModel::pull_kx()
+{
+    check(ql_get_state(ctx_, nullptr, nullptr, kx_device_.data()));
+    for (int i = 0; i < Dimxy(); ++i) K->kx[i] = dcmplx(0, kx_device_[i]);
+}
+
+void GpuModel::push_kx()
+{
+    bool changed = false;
+    for (int i = 0; i < Dimxy() && !changed; ++i) changed = K->kx[i].imag() != kx_device_[i];
+    if (!changed) return;
+    for (int i = 0; i < Dimxy(); ++i) kx_device_[i] = K->kx[i].imag();
+    check(ql_set_state(ctx_, nullptr, nullptr, kx_device_.data()));
+}
+
+void GpuModel::set_kx(const double *kx0)
+{
+    for (int i = 0; i < Dimxy(); ++i) K->kx[i] = dcmplx(0, kx0[i]);
+    push_kx();
 }
 
 void GpuModel::check(int rc) const
@@ -81,6 +156,7 @@ void GpuModel::check(int rc) const
 
 void GpuModel::rhs(const double t, const double dt_lin, solution *SolIn, solution *SolOut, doubVec **linOpFluct)
 {
+    push_kx();
     SolIn->push();
     const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs(), nL = num_Lin();
     std::vector<dcmplx> out((size_t)nxy * nL * nz), omf((size_t)nMF * nz);
@@ -96,6 +172,7 @@ void GpuModel::rhs(const double t, const double dt_lin, solution *SolIn, solutio
 
 void GpuModel::linearOPs_Init(double t0, doubVec **linOpFluct, doubVec *linOpMF)
 {
+    push_kx();
     const int nxy = Dimxy(), nz = NZ(), nMF = num_MFs(), nL = num_Lin();
     std::vector<double> L((size_t)nxy * nL * nz), Lmf((size_t)nMF * nz);
     check(ql_linear_ops_init(ctx_, t0, L.data(), Lmf.data()));
@@ -110,14 +187,17 @@ void GpuModel::linearOPs_Init(double t0, doubVec **linOpFluct, doubVec *linOpMF)
 
 void GpuModel::ShearingBox_Remap(double qt, solution *sol)
 {
+    push_kx();
     sol->push();
     check(ql_remap(ctx_, qt));
     sol->mark_device_newer();
+    pull_kx();  // K->kx of the remapped modes (MHD_FullQlin.cpp:474)
 }
 
 void GpuModel::DrivingNoise(double t, double dt, solution *sol)
 {
     // MHD_FullQlin.cpp:786-1053 / MHD_BQlin.cpp:558-740 on the device
+    push_kx();
     sol->push();
     check(ql_driving_noise(ctx_, t, dt));
     sol->mark_device_newer();
@@ -135,6 +215,7 @@ void GpuModel::ChangeNoiseRange(double fnoise, double kmin, double kmax)
 
 void GpuModel::Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *sol)
 {
+    push_kx();
     const_cast<solution *>(sol)->push();
     double out[18];
     check(ql_diagnostics(ctx_, t, out));
@@ -166,6 +247,7 @@ double GpuModel::Calculate_CFL(const solution *sol)
 {
     // MHD_FullQlin.cpp:769-780 / MHD_BQlin.cpp:540-552 on the maxima the
     // device records: profiles of the last rhs (and MHD_BQlin's current Bx)
+    push_kx();
     const_cast<solution *>(sol)->push();
     double v = 0.0;
     check(ql_calculate_cfl(ctx_, &v));
@@ -252,6 +334,7 @@ GpuIntegrator::GpuIntegrator(double t0, const Inputs &SP, GpuModel *mod) : model
 
 double GpuIntegrator::Step(double t, solution *sol)
 {
+    model_->push_kx();
     sol->push();
     double dt = 0.0;
     model_->check(ql_step(model_->ctx(), t, &dt));

@@ -22,6 +22,7 @@
 #pragma once
 
 #include <complex>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -81,6 +82,33 @@ struct Inputs {
 
 class solution;
 class TimeVariables;
+class Model;
+
+// Auxiliary/Kdata.h:8-27: the wavenumber tables, k stored as i*k as the
+// reference stores them, for callers that read Model::K (FullSave_Load.cpp:110,
+// 134, 196, 233, 291, 344).  One device holds every mode, so the local and
+// full index tables are the same.  kx is the current (remapped) kx0 of each
+// mode; the model refreshes it after each remap, and a value the caller
+// writes (a restart, FullSave_Load.cpp:344) reaches the device at the next
+// device operation, as solution::pLin does.
+class Kdata {
+public:
+    explicit Kdata(const Model &mod);
+    Kdata(const Kdata &) = delete;
+    Kdata &operator=(const Kdata &) = delete;
+    dcmplx *kx = nullptr, *ky = nullptr;  // [nxy]
+    int *ky_index = nullptr, *kx_index = nullptr;
+    int *ky_index_full = nullptr, *kx_index_full = nullptr;
+    dcmplxVec kz;                            // [nz], slot order (0..nzl, -nzl..-1)
+    doubVec kz2;                             // -kz^2
+    doubVec *lap2 = nullptr, *ilap2 = nullptr;  // per ky index: -ky^2 + kz2 and 1/that (1 at ky = kz = 0)
+    double kxLfac = 0, kyLfac = 0, kzLfac = 0;
+
+private:
+    std::vector<dcmplx> kx_, ky_;
+    std::vector<int> kxi_, kyi_;
+    std::vector<doubVec> lap2_, ilap2_;
+};
 
 // Models/Model.h:24-112
 class Model {
@@ -111,6 +139,7 @@ public:
     virtual int num_reynolds_saves() = 0;
     virtual void Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *sol) = 0;
     virtual double Calculate_CFL(const solution *sol) = 0;
+    Kdata *K = nullptr;  // Model.h:74
 
 protected:
     int Nxy_[2];
@@ -169,8 +198,12 @@ public:
     void Calc_Energy_AM_Diss(TimeVariables *tv, double t, const solution *sol) override;
     double Calculate_CFL(const solution *sol) override;
 
-    // current (possibly remapped) Im(kx) per mode, the reference's K->kx
+    // current (possibly remapped) Im(kx) per mode from the device
     std::vector<double> kx() const;
+    // sets every mode's kx0 (K->kx and the device)
+    void set_kx(const double *kx0);
+    // K->kx written by the caller since the device last saw it: upload it
+    void push_kx();
     ql_ctx *ctx() const { return ctx_; }
     const Inputs &inputs() const { return sp_; }
     void check(int rc) const;
@@ -184,6 +217,9 @@ private:
     double f_noise_ = 0.0;
     ql_noise noise_{};
     bool hd_ = false;
+    Kdata *k_ = nullptr;             // owned; K points at it
+    std::vector<double> kx_device_;  // kx0 as the device holds it
+    void pull_kx();
 };
 
 // solution.h:23-49

@@ -371,6 +371,7 @@ void FullSave_Load::SaveSolution(double t, solution *sol)
     const qlio_layout L = layout_of(eqs_);
     std::vector<dcmplx> lin, mf;
     sol->pack(lin, mf);
+    eqs_->push_kx();
     const std::vector<double> kx0 = eqs_->kx();
     io_check(qlio_h5_save(file_name_.c_str(), fresh_ ? 1 : 0, &L, t, eqs_->inputs().q,
                           reinterpret_cast<const double *>(lin.data()), reinterpret_cast<const double *>(mf.data()),
@@ -387,7 +388,7 @@ double FullSave_Load::LoadSolution_forRestart(solution *sol)
     io_check(qlio_h5_load_latest(file_name_.c_str(), &L, eqs_->inputs().q, &t, reinterpret_cast<double *>(lin.data()),
                                  reinterpret_cast<double *>(mf.data()), kx0.data()));
     sol->unpack(lin, mf);
-    eqs_->check(ql_set_state(eqs_->ctx(), nullptr, nullptr, kx0.data()));
+    eqs_->set_kx(kx0.data());  // K->kx and the device (FullSave_Load.cpp:344)
     return t;
 }
 

@@ -9,7 +9,11 @@
 // state.bin: lin [nxy][4][nz] then mf [nMF][nz], complex128.
 // out.bin: rhs_lin, rhs_mf, linop (from the initial state at t = 0.0, dt_lin = 0.0053),
 //          then lin, mf, kx, t, 18 diagnostics after nsteps, the Reynolds-save
-//          count and values.
+//          count and values, then the Kdata tables Model::K holds: Im(ky),
+//          kx_index_full, ky_index_full (per mode), Im(kz), kz2 (per kz slot),
+//          lap2 and ilap2 of every ky index.  Before the loop the last mode's
+//          K->kx is moved by half a kx spacing through Model::K (as a restart
+//          writes it, FullSave_Load.cpp:344); kx is read back from K->kx.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -67,6 +71,9 @@ int main(int argc, char **argv)
         for (int i = 0; i < nxy; ++i)
             for (int j = 0; j < nLin; ++j) std::fwrite(Lrows[i][j].data(), sizeof(double), nz, fo);
 
+        // a write through Model::K, as FullSave_Load's restart does
+        fluidEqs->K->kx[nxy - 1] = dcmplx(0, fluidEqs->K->kx[nxy - 1].imag() + 0.5 * fluidEqs->K->kxLfac);
+
         // main.cpp:90-123
         double t = SP.t_start, dt;
         Integrator *integrator = new GpuIntegrator(t, SP, fluidEqs);
@@ -84,7 +91,8 @@ int main(int argc, char **argv)
         for (int i = 0; i < nxy; ++i)
             for (int j = 0; j < nLin; ++j) std::fwrite(sol->pLin(i, j)->data(), sizeof(dcmplx), nz, fo);
         for (int f = 0; f < nMF; ++f) std::fwrite(sol->pMF(f)->data(), sizeof(dcmplx), nz, fo);
-        std::vector<double> kx = fluidEqs->kx();
+        std::vector<double> kx(nxy);
+        for (int i = 0; i < nxy; ++i) kx[i] = fluidEqs->K->kx[i].imag();
         std::fwrite(kx.data(), sizeof(double), nxy, fo);
         std::fwrite(&t, sizeof(double), 1, fo);
         const TimeVariables::Record &r = tv.records().back();
@@ -102,6 +110,19 @@ int main(int argc, char **argv)
         const double nrey = (double)fluidEqs->num_reynolds_saves();
         std::fwrite(&nrey, sizeof(double), 1, fo);
         std::fwrite(r.reynolds.data(), sizeof(double), (size_t)nrey, fo);
+        const Kdata *K = fluidEqs->K;
+        for (int i = 0; i < nxy; ++i) {
+            const double v[3] = {K->ky[i].imag(), (double)K->kx_index_full[i], (double)K->ky_index_full[i]};
+            std::fwrite(v, sizeof(double), 3, fo);
+        }
+        for (int m = 0; m < nz; ++m) {
+            const double v[2] = {K->kz(m).imag(), K->kz2(m)};
+            std::fwrite(v, sizeof(double), 2, fo);
+        }
+        for (int j = 0; j < SP.NXY[1] / 2; ++j) {
+            std::fwrite(K->lap2[j].data(), sizeof(double), nz, fo);
+            std::fwrite(K->ilap2[j].data(), sizeof(double), nz, fo);
+        }
         std::fclose(fo);
         std::printf("mean time step %.6g over %d steps\n", integrator->mean_time_step(), nsteps);
         delete integrator;

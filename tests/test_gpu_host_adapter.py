@@ -48,6 +48,9 @@ def test_main_loop_through_cpp_adapter(tmp_path, model, integrator, dt, cfl):
     fl, fm, kx, t, diag = take(nl), take(nm), take(p.nxy, False), take(1, False)[0], take(18, False)
     nrey = int(take(1, False)[0])
     rey = take(nrey, False)
+    ktab = take(3 * p.nxy, False).reshape(p.nxy, 3)
+    kzt = take(2 * p.nz, False).reshape(p.nz, 2)
+    laps = take(2 * (p.NY // 2) * p.nz, False).reshape(p.NY // 2, 2, p.nz)
     assert o_ == raw.size
 
     o = Oracle.from_params(p, nthreads=8)
@@ -57,6 +60,21 @@ def test_main_loop_through_cpp_adapter(tmp_path, model, integrator, dt, cfl):
     assert rel_err(rhs_lin, ro[0].ravel()) < 1e-10
     assert rel_err(rhs_mf, ro[1].ravel()) < 1e-10
     assert rel_err(linop, ro[2].ravel()) < 1e-10
+    # Model::K: the reference's tables (Kdata.cpp:28-125) against the oracle's
+    kx0_t, ky_t, kxi_t, kyi_t, kz_t = o.tables()
+    assert np.array_equal(ktab[:, 0], ky_t) and np.array_equal(ktab[:, 1], kxi_t) and np.array_equal(ktab[:, 2], kyi_t)
+    assert np.array_equal(kzt[:, 0], kz_t) and np.array_equal(kzt[:, 1], -kz_t * kz_t)
+    for j in range(p.NY // 2):
+        lap2 = -ky_t[j] ** 2 - kz_t ** 2
+        ilap2 = 1 / np.where(lap2 == 0, 1, lap2)
+        if ky_t[j] == 0:
+            lap2[0], ilap2[0] = 0.0, 1.0
+        assert np.array_equal(laps[j, 0], lap2) and np.array_equal(laps[j, 1], ilap2)
+    # the K->kx write of the C++ program before its loop
+    kx_w = kx0_t.copy()
+    kx_w[-1] += 0.5 * 2 * np.pi / p.Lx
+    o.set_state(kx0=kx_w)
+    o.linear_ops_init(0.0)  # the integrator is constructed after the write
     to = o.steps(nsteps, 0.0)
     lo, mo, ko = o.get_state()
     if dt > 0:
