@@ -29,19 +29,39 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
+FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
+
+
 def build(verbose: bool = False) -> str:
+    """Compiles the sources to objects in parallel (one hipcc each), then links
+    libql_dns.so."""
+    import tempfile
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-result",
-           "-I", os.path.join(ROOT, "include"),
-           "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libql_dns.so")
+    with tempfile.TemporaryDirectory(prefix="ql_build_") as odir:
+        procs = []
+        for src in SOURCES:
+            obj = os.path.join(odir, os.path.basename(src) + ".o")
+            cmd = [hipcc()] + FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", "-o", obj,
+                                       os.path.join(CSRC, src)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        failed = False
+        for obj, pr in procs:
+            out, _ = pr.communicate()
+            if pr.returncode != 0:
+                sys.stderr.write(out)
+                failed = True
+        if failed:
+            raise RuntimeError("hipcc failed compiling libql_dns.so")
+        cmd = [hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp] + [o for o, _ in procs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed linking libql_dns.so")
     os.replace(tmp, LIB)
     io = build_io(verbose)
     build_host_test(verbose)
