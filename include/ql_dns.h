@@ -23,6 +23,8 @@
 #ifndef QL_DNS_H
 #define QL_DNS_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -87,6 +89,25 @@ int ql_linear_ops_init(ql_ctx *ctx, double t0, double *linop, double *linop_mf);
  * (Models/Model.h:46-47; MHD_FullQlin.cpp:171-427; MHD_BQlin.cpp:117-284).
  * The state is not modified.  Outputs may be NULL. */
 int ql_rhs(ql_ctx *ctx, double t, double dt_lin, double *rhs_lin, double *rhs_mf, double *linop);
+
+/* Scratch of ql_rhs_noalloc, per device: 8 MiB. */
+#define QL_NOALLOC_SCRATCH_BYTES ((size_t)8 << 20)
+
+/* No counterpart in the reference: ql_rhs for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same computation as ql_rhs on the context's current
+ * state, with the outputs in caller-owned DEVICE memory in the same layouts
+ * (d_rhs_lin [nxy][4][nz] complex, d_rhs_mf [nMF][nz] complex, d_linop
+ * [nxy][4][nz] double; any may be NULL) and the drive of ql_get_drive updated.
+ * Its stress partial rows and column sums live in a fixed per-device scratch
+ * area of QL_NOALLOC_SCRATCH_BYTES, so it calls no hipMalloc: launches with
+ * (workgroups + 1) * nMF * NZ + 2 * nMF * nz doubles above that are refused
+ * (QL_ERR_ARG).  The work is enqueued on the context's stream and the call
+ * returns without waiting (ql_synchronize before reading the outputs).  Every
+ * call on one device shares that scratch: calls that overlap in time on one
+ * device (different streams or threads) race.  MHD_FullQlin and MHD_BQlin
+ * contexts; HD_fullU returns QL_ERR_UNSUPPORTED. */
+int ql_rhs_noalloc(ql_ctx *ctx, double t, double dt_lin, double *d_rhs_lin, double *d_rhs_mf, double *d_linop);
 
 /* Integrator::Step(t, sol) (Integrators/Integrator.h:23; RK3CN.cpp:77-201,
  * RK3CN_Elin.cpp:275-389).  Writes the step size to *dt_out (may be NULL):

@@ -25,7 +25,8 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_linear_ops_init", "ql_batch_steps", "ql_batch_tune", "ql_batch_get_config",
            "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward", "ql_calculate_cfl",
            "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution", "ql_set_noise",
-           "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise", "ql_drive_real")
+           "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise", "ql_drive_real",
+           "ql_rhs_noalloc")
 
 QL_ERR_STATE = -4
 
@@ -79,6 +80,7 @@ def load(path: str = LIB_PATH):
         "ql_get_state": ([P, D, D, D], ctypes.c_int),
         "ql_linear_ops_init": ([P, ctypes.c_double, D, D], ctypes.c_int),
         "ql_rhs": ([P, ctypes.c_double, ctypes.c_double, D, D, D], ctypes.c_int),
+        "ql_rhs_noalloc": ([P, ctypes.c_double, ctypes.c_double, P, P, P], ctypes.c_int),
         "ql_step": ([P, ctypes.c_double, D], ctypes.c_int),
         "ql_steps": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_remap": ([P, ctypes.c_double], ctypes.c_int),

@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "ql_mode.hpp"
 #include "ql_mode_wave.hpp"
@@ -534,7 +535,9 @@ int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool
     return QL_OK;
 }
 
-// mode kernel -> column reduction of its partials -> mean-field kernel
+// mode kernel -> column reduction of its partials -> mean-field kernel; the
+// partial rows and column sums are a.partials and m.sums (the group's own
+// buffers, or ql_rhs_noalloc's scratch)
 int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFArgs &m, hipStream_t st)
 {
     if (b.timing) QL_HIP(hipEventRecord(b.ev[0], st));
@@ -542,7 +545,8 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
     QL_HIP(hipGetLastError());
     if (b.timing) QL_HIP(hipEventRecord(b.ev[1], st));
     const int ncols = b.nMF * g.NZ;
-    qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(g.part, g.rp, ncols, g.sums);
+    qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(a.partials, g.rp, ncols,
+                                                                                                  const_cast<double *>(m.sums));
     QL_HIP(hipGetLastError());
     launch_mf(g, m, st);
     QL_HIP(hipGetLastError());
@@ -1015,6 +1019,31 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
     return rc;
 }
 
+// Scratch of ql_rhs_noalloc (the one entry point that allocates nothing): the
+// mode kernel's stress partial rows, their column sums and a mean-field RHS
+// slot.  Each device holds its own copy from module load until the process
+// ends; every ql_rhs_noalloc call on a device uses it.
+constexpr size_t NOALLOC_DOUBLES = QL_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device (once per device):
+// the array's name is a host-side stand-in that must never reach a kernel
+int resolve_scratch(int dev)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(QL_ERR_ARG, "device %d beyond %d", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return QL_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(QL_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(ptr);
+    return QL_OK;
+}
+
 int init_batch(Batch &b, const ql_params *ps, int nruns, const char *who)
 {
     if (nruns < 1) return fail(QL_ERR_ARG, "%s: nruns=%d", who, nruns);
@@ -1034,6 +1063,7 @@ int init_batch(Batch &b, const ql_params *ps, int nruns, const char *who)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(QL_ERR_HIP, "%s: no HIP device", who);
     if (ps[0].device < 0 || ps[0].device >= ndev) return fail(QL_ERR_ARG, "%s: device %d of %d", who, ps[0].device, ndev);
     QL_HIP(hipSetDevice(ps[0].device));
+    QL_RC(resolve_scratch(ps[0].device));
     int rc = create_batch(b, ps, nruns);
     if (rc) release(b);
     return rc;
@@ -1163,6 +1193,42 @@ int ql_rhs(ql_ctx *c, double t, double dt_lin, double *rhs_lin, double *rhs_mf, 
     if (b.hd && rhs_lin) lin_to_ext(tl.data(), reinterpret_cast<double2 *>(rhs_lin), g.modes, nL, g.nz);
     if (b.hd && linop) lin_to_ext(tL.data(), linop, g.modes, nL, g.nz);
     return QL_OK;
+}
+
+int ql_rhs_noalloc(ql_ctx *c, double t, double dt_lin, double *d_rhs_lin, double *d_rhs_mf, double *d_linop)
+{
+    if (!c) return fail(QL_ERR_ARG, "ql_rhs_noalloc: null context");
+    Batch &b = c->bt.b;
+    if (b.hd) return fail(QL_ERR_UNSUPPORTED, "ql_rhs_noalloc: HD_fullU (its host layout differs from the device's)");
+    Group &g = b.groups[0];
+    double *const sp = (b.device >= 0 && b.device < MAX_DEVICES) ? g_scratch[b.device] : nullptr;
+    if (!sp) return fail(QL_ERR_HIP, "ql_rhs_noalloc: scratch of device %d not resolved", b.device);
+    const size_t ncols = (size_t)b.nMF * g.NZ;
+    const size_t need = (size_t)g.W * ncols + ncols + (size_t)2 * b.nMF * g.nz;
+    if (need > NOALLOC_DOUBLES)
+        return fail(QL_ERR_ARG,
+                    "ql_rhs_noalloc: %d workgroups x %zu stress columns need %zu bytes of scratch, above the %zu "
+                    "available (use a launch shape with fewer workgroups, ql_batch_set_config)",
+                    g.W, ncols, need * sizeof(double), (size_t)QL_NOALLOC_SCRATCH_BYTES);
+    QL_HIP(hipSetDevice(b.device));
+    // every partial row and column sum is written in full before it is read,
+    // so the contents left by an earlier call need no clearing
+    double *part = sp, *sums = sp + (size_t)g.W * ncols;
+    double2 *mf_slot = reinterpret_cast<double2 *>(sums + ncols);
+    QL_RC(refresh_profiles(b, g, b.s));
+    qlk::ModeArgs a = mode_args(b, g);
+    a.s = 0;
+    a.mode = qlk::MODE_RHS;
+    a.ts_x = t;
+    a.dtl_x = dt_lin;
+    a.n_out = reinterpret_cast<double2 *>(d_rhs_lin);
+    a.lin_out = d_linop;
+    a.partials = part;
+    qlk::MFArgs m = mf_args(b, g);
+    m.mode = qlk::MODE_RHS;
+    m.sums = sums;
+    m.mfn_out = d_rhs_mf ? reinterpret_cast<double2 *>(d_rhs_mf) : mf_slot;
+    return enqueue_sequence(b, g, a, m, b.s);
 }
 
 int ql_step(ql_ctx *c, double t, double *dt_out)
