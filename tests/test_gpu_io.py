@@ -3,8 +3,11 @@
 A run that saves FullSolution.h5 after n1 steps and continues n2 steps is
 compared with a run restarted from that save (LoadSolution_forRestart, a new
 integrator at t_start, RK3CN_Elin): bit for bit with the float64 dump, to
-float rounding with the reference's float32 dump.  The .dat time series hold
-one record per save.
+float rounding with the reference's float32 dump.  Both are anchored to the
+oracle: the continuous run against the oracle's n1 + n2 steps from the same
+initial state, the restarted run against the oracle's n2 steps from the state
+read back from the file, each within 1e-10 (kx0 and t exactly).  The .dat time
+series hold one record per save.
 """
 import os
 import subprocess
@@ -37,6 +40,21 @@ def _read(path, nxy, nz, nmf):
     return out
 
 
+def _anchor(p, lin, mf, n, got, t0=0.0, kx0=None):
+    """The oracle's n steps of main.cpp's loop from (lin, mf, kx0) at t0 (its
+    integrator constructed at t0, as a restart constructs it) against got."""
+    from oracle.oracle import Oracle
+    o = Oracle.from_params(p, nthreads=8)
+    o.set_state(lin, mf, kx0)
+    o.linear_ops_init(t0)
+    t = o.steps(n, t0)
+    lo, mo, ko = o.get_state()
+    lg, mg, kg, tg = got
+    assert tg == t and np.array_equal(kg, ko)
+    assert rel_err(mg, mo) < 1e-10 and rel_err(lg, lo) < 1e-10
+    o.close()
+
+
 @pytest.mark.parametrize("fp64", [1, 0])
 def test_cpp_dump_and_restart(tmp_path, fp64):
     from ql_dns_amd.ic import initial_state
@@ -62,6 +80,10 @@ def test_cpp_dump_and_restart(tmp_path, fp64):
         # as float32 (FullSave_Load.cpp:183-188, 231-240)
         assert abs(tb - ta) < 1e-12 and rel_err(kb, ka) < 1e-6
         assert rel_err(mb, ma) < 1e-5 and rel_err(lb, la) < 1e-5
+    _anchor(p, lin, mf, n1 + n2, (la, ma, ka, ta))
+    from ql_dns_amd import qlio
+    t0, l0, m0, k0 = qlio.h5_load_latest(os.path.join(d, "FullSolution.h5"), p)
+    _anchor(p, l0, m0, n2, (lb, mb, kb, tb), t0, k0)
     n = n1 + n2 + 1
     rd = lambda name: np.fromfile(os.path.join(d, name), dtype="<f8")
     assert rd("time_vec.dat").size == n
@@ -95,3 +117,6 @@ def test_python_dump_and_restart(tmp_path):
     assert tb == ta
     for x, y in zip(a.get_state(), b.get_state()):
         assert np.array_equal(x, y)
+    lb, mb, kb = b.get_state()
+    _anchor(p, lin, mf, 60, (lb, mb, kb, tb))
+    _anchor(p, l0, m0, 30, (lb, mb, kb, tb), t0, k0)
