@@ -189,6 +189,7 @@ def bench_mixed(name: str = "box128x128x256", marks=(100, 500, 2000)):
         c.set_state(lin, mf)
         c.linear_ops_init(0.0)
         ctx[prec] = c
+    fp64_modes_start = ctx["mixed"].mixed_split()
     el = {"fp64": 0.0, "mixed": 0.0}
     t = {"fp64": 0.0, "mixed": 0.0}
     errs = []
@@ -211,11 +212,13 @@ def bench_mixed(name: str = "box128x128x256", marks=(100, 500, 2000)):
         e["within_1e-6"] = all(e[k] <= ERROR_BAR for k in ("mean_fields", "emf_stress_drive", "fluct_energy_u",
                                                             "fluct_energy_b"))
         errs.append(e)
+    fp64_modes_end = ctx["mixed"].mixed_split()
     for c in ctx.values():
         c.close()
     n = marks[-1]
     return {"workload": name, "steps": n, "fp64_steps_per_s": n / el["fp64"],
             "mixed_steps_per_s": n / el["mixed"], "speedup": el["fp64"] / el["mixed"],
+            "fp64_modes": [fp64_modes_start, fp64_modes_end], "modes": p.nxy,
             "errors_vs_fp64": errs, "error_bar": ERROR_BAR,
             "error_bar_met": all(e["within_1e-6"] for e in errs)}
 

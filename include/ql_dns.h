@@ -239,6 +239,13 @@ int ql_batch_steps(ql_batch *b, int nsteps, double *t_inout);
  * The shape sets the summation order of the stress reduction: results of two
  * shapes agree to rounding, results of one shape are bitwise reproducible. */
 int ql_batch_tune(ql_batch *b, int trial_steps, double *step_ms);
+/* Mixed precision (QL_PREC_MIXED): a mode whose largest |value| reaches tau
+ * times the largest of its run runs in FP64 (a companion launch of the FP64
+ * kernel before the FP32 one), the others in FP32; the choice is made again
+ * every 16 steps of the clock and after each state upload.  Sets tau when
+ * tau > 0 (default 0.03; tau <= 0 leaves it) and writes to *nfp64 (may be
+ * NULL) how many modes of the set reach it for the current state. */
+int ql_batch_mixed_split(ql_batch *b, double tau, int *nfp64);
 /* current shape per group: cfg[5k..5k+4] = NZ, line, target, modes per
  * workgroup pass, workgroups; *ngroups receives the group count */
 int ql_batch_get_config(const ql_batch *b, int max_groups, int *ngroups, int *cfg);

@@ -26,7 +26,7 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward", "ql_calculate_cfl",
            "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution", "ql_set_noise",
            "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise", "ql_drive_real",
-           "ql_rhs_noalloc")
+           "ql_rhs_noalloc", "ql_batch_mixed_split")
 
 QL_ERR_STATE = -4
 
@@ -104,6 +104,7 @@ def load(path: str = LIB_PATH):
         "ql_batch_tune": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_batch_get_config": ([P, ctypes.c_int, I, I], ctypes.c_int),
         "ql_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "ql_batch_mixed_split": ([P, ctypes.c_double, I], ctypes.c_int),
         "ql_adjoint_forward": ([P, ctypes.c_int, ctypes.c_int, D], ctypes.c_int),
         "ql_adjoint_backward": ([P, D, D, D, D, D], ctypes.c_int),
         "ql_calculate_cfl": ([P, D], ctypes.c_int),
@@ -321,6 +322,13 @@ class QLContext:
 
     def set_config(self, NZ: int, line: int, target: int):
         self._check(self._L.ql_batch_set_config(self._L.ql_batch_of(self._h), NZ, line, target))
+
+    def mixed_split(self, tau: float = 0.0) -> int:
+        """Mixed precision: sets the FP64 amplitude fraction when tau > 0 and
+        returns how many modes currently reach it (ql_batch_mixed_split)."""
+        n = ctypes.c_int()
+        self._check(self._L.ql_batch_mixed_split(self._L.ql_batch_of(self._h), tau, ctypes.byref(n)))
+        return n.value
 
 
 class QLBatch:

@@ -54,6 +54,7 @@ void release(Batch &b)
         dfree(g.tw), dfree(g.tw32), dfree(g.kx0), dfree(g.ky), dfree(g.kzv), dfree(g.mfz), dfree(g.part);
         dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg);
         dfree(g.clock), dfree(g.pm), dfree(g.mode_run), dfree(g.np);
+        dfree(g.wg2), dfree(g.amp), dfree(g.hi);
         if (g.join) (void)hipEventDestroy(g.join);
         if (g.s) (void)hipStreamDestroy(g.s);
         g.join = nullptr;
@@ -184,6 +185,24 @@ qlk::RunParams run_params(const RunInfo &r)
     return rp;
 }
 
+// workgroups of a run's FP64 companion (mixed precision): a prime near
+// COMP_TARGET_WG that does not divide the mode count, so that the stride
+// dealing does not line up with the kx rows (the large modes sit at the start
+// of the rows), and at least enough that no workgroup has more than
+// qlk::COMP_LIST candidates
+int comp_nwg(int nxy)
+{
+    auto prime = [](int n) {
+        if (n < 2) return false;
+        for (int d = 2; d * d <= n; ++d)
+            if (n % d == 0) return false;
+        return true;
+    };
+    int n = std::max((nxy + qlk::COMP_LIST - 1) / qlk::COMP_LIST, std::min(COMP_TARGET_WG, nxy));
+    while (!(prime(n) && nxy % n != 0) && n < nxy) ++n;
+    return std::min(n, nxy);
+}
+
 // workgroup -> (run, mode-group range) tables for a launch shape: modes per
 // workgroup pass MB = line/NZ; consecutive passes per workgroup chosen so that
 // the group launches about `target` workgroups; each run's passes split into
@@ -216,14 +235,39 @@ int configure(Batch &b, Group &g, int line, int target)
         }
     }
     const int W = (int)wi.size();
+    // mixed precision: the FP64 companion (one-wavefront-per-mode kernel);
+    // workgroup j of a run's nw2 has g0 = j, g1 = nw2 and strides over the
+    // run's modes
+    std::vector<qlk::WgInfo> wi2;
+    if (g.split) {
+        for (int k = 0; k < g.nruns; ++k) {
+            qlk::RunParams &rp = g.h_rp[k];
+            const int nw2 = comp_nwg(rp.nxy);
+            rp.wg_off2 = W + (int)wi2.size();
+            rp.nwg2 = nw2;
+            for (int j = 0; j < nw2; ++j) {
+                qlk::WgInfo w{};
+                w.run = k;
+                w.g0 = j;
+                w.g1 = nw2;
+                w.mode_off = rp.mode_off;
+                w.nxy = rp.nxy;
+                wi2.push_back(w);
+            }
+        }
+    }
+    const int W2 = (int)wi2.size();
     const int ncols = (b.nMF) * g.NZ;
-    if (W > g.wg_cap) {
-        dfree(g.wg), dfree(g.part);
-        QL_RC(dalloc(&g.wg, W));
-        QL_RC(dalloc(&g.part, (size_t)W * ncols));
-        g.wg_cap = W;
+    if (W + W2 > g.wg_cap) {
+        dfree(g.wg), dfree(g.wg2), dfree(g.part);
+        QL_RC(dalloc(&g.wg, W + W2));
+        QL_RC(dalloc(&g.wg2, W + W2));
+        QL_RC(dalloc(&g.part, (size_t)(W + W2) * ncols));
+        g.wg_cap = W + W2;
     }
     QL_HIP(hipMemcpy(g.wg, wi.data(), sizeof(qlk::WgInfo) * W, hipMemcpyHostToDevice));
+    if (W2) QL_HIP(hipMemcpy(g.wg2, wi2.data(), sizeof(qlk::WgInfo) * W2, hipMemcpyHostToDevice));
+    g.W2 = W2;
     QL_HIP(hipMemcpy(g.rp, g.h_rp.data(), sizeof(qlk::RunParams) * g.nruns, hipMemcpyHostToDevice));
     g.line = line;
     g.target = target;
@@ -323,6 +367,11 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_RC(dalloc(&g.mode_run, g.modes));
         QL_RC(dalloc(&g.np, g.nruns));
         QL_RC(dalloc(&g.clock, 1));
+        g.split = b.precision == QL_PREC_MIXED;
+        if (g.split) {
+            QL_RC(dalloc(&g.amp, g.modes));
+            QL_RC(dalloc(&g.hi, g.modes));
+        }
         std::vector<double> kx0(g.modes), ky(g.modes);
         std::vector<int> mrun(g.modes);
         g.h_np.assign(g.nruns, qlk::NoiseParams{});
@@ -402,6 +451,36 @@ void launch_wave_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
     if (a.mode == qlk::MODE_ELIN) launch_wave_m<NZ, qlk::MODE_ELIN>(b, g, a, st);
     else if (a.mode == qlk::MODE_CN) launch_wave_m<NZ, qlk::MODE_CN>(b, g, a, st);
     else launch_wave_m<NZ, qlk::MODE_RHS>(b, g, a, st);
+}
+
+// the mixed-precision FP64 companion: the FP64 workgroup kernel over the
+// group's companion table, at the smallest pass of each NZ
+template <int NZ, int MODE>
+void launch_comp_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    constexpr int MB = NZ == 32 ? 4 : (NZ == 64 ? 2 : 1);
+    constexpr int T = qlk::mode_threads(NZ, MB);
+    if (b.hd) qlk::k_mode64s<NZ, MB, qlk::ModelHD, MODE><<<g.W2, T, 0, st>>>(a);
+    else if (b.model == QL_MODEL_MHD_BQLIN) qlk::k_mode64s<NZ, MB, qlk::ModelBQ, MODE><<<g.W2, T, 0, st>>>(a);
+    else qlk::k_mode64s<NZ, MB, qlk::ModelFull, MODE><<<g.W2, T, 0, st>>>(a);
+}
+
+template <int NZ>
+void launch_comp_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    if (a.mode == qlk::MODE_ELIN) launch_comp_m<NZ, qlk::MODE_ELIN>(b, g, a, st);
+    else if (a.mode == qlk::MODE_CN) launch_comp_m<NZ, qlk::MODE_CN>(b, g, a, st);
+    else launch_comp_m<NZ, qlk::MODE_RHS>(b, g, a, st);
+}
+
+void launch_comp(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
+{
+    switch (g.NZ) {
+    case 32: launch_comp_t<32>(b, g, a, st); break;
+    case 64: launch_comp_t<64>(b, g, a, st); break;
+    case 128: launch_comp_t<128>(b, g, a, st); break;
+    default: launch_comp_t<256>(b, g, a, st); break;
+    }
 }
 
 template <int NZ, int MB>
@@ -492,6 +571,40 @@ qlk::ModeArgs mode_args(const Batch &b, const Group &g)
     return a;
 }
 
+// mixed-precision split of a mode launch: the FP32 launch skips the flagged
+// modes, which the FP64 companion takes
+void set_split(const Group &g, qlk::ModeArgs &a)
+{
+    if (!g.split) return;
+    a.split = qlk::SPLIT_LO;
+    a.hi = g.hi;
+}
+
+// the precision flags from the mode amplitudes of the current state: always
+// (clocked = false: after the state changed outside the steps), or inside a
+// step when the group's clock is at a multiple of SPLIT_EVERY steps
+void launch_flags(const Batch &b, const Group &g, bool clocked, hipStream_t st)
+{
+    const qlk::Clock *ck = clocked ? g.clock : nullptr;
+    qlk::k_amp_modes<<<g.modes, 64, 0, st>>>(g.u, g.nz, g.amp, ck, SPLIT_EVERY);
+    qlk::k_amp_flags<<<g.nruns, qlk::AMP_NT, 0, st>>>(g.amp, g.rp, b.amp_tau, g.hi, ck, SPLIT_EVERY);
+}
+
+int prime_split(Batch &b, Group &g, hipStream_t st)
+{
+    if (!g.split || !g.amp_stale) return QL_OK;
+    launch_flags(b, g, false, st);
+    QL_HIP(hipGetLastError());
+    g.amp_stale = false;
+    return QL_OK;
+}
+
+int prime_split_all(Batch &b)
+{
+    for (Group &g : b.groups) QL_RC(prime_split(b, g, b.s));
+    return QL_OK;
+}
+
 qlk::MFArgs mf_args(const Batch &b, const Group &g)
 {
     qlk::MFArgs m{};
@@ -541,12 +654,26 @@ int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool
 int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFArgs &m, hipStream_t st)
 {
     if (b.timing) QL_HIP(hipEventRecord(b.ev[0], st));
-    launch_mode(b, g, a, st);
-    QL_HIP(hipGetLastError());
-    if (b.timing) QL_HIP(hipEventRecord(b.ev[1], st));
     const int ncols = b.nMF * g.NZ;
-    qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(a.partials, g.rp, ncols,
-                                                                                                  const_cast<double *>(m.sums));
+    if (a.split == qlk::SPLIT_LO) {
+        // the FP64 companion first, then the FP32 launch (the FP32 launch
+        // fills every CU in one round of workgroups, so running them side by
+        // side would only delay the workgroups the companion displaces)
+        qlk::ModeArgs a2 = a;
+        a2.split = qlk::SPLIT_HI;
+        a2.wg = g.wg2;
+        a2.partials = a.partials + (size_t)g.W * ncols;
+        launch_comp(b, g, a2, st);
+        QL_HIP(hipGetLastError());
+        launch_mode(b, g, a, st);
+        QL_HIP(hipGetLastError());
+    } else {
+        launch_mode(b, g, a, st);
+        QL_HIP(hipGetLastError());
+    }
+    if (b.timing) QL_HIP(hipEventRecord(b.ev[1], st));
+    qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(
+        a.partials, g.rp, ncols, const_cast<double *>(m.sums));
     QL_HIP(hipGetLastError());
     launch_mf(g, m, st);
     QL_HIP(hipGetLastError());
@@ -601,6 +728,7 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
     qlk::ModeArgs a = mode_args(b, g);
     a.s = s;
     a.mode = cn ? qlk::MODE_CN : qlk::MODE_ELIN;
+    set_split(g, a);
     a.linC = lc;
     a.nlc1 = nlc1;
     a.nlc2 = nlc2;
@@ -636,9 +764,19 @@ int enqueue_substep_all(Batch &b, int s, bool remap_after, bool advance)
     return QL_OK;
 }
 
+// mixed precision: the clocked flag refresh at the start of a step
+int enqueue_flags(Batch &b)
+{
+    for (Group &g : b.groups)
+        if (g.split) launch_flags(b, g, true, b.s);
+    QL_HIP(hipGetLastError());
+    return QL_OK;
+}
+
 // one step of main.cpp:108-123 (noise off): Step, t += dt, remap
 int enqueue_step(Batch &b, bool remap)
 {
+    QL_RC(enqueue_flags(b));
     for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, remap && s == 3, true));
     return QL_OK;
 }
@@ -650,6 +788,7 @@ int set_clock(Batch &b, double t)
     c.t_prev = b.t_prev;
     c.t_init = b.t_init;
     c.old_init = b.old_init;
+    c.nstep = b.nstep;
     c.dt = b.dt;
     c.dt_last = b.dt_last;
     for (Group &g : b.groups) qlk::k_clock_set<<<1, 64, 0, b.s>>>(g.clock, c, g.pm, g.rp, b.model, b.vdt);
@@ -691,6 +830,7 @@ int run_steps(Batch &b, int nsteps, double *t_inout)
 {
     double t = *t_inout;
     QL_RC(set_clock(b, t));
+    QL_RC(prime_split_all(b));
     if (b.timing) {
         for (int n = 0; n < nsteps; ++n) QL_RC(enqueue_step(b, true));
     } else if (nsteps > 0) {
@@ -717,6 +857,7 @@ int run_steps(Batch &b, int nsteps, double *t_inout)
         }
     }
     if (nsteps > 0) b.old_init = 0;
+    b.nstep += nsteps;
     *t_inout = t;
     return QL_OK;
 }
@@ -797,9 +938,11 @@ int set_state(Batch &b, int run, const double *lin, const double *mf, const doub
         std::copy(um, um + 2 * g.nz, tm.begin() + 2 * g.nz);
         um = tm.data();
     }
-    if (ul)
+    if (ul) {
         QL_HIP(hipMemcpyAsync(g.u + (size_t)r.mode_off * 4 * g.nz, ul, sizeof(double2) * (size_t)r.nxy * 4 * g.nz,
                               hipMemcpyHostToDevice, b.s));
+        g.amp_stale = true;
+    }
     if (um) QL_HIP(hipMemcpyAsync(g.mf + mo, um, sizeof(double2) * b.nMF * g.nz, hipMemcpyHostToDevice, b.s));
     if (kx0)
         QL_HIP(hipMemcpyAsync(g.kx0 + r.mode_off, kx0, sizeof(double) * r.nxy, hipMemcpyHostToDevice, b.s));
@@ -1006,6 +1149,7 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
                 hipMemcpyAsync(g.drive, sn.drive, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s) != hipSuccess ||
                 hipMemcpyAsync(g.kx0, sn.kx0, sizeof(double) * g.modes, hipMemcpyDeviceToDevice, b.s) != hipSuccess)
                 if (rc == QL_OK) rc = fail(QL_ERR_HIP, "tune: state restore failed");
+            g.amp_stale = true;
             if (rc == QL_OK) rc = refresh_profiles(b, g, b.s);
             // after the refresh, which rewrites the current-state maxima
             if (sn.pm && hipMemcpyAsync(g.pm, sn.pm, sizeof(double) * g.nruns * qlk::PM_STRIDE,
@@ -1168,9 +1312,11 @@ int ql_rhs(ql_ctx *c, double t, double dt_lin, double *rhs_lin, double *rhs_mf, 
     const size_t nl = (size_t)g.modes * 4 * g.nz;
     if (!g.lin_tmp) QL_RC(dalloc(&g.lin_tmp, nl));
     QL_RC(refresh_profiles(b, g, b.s));
+    QL_RC(prime_split(b, g, b.s));
     qlk::ModeArgs a = mode_args(b, g);
     a.s = 0;
     a.mode = qlk::MODE_RHS;
+    set_split(g, a);
     a.ts_x = t;
     a.dtl_x = dt_lin;
     a.n_out = g.N1;
@@ -1204,7 +1350,7 @@ int ql_rhs_noalloc(ql_ctx *c, double t, double dt_lin, double *d_rhs_lin, double
     double *const sp = (b.device >= 0 && b.device < MAX_DEVICES) ? g_scratch[b.device] : nullptr;
     if (!sp) return fail(QL_ERR_HIP, "ql_rhs_noalloc: scratch of device %d not resolved", b.device);
     const size_t ncols = (size_t)b.nMF * g.NZ;
-    const size_t need = (size_t)g.W * ncols + ncols + (size_t)2 * b.nMF * g.nz;
+    const size_t need = (size_t)(g.W + g.W2) * ncols + ncols + (size_t)2 * b.nMF * g.nz;
     if (need > NOALLOC_DOUBLES)
         return fail(QL_ERR_ARG,
                     "ql_rhs_noalloc: %d workgroups x %zu stress columns need %zu bytes of scratch, above the %zu "
@@ -1213,12 +1359,14 @@ int ql_rhs_noalloc(ql_ctx *c, double t, double dt_lin, double *d_rhs_lin, double
     QL_HIP(hipSetDevice(b.device));
     // every partial row and column sum is written in full before it is read,
     // so the contents left by an earlier call need no clearing
-    double *part = sp, *sums = sp + (size_t)g.W * ncols;
+    double *part = sp, *sums = sp + (size_t)(g.W + g.W2) * ncols;
     double2 *mf_slot = reinterpret_cast<double2 *>(sums + ncols);
     QL_RC(refresh_profiles(b, g, b.s));
+    QL_RC(prime_split(b, g, b.s));
     qlk::ModeArgs a = mode_args(b, g);
     a.s = 0;
     a.mode = qlk::MODE_RHS;
+    set_split(g, a);
     a.ts_x = t;
     a.dtl_x = dt_lin;
     a.n_out = reinterpret_cast<double2 *>(d_rhs_lin);
@@ -1236,9 +1384,12 @@ int ql_step(ql_ctx *c, double t, double *dt_out)
     if (!c) return fail(QL_ERR_ARG, "ql_step: null context");
     Batch &b = c->bt.b;
     QL_RC(set_clock(b, t));
+    QL_RC(prime_split_all(b));
+    QL_RC(enqueue_flags(b));
     for (int s = 1; s <= 3; ++s) QL_RC(enqueue_substep_all(b, s, false, false));
     b.t_prev = t;
     b.old_init = 0;
+    b.nstep += 1;
     QL_HIP(hipStreamSynchronize(b.s));
     double dt = b.dt;
     if (b.vdt) {
@@ -1416,6 +1567,7 @@ int ql_batch_linear_ops_init(ql_batch *bt, double t0)
 {
     if (!bt) return fail(QL_ERR_ARG, "ql_batch_linear_ops_init: null handle");
     bt->b.old_init = 1;
+    bt->b.nstep = 0;
     bt->b.t_init = t0;
     bt->b.t_prev = t0;
     return QL_OK;
@@ -1529,6 +1681,31 @@ int ql_batch_driving_noise(ql_batch *bt, double t, double dt)
     for (Group &g : b.groups)
         if (g.noise) QL_RC(launch_noise(g, false, t, dt, true, false, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    return QL_OK;
+}
+
+int ql_batch_mixed_split(ql_batch *bt, double tau, int *nfp64)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_mixed_split: null handle");
+    Batch &b = bt->b;
+    if (b.precision != QL_PREC_MIXED) return fail(QL_ERR_ARG, "ql_batch_mixed_split: not a mixed-precision set");
+    if (!(tau == tau) || tau > 1.0) return fail(QL_ERR_ARG, "ql_batch_mixed_split: tau %g outside (0, 1]", tau);
+    if (tau > 0.0) {
+        b.amp_tau = tau;
+        drop_graphs(b);  // the fraction is a launch argument of the captured steps
+        for (Group &g : b.groups) g.amp_stale = true;
+    }
+    if (nfp64) {
+        QL_RC(prime_split_all(b));
+        int n = 0;
+        for (Group &g : b.groups) {
+            std::vector<unsigned char> hi(g.modes);
+            QL_HIP(hipMemcpyAsync(hi.data(), g.hi, g.modes, hipMemcpyDeviceToHost, b.s));
+            QL_HIP(hipStreamSynchronize(b.s));
+            for (int i = 0; i < g.modes; ++i) n += hi[i] ? 1 : 0;
+        }
+        *nfp64 = n;
+    }
     return QL_OK;
 }
 

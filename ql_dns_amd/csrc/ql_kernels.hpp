@@ -56,12 +56,19 @@ struct RunParams {
     int nxy;             // modes of the run
     int mode_off;        // first mode of the run in the group arrays
     int wg_off, nwg;     // its mode-kernel workgroups = its partial rows
+    int wg_off2, nwg2;   // its rows of the FP64 companion launch (mixed precision), after them
     double kmax;         // CFL wavenumber (MHD_FullQlin.cpp:92-97)
     double cfl, dtmax;   // CFL number and -dt of variable steps (RK3CN.cpp:20-30)
     double Lz;
     int hd;              // HD_fullU run (MHD_FullQlin kernels with b = B = 0)
     int pad;
 };
+
+// row r of a run's partial rows: its mode-kernel rows, then its companion rows
+__device__ __forceinline__ int run_row(const RunParams &rp, int r)
+{
+    return r < rp.nwg ? rp.wg_off + r : rp.wg_off2 + (r - rp.nwg);
+}
 
 // one mode-kernel workgroup: its run, its mode-group range [g0, g1), and the
 // run's first mode and mode count (one record, one scalar load)
@@ -75,7 +82,7 @@ struct WgInfo {
 // taken, dt_last the one taken last
 struct Clock {
     double t, t_prev, t_init;
-    int old_init, pad;
+    int old_init, nstep;  // nstep: steps since the integrator was set up
     double dt, dt_last;
 };
 
@@ -101,7 +108,23 @@ struct ModeArgs {
     double dt, linC, nlc1, nlc2;
     double dtP1, dtP2, dtl1, dtl2, dtl3;  // dt*8/15, dt*2/3, and the three dt_lin
     double fac;             // 1/NZ
+    // mixed-precision split (split == SPLIT_OFF: not used): a mode flagged in
+    // hi (its amplitude reached a fraction of its run's largest when the flags
+    // were last set, k_amp_flags) runs in FP64, the others in FP32: the FP32
+    // launch (SPLIT_LO) skips the flagged modes and an FP64 companion launch
+    // (SPLIT_HI) takes only them
+    const unsigned char *hi;  // [modes]
+    int split;
 };
+
+enum { SPLIT_OFF = 0, SPLIT_LO = 1, SPLIT_HI = 2 };
+constexpr int COMP_LIST = 512;  // candidate modes per companion workgroup, at most
+
+// max(|re|, |im|) of two complex values, as a float
+__device__ __forceinline__ float amp2(double2 x, double2 y)
+{
+    return (float)fmax(fmax(fabs(x.x), fabs(x.y)), fmax(fabs(y.x), fabs(y.y)));
+}
 
 // substep coefficients of RK3CN_Elin::Step (RK3CN_Elin.cpp:88-90, :118-121,
 // :151-154) and RK3CN::Step (linC with dt/2, RK3CN.cpp:92-95, :130-133,
@@ -232,6 +255,7 @@ __device__ __forceinline__ void clock_advance(Clock *c)
     c->t = rn_add(t, dt);
     c->dt_last = dt;
     c->old_init = 0;
+    c->nstep += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -251,16 +275,16 @@ constexpr int RED_CW = 16;  // columns per block (4, 8 and 32 measured slower)
     const RunParams &rp = runs[blockIdx.y];
     const int cl = threadIdx.x % CW, chunk = threadIdx.x / CW;
     const int c = blockIdx.x * CW + cl;
-    const int nrows = rp.nwg;
+    const int nrows = rp.nwg + rp.nwg2;
     const int per = (nrows + CH - 1) / CH;
-    const int r0 = rp.wg_off + chunk * per, r1 = rp.wg_off + min(nrows, chunk * per + per);
+    const int r0 = chunk * per, r1 = min(nrows, chunk * per + per);
     constexpr int B = 256 / CH;  // rows in flight per thread (one batch up to 256 rows)
     double s = 0.0, e = 0.0;
     if (c < ncols) {
         for (int r = r0; r < r1; r += B) {
             double v[B];
 #pragma unroll
-            for (int k = 0; k < B; ++k) v[k] = (r + k < r1) ? part[(size_t)(r + k) * ncols + c] : 0.0;
+            for (int k = 0; k < B; ++k) v[k] = (r + k < r1) ? part[(size_t)run_row(rp, r + k) * ncols + c] : 0.0;
 #pragma unroll
             for (int k = 0; k < B; ++k) {
                 const double t = s + v[k];
@@ -283,6 +307,44 @@ constexpr int RED_CW = 16;  // columns per block (4, 8 and 32 measured slower)
         }
         sums[(size_t)blockIdx.y * ncols + c] = t + te;
     }
+}
+
+// mixed precision: every `every` steps of the clock (always when clock is
+// null), max |value| of each mode's state, one wavefront per mode
+[[maybe_unused]] static __global__ __launch_bounds__(64) void k_amp_modes(const double2 *__restrict__ u, int nz,
+                                                                          float *__restrict__ amp,
+                                                                          const Clock *__restrict__ clock, int every)
+{
+    if (clock && clock->nstep % every != 0) return;
+    const size_t i = blockIdx.x;
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < 2 * nz; k += 64) m = fmaxf(m, amp2(u[i * 4 * nz + k], u[i * 4 * nz + 2 * nz + k]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (threadIdx.x == 0) amp[i] = m;
+}
+
+// then, per run (one workgroup each), its largest amplitude and the flag of
+// each mode that reaches tau times it (the modes the FP64 companion takes)
+constexpr int AMP_NT = 1024;
+[[maybe_unused]] static __global__ __launch_bounds__(AMP_NT) void k_amp_flags(const float *__restrict__ amp,
+                                                                             const RunParams *__restrict__ runs,
+                                                                             double tau, unsigned char *__restrict__ hi,
+                                                                             const Clock *__restrict__ clock, int every)
+{
+    if (clock && clock->nstep % every != 0) return;
+    __shared__ float red[AMP_NT / 64];
+    const RunParams &rp = runs[blockIdx.x];
+    const int tid = threadIdx.x;
+    float m = 0.0f;
+    for (int l = tid; l < rp.nxy; l += AMP_NT) m = fmaxf(m, amp[rp.mode_off + l]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (tid % 64 == 0) red[tid / 64] = m;
+    __syncthreads();
+    float top = 0.0f;
+#pragma unroll
+    for (int w = 0; w < AMP_NT / 64; ++w) top = fmaxf(top, red[w]);
+    const double thr = top > 0.0f ? tau * (double)top : __builtin_inf();
+    for (int l = tid; l < rp.nxy; l += AMP_NT) hi[rp.mode_off + l] = (double)amp[rp.mode_off + l] >= thr ? 1 : 0;
 }
 
 // sum over the workgroup's threads in a fixed order (every thread gets it)

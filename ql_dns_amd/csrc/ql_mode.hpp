@@ -403,7 +403,7 @@ struct CompactDst {
 // threads of a mode-kernel workgroup: one per z point of a pass
 constexpr int mode_threads(int NZ, int MB) { return NZ * MB < NT ? NZ * MB : NT; }
 
-template <int NZ, int MB, class P, class C, int MODE>
+template <int NZ, int MB, class P, class C, int MODE, bool SPL>
 __device__ __forceinline__ void mode_body(const ModeArgs &a)
 {
     typedef RealOf<C> R;
@@ -468,8 +468,45 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         for (int w = 0; w < MFN; ++w) mfv[w] = (tid + w * TPB < NZ * P::NMFZ) ? mfz[tid + w * TPB] : 0.0;
     }
 
+    // the FP64 companion of a mixed-precision launch (SPLIT_HI): workgroup g0
+    // of the g1 its run has takes those of the run's modes l = g0 + k g1 whose
+    // amplitude reaches the threshold, listed in mode order (the host sizes
+    // g1 so that no workgroup has more than COMP_LIST candidates)
+    const bool comp = SPL && a.split == SPLIT_HI;
+    __shared__ int slist[SPL ? COMP_LIST : 1];
+    __shared__ int scnt[SPL ? (TPB + 63) / 64 : 1];
+    int ncomp = 0;
+    if constexpr (SPL) {
+        if (comp) {
+            constexpr int NWV = (TPB + 63) / 64;
+            const int wv = tid / 64, ln = tid % 64;
+            int base = 0;
+            for (int c0 = 0; g0 + c0 * g1 < nxy; c0 += TPB) {
+                const int l = g0 + (c0 + tid) * g1;
+                const bool hi = l < nxy && a.hi[mode_off + l];
+                const unsigned long long bal = __ballot(hi);
+                if (ln == 0) scnt[wv] = __popcll(bal);
+                __syncthreads();
+                int off = base, tot = 0;
+#pragma unroll
+                for (int w = 0; w < NWV; ++w) {
+                    off += w < wv ? scnt[w] : 0;
+                    tot += scnt[w];
+                }
+                if (hi) slist[off + __popcll(bal & ((1ull << ln) - 1ull))] = l;
+                base += tot;
+                __syncthreads();
+            }
+            ncomp = base;
+        }
+    }
+    const int gbeg = comp ? 0 : g0, gend = comp ? (ncomp + MB - 1) / MB : g1;
     // run-local mode index of this thread's item in group g, or -1
     auto local_mode = [&](int g) {
+        if (comp) {
+            const int k = g * MB + imb;
+            return (has_it && g < gend && k < ncomp) ? slist[k] : -1;
+        }
         const int l = g * MB + imb;
         return (has_it && g < g1 && l < nxy) ? l : -1;
     };
@@ -478,6 +515,9 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     constexpr bool AHEAD = !(sizeof(C) == 16 && NZ >= 256 && QL_NZ256_FETCH_AT_USE);
     double2 upre[4];
     double kpre[2];
+    // SPL: the mode's precision flag, fetched with its state and tested where
+    // the state is used (a test here would wait for the whole prefetch)
+    unsigned char hpre = 0;
     auto fetch = [&](int g) {
         const int l = local_mode(g);
         const int i = mode_off + (l < 0 ? 0 : l);
@@ -487,8 +527,9 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
             upre[f] = (l >= 0 && f < 2 * P::NPAIR) ? a.u[off + f * NZD] : make_double2(0.0, 0.0);
         kpre[0] = (l >= 0) ? a.kx0[i] : 0.0;
         kpre[1] = (l >= 0) ? a.ky[i] : 0.0;
+        if constexpr (SPL) hpre = (l >= 0 && a.split == SPLIT_LO) ? a.hi[i] : 0;
     };
-    if (AHEAD) fetch(g0);
+    if (AHEAD) fetch(gbeg);
 #pragma unroll
     for (int w = 0; w < TWN; ++w)
         if (tid + w * TPB < NZ) tw[tid + w * TPB] = twv[w];
@@ -502,15 +543,17 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 #pragma unroll
     for (int c = 0; c < P::NACC; ++c) acc[c] = 0.0;
 
-    for (int g = g0; g < g1; ++g) {
+    for (int g = gbeg; g < gend; ++g) {
         // B: state of the item (prefetched), next group's state, previous RHS;
         // kx(t), 1/lap2, 1/lapF (assign_laplacians_, MHD_FullQlin.cpp:722-738,
         // Kdata.cpp:100-120) and the base spectral fields
-        const int l = local_mode(g);
+        int l = local_mode(g);
         if (!AHEAD) fetch(g);
         const double2 B0 = upre[0], B1 = upre[1], B2 = upre[2], B3 = upre[3];
         const double kx0 = kpre[0], ky = kpre[1];
+        const unsigned char hcur = hpre;
         if (AHEAD) fetch(g + 1);
+        if (SPL && hcur) l = -1;  // the FP64 companion takes this mode
         const size_t off = (size_t)(mode_off + (l < 0 ? 0 : l)) * 4 * NZD + im;
         double kx = 0.0, ilapF = 1.0;
         if (has_it) {
@@ -677,13 +720,20 @@ template <int NZ, int MB, class P, int MODE>
 __global__ __launch_bounds__(mode_threads(NZ, MB)) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES64_HALF : QL_WAVES64)))
 void k_mode64(ModeArgs a)
 {
-    mode_body<NZ, MB, P, double2, MODE>(a);
+    mode_body<NZ, MB, P, double2, MODE, false>(a);
+}
+// the FP64 companion of a mixed-precision launch (ModeArgs::split)
+template <int NZ, int MB, class P, int MODE>
+__global__ __launch_bounds__(mode_threads(NZ, MB)) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES64_HALF : QL_WAVES64)))
+void k_mode64s(ModeArgs a)
+{
+    mode_body<NZ, MB, P, double2, MODE, true>(a);
 }
 template <int NZ, int MB, class P, int MODE>
 __global__ __launch_bounds__(mode_threads(NZ, MB)) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES32_HALF : QL_WAVES32)))
 void k_mode32(ModeArgs a)
 {
-    mode_body<NZ, MB, P, float2, MODE>(a);
+    mode_body<NZ, MB, P, float2, MODE, true>(a);
 }
 
 }  // namespace qlk

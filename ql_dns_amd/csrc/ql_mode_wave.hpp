@@ -124,7 +124,7 @@ struct WaveCompactDst {
     }
 };
 
-template <int NZ, class P, class C, int MODE, int NW>
+template <int NZ, class P, class C, int MODE, int NW, bool SPL>
 __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
 {
     typedef RealOf<C> R;
@@ -188,9 +188,25 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     __syncthreads();
 
     // the workgroup's modes (run-local l in [g0, g1), one per pass), dealt
-    // round robin to its wavefronts
+    // round robin to its wavefronts; with the mixed-precision split
+    // (SPLIT_LO) the FP64 companion takes the modes whose amplitude reaches
+    // the threshold
+    unsigned long long skip = 0ull;  // SPL: which of the chunk's 64 modes the companion takes
+    int cbase = g0 + wave;
     for (int l = g0 + wave; l < g1; l += NW) {
         const int i = mode_off + l;
+        if constexpr (SPL) {
+            if (a.split == SPLIT_LO) {
+                // the flags of this wavefront's next 64 modes, one per lane in
+                // one round trip, then read from a ballot
+                if ((l - g0 - wave) % (64 * NW) == 0) {
+                    const int lq = l + lane * NW;
+                    skip = __ballot(lq < g1 && a.hi[mode_off + lq]);
+                    cbase = l;
+                }
+                if ((skip >> ((l - cbase) / NW)) & 1ull) continue;
+            }
+        }
         const double kx0 = to_sgpr(a.kx0[i]), ky = to_sgpr(a.ky[i]);
         const double kx = to_sgpr(rn_add(kx0, rn_mul(qt, ky)));  // kx(t), MHD_FullQlin.cpp:727
         const size_t row = (size_t)i * 4 * NZD;
@@ -387,13 +403,13 @@ template <int NZ, class P, int MODE>
 __global__ __launch_bounds__(mode_waves<P>(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ<P>(NZ))))
 void k_wave64(ModeArgs a)
 {
-    mode_wave_body<NZ, P, double2, MODE, mode_waves<P>(NZ)>(a);
+    mode_wave_body<NZ, P, double2, MODE, mode_waves<P>(NZ), false>(a);
 }
 template <int NZ, class P, int MODE>
 __global__ __launch_bounds__(mode_waves<P>(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ<P>(NZ))))
 void k_wave32(ModeArgs a)
 {
-    mode_wave_body<NZ, P, float2, MODE, mode_waves<P>(NZ)>(a);
+    mode_wave_body<NZ, P, float2, MODE, mode_waves<P>(NZ), true>(a);
 }
 
 }  // namespace qlk

@@ -97,12 +97,35 @@ struct Group {
     qlk::WgInfo *wg = nullptr;
     qlk::Clock *clock = nullptr;  // the group's step times
     std::vector<qlk::RunParams> h_rp;
+    // mixed precision: the FP64 companion launch of the modes that carry the
+    // norm (ModeArgs::split)
+    bool split = false;
+    int W2 = 0;                      // companion workgroups (rows after the W)
+    qlk::WgInfo *wg2 = nullptr;      // companion table
+    float *amp = nullptr;            // [modes] mode amplitudes when the flags were last set
+    unsigned char *hi = nullptr;     // [modes] the modes the companion takes
+    bool amp_stale = true;           // a state was uploaded (or the fraction changed) since the flags were set
 };
+
+// mixed precision: a mode runs in FP64 when its amplitude reaches this
+// fraction of its run's largest (measured at 128x128x256 over 2000 steps: 29
+// of 8128 modes in FP64 and the mean fields, drives and energies within 3e-9
+// of the FP64 run, against 1.7e-6 with every mode in FP32); the flags are
+// set again every SPLIT_EVERY steps of the clock (a mode's amplitude changes
+// by a few per cent over that many steps) and after a state upload.  Remaps
+// and driving noise leave them: a zeroed mode gives zeros in either
+// precision, and the noise is small against the modes that set the flags
+constexpr double MIXED_AMP_TAU = 0.03;
+constexpr int SPLIT_EVERY = 16;
+// companion launch: about this many workgroups per run
+constexpr int COMP_TARGET_WG = 128;
 
 struct AdjState;  // adjoint checkpoints and buffers (ql_adjoint.hip)
 
 struct Batch {
     int model = 0, integrator = 0, precision = 0, device = 0, nMF = 4;
+    double amp_tau = MIXED_AMP_TAU;  // mixed precision: FP64 from this fraction of a run's largest mode
+    int nstep = 0;                   // steps since the integrator was set up (the device clock's count)
     // HD_fullU runs on the MHD_FullQlin kernels with the magnetic fields zero;
     // at the boundary they have 2 fluctuation fields (u, zeta) and 2 mean
     // fields (Ux, Uy, internal slots 2 and 3)
