@@ -312,6 +312,9 @@ def main():
                     help="arithmetic of the main workload (mixed: FP32 transforms; for profiling)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shape (profiling runs: only steady-state launches)")
+    ap.add_argument("--transform", choices=["fft", "dft"], default=None,
+                    help="z transforms of the workgroup kernel: LDS radix passes or matrix-core products "
+                         "(NZ 32 and 64, FP64; implies --no-tune and line 128)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -336,7 +339,10 @@ def main():
     lin, mf = initial_state(p)
     ctx = QLContext(p)
     ctx.set_state(lin, mf)
-    if not args.no_tune:
+    if args.transform:
+        ctx.set_config(p.NZ, 128, 1024)
+        ctx.set_transform(p.NZ, 1 if args.transform == "dft" else 0)
+    elif not args.no_tune:
         ctx.tune(3)  # launch shape; restores the state
     ctx.linear_ops_init(0.0)
     t = ctx.steps(args.warmup, 0.0)
@@ -363,7 +369,7 @@ def main():
     bits = 64 if args.precision == "fp64" else 32
     pol = {"MHD_FullQlin": "ModelFull", "MHD_BQlin": "ModelBQ", "HD_fullU": "ModelHD"}[args.equations]
     kname = (f"k_wave{bits}<{p.NZ},{pol},{elin}>" if shape.get("line") == 64
-             else f"k_mode{bits}<{p.NZ},{shape.get('modes_per_pass')},{pol},{elin}>")
+             else f"k_mode{bits}{'d' if shape.get('transform') else ''}<{p.NZ},{shape.get('modes_per_pass')},{pol},{elin}>")
     out = {
         "metric": metric_name(p, args.workload, args.equations, args.integrator, args.precision),
         "value": steps_per_s,

@@ -239,6 +239,13 @@ int ql_batch_steps(ql_batch *b, int nsteps, double *t_inout);
  * The shape sets the summation order of the stress reduction: results of two
  * shapes agree to rounding, results of one shape are bitwise reproducible. */
 int ql_batch_tune(ql_batch *b, int trial_steps, double *step_ms);
+/* Transform of the workgroup mode kernel for the NZ group (fftwPlans.h:33-97):
+ * the LDS radix passes (QL_TRANSFORM_FFT, the default) or dense complex
+ * matrix products on the matrix cores (QL_TRANSFORM_DFT_MFMA: FP64, NZ 32 or
+ * 64, line 128 or 256, else QL_ERR_ARG).  ql_batch_tune times both where both
+ * exist.  A later ql_batch_set_config keeps it where the new shape has it. */
+enum { QL_TRANSFORM_FFT = 0, QL_TRANSFORM_DFT_MFMA = 1 };
+int ql_batch_set_transform(ql_batch *b, int NZ, int transform);
 /* Mixed precision (QL_PREC_MIXED): a mode whose largest |value| reaches tau
  * times the largest of its run runs in FP64 (a companion launch of the FP64
  * kernel before the FP32 one), the others in FP32; the choice is made again
@@ -246,8 +253,9 @@ int ql_batch_tune(ql_batch *b, int trial_steps, double *step_ms);
  * tau > 0 (default 0.03; tau <= 0 leaves it) and writes to *nfp64 (may be
  * NULL) how many modes of the set reach it for the current state. */
 int ql_batch_mixed_split(ql_batch *b, double tau, int *nfp64);
-/* current shape per group: cfg[5k..5k+4] = NZ, line, target, modes per
- * workgroup pass, workgroups; *ngroups receives the group count */
+/* current shape per group: cfg[6k..6k+5] = NZ, line, target, modes per
+ * workgroup pass, workgroups, transform (QL_TRANSFORM_*); *ngroups receives
+ * the group count */
 int ql_batch_get_config(const ql_batch *b, int max_groups, int *ngroups, int *cfg);
 /* sets the shape of the NZ group: line in {64, 128, 256}, target >= 1.
  * line 64 selects the kernel with one wavefront per mode (modes per pass 1;

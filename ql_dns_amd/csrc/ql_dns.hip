@@ -211,6 +211,7 @@ int configure(Batch &b, Group &g, int line, int target)
 {
     if (line == LINE_WAVE && !wave_ok(g.NZ, b.precision, b.hd))
         return fail(QL_ERR_ARG, "line %d (one wavefront per mode) at NZ = 256 needs FP64 (or HD_fullU)", LINE_WAVE);
+    if (g.dft && !dft_ok(g.NZ, b.precision, line)) g.dft = 0;  // the shape has no matrix-core variant
     // the tables are rewritten below: nothing queued may still read them
     QL_RC(sync_all(b));
     const int MB = mb_for(g.NZ, line);
@@ -417,6 +418,12 @@ void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
         if (b.hd) qlk::k_mode32<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
         else if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
         else qlk::k_mode32<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
+    } else if (g.dft) {
+        if constexpr (NZ <= 64) {  // dft_ok
+            if (b.hd) qlk::k_mode64d<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
+            else if (full) qlk::k_mode64d<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
+            else qlk::k_mode64d<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
+        }
     } else {
         if (b.hd) qlk::k_mode64<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
         else if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
@@ -1099,13 +1106,16 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
     for (size_t gi = 0; gi < b.groups.size() && rc == QL_OK; ++gi) {
         Group &g = b.groups[gi];
         double best = 1e300;
-        int best_line = g.line, best_target = g.target;
+        int best_line = g.line, best_target = g.target, best_dft = g.dft;
         std::vector<std::pair<int, int>> seen;  // (kernel and MB, W) already timed
-        for (int line : LINES) {
+        for (int cand = 0; cand < 2 * (int)(sizeof(LINES) / sizeof(LINES[0])); ++cand) {
+            const int line = LINES[cand / 2], dft = cand % 2;
             if (line == LINE_WAVE && !wave_ok(g.NZ, b.precision, b.hd)) continue;
+            if (dft && !dft_ok(g.NZ, b.precision, line)) continue;
             for (int target : TARGETS) {
+                g.dft = dft;
                 if ((rc = configure(b, g, line, target))) break;
-                const int kind = line == LINE_WAVE ? -1 : g.MB;
+                const int kind = line == LINE_WAVE ? -1 : g.MB + 100 * dft;
                 bool dup = false;
                 for (auto &sw : seen) dup = dup || (sw.first == kind && sw.second == g.W);
                 if (dup) continue;
@@ -1130,10 +1140,12 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
                     best = per;
                     best_line = line;
                     best_target = target;
+                    best_dft = dft;
                 }
             }
             if (rc) break;
         }
+        g.dft = best_dft;
         if (rc == QL_OK) rc = configure(b, g, best_line, best_target);
         total_best += best;
     }
@@ -1684,6 +1696,24 @@ int ql_batch_driving_noise(ql_batch *bt, double t, double dt)
     return QL_OK;
 }
 
+int ql_batch_set_transform(ql_batch *bt, int NZ, int transform)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_set_transform: null handle");
+    Batch &b = bt->b;
+    for (Group &g : b.groups) {
+        if (g.NZ != NZ) continue;
+        if (transform != QL_TRANSFORM_FFT && transform != QL_TRANSFORM_DFT_MFMA)
+            return fail(QL_ERR_ARG, "ql_batch_set_transform: unknown transform %d", transform);
+        if (transform == QL_TRANSFORM_DFT_MFMA && !dft_ok(g.NZ, b.precision, g.line))
+            return fail(QL_ERR_ARG,
+                        "ql_batch_set_transform: the matrix-core transform needs FP64, NZ 32 or 64 and line 128 "
+                        "or 256 (NZ %d, line %d)", g.NZ, g.line);
+        g.dft = transform;
+        return configure(b, g, g.line, g.target);
+    }
+    return fail(QL_ERR_ARG, "ql_batch_set_transform: no group with NZ = %d", NZ);
+}
+
 int ql_batch_mixed_split(ql_batch *bt, double tau, int *nfp64)
 {
     if (!bt) return fail(QL_ERR_ARG, "ql_batch_mixed_split: null handle");
@@ -1723,11 +1753,12 @@ int ql_batch_get_config(const ql_batch *bt, int max_groups, int *ngroups, int *c
     if (cfg)
         for (int k = 0; k < max_groups && k < (int)b.groups.size(); ++k) {
             const Group &g = b.groups[k];
-            cfg[5 * k + 0] = g.NZ;
-            cfg[5 * k + 1] = g.line;
-            cfg[5 * k + 2] = g.target;
-            cfg[5 * k + 3] = g.MB;
-            cfg[5 * k + 4] = g.W;
+            cfg[6 * k + 0] = g.NZ;
+            cfg[6 * k + 1] = g.line;
+            cfg[6 * k + 2] = g.target;
+            cfg[6 * k + 3] = g.MB;
+            cfg[6 * k + 4] = g.W;
+            cfg[6 * k + 5] = g.dft;
         }
     return QL_OK;
 }

@@ -23,6 +23,7 @@
 //   F  RHS assembly, dissipation operator at the substep end, substep update
 //      per (mode, kz slot, field pair), remap predicate on the new state.
 #pragma once
+#include "ql_dft.hpp"
 #include "ql_kernels.hpp"
 
 // occupancy requests (waves per SIMD, which bound the VGPR budget) for
@@ -389,6 +390,30 @@ struct LineSrcI {
     }
 };
 
+// slot-m input of line `line` (interleaved layout) for the matrix-core
+// inverse transform (ql_dft.hpp)
+template <int NZ, int MB, class P>
+struct SlotSrc {
+    const double2 *sbase;  // [(base * MB + mb)][NZDP]
+    const double *skz;
+    __device__ double2 operator()(int line, int m) const
+    {
+        const int l = line / MB, mb = line - l * MB;
+        const double2 v = sbase[(P::base_of(l) * MB + mb) * GroupLayout<NZ, MB, P>::NZDP + m];
+        return P::dz_of(l) ? imul(skz[m], v) : v;
+    }
+};
+
+// band sink of the matrix-core forward transform: [line][nz], one part
+struct SlotDst {
+    double2 *ct;
+    int nz;
+    __device__ void operator()(int line, int m, int part, double v) const
+    {
+        reinterpret_cast<double *>(&ct[line * nz + m])[part] = v;
+    }
+};
+
 // last-pass sink of the forward transforms: keep the dealiased band, compact
 template <int NZ, class C>
 struct CompactDst {
@@ -403,7 +428,7 @@ struct CompactDst {
 // threads of a mode-kernel workgroup: one per z point of a pass
 constexpr int mode_threads(int NZ, int MB) { return NZ * MB < NT ? NZ * MB : NT; }
 
-template <int NZ, int MB, class P, class C, int MODE, bool SPL>
+template <int NZ, int MB, class P, class C, int MODE, bool SPL, bool DFT = false>
 __device__ __forceinline__ void mode_body(const ModeArgs &a)
 {
     typedef RealOf<C> R;
@@ -573,7 +598,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
         __syncthreads();
 
         // C: inverse transforms of the padded lines
-        fft2_inv_from<NZ, L::NL, L::LS, TPB>(buf, tw, LineSrcI<NZ, MB, P, C>{sbase, skzr});
+        if constexpr (DFT) dft_inv_mfma<NZ, L::NL, L::LS, TPB>(buf, tw, SlotSrc<NZ, MB, P>{sbase, skzr});
+        else fft2_inv_from<NZ, L::NL, L::LS, TPB>(buf, tw, LineSrcI<NZ, MB, P, C>{sbase, skzr});
 
         // D: products on the z grid, in place, and the EMF / stress sums
         if (has_z) {
@@ -613,7 +639,8 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
 
         // E: forward transforms of the product lines, dealiased band kept
         C *ct = buf + L::CT_OFF;  // [line][nz]
-        fft2_fwd_to<NZ, L::NLO, L::LS, TPB>(buf, tw, CompactDst<NZ, C>{ct});
+        if constexpr (DFT) dft_fwd_mfma<NZ, L::NLO, L::LS, TPB>(buf, tw, SlotDst{ct, NZD});
+        else fft2_fwd_to<NZ, L::NLO, L::LS, TPB>(buf, tw, CompactDst<NZ, C>{ct});
 
         // F: RHS, dissipation operator at the substep end, update
         if (l >= 0) {
@@ -721,6 +748,14 @@ __global__ __launch_bounds__(mode_threads(NZ, MB)) __attribute__((amdgpu_waves_p
 void k_mode64(ModeArgs a)
 {
     mode_body<NZ, MB, P, double2, MODE, false>(a);
+}
+// FP64 with the dealiased transforms as matrix-core products (ql_dft.hpp;
+// NZ = 32 and 64)
+template <int NZ, int MB, class P, int MODE>
+__global__ __launch_bounds__(mode_threads(NZ, MB)) __attribute__((amdgpu_waves_per_eu(half_pass(NZ, MB) ? QL_WAVES64_HALF : QL_WAVES64)))
+void k_mode64d(ModeArgs a)
+{
+    mode_body<NZ, MB, P, double2, MODE, false, true>(a);
 }
 // the FP64 companion of a mixed-precision launch (ModeArgs::split)
 template <int NZ, int MB, class P, int MODE>

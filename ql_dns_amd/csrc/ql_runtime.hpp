@@ -64,6 +64,11 @@ inline int default_line(int NZ, int precision, bool hd)
     return (NZ >= 128 && wave_ok(NZ, precision, hd)) ? LINE_WAVE : DEFAULT_LINE;
 }
 inline int default_target(int line) { return line == LINE_WAVE ? 256 : DEFAULT_TARGET; }
+// the matrix-core transform (QL_TRANSFORM_DFT_MFMA): FP64 workgroup kernel, NZ 32 and 64
+inline bool dft_ok(int NZ, int precision, int line)
+{
+    return precision == QL_PREC_FP64 && (NZ == 32 || NZ == 64) && line != LINE_WAVE;
+}
 
 // one run: parameters, sizes and its place in its group
 struct RunInfo {
@@ -80,6 +85,7 @@ struct Group {
     int NZ = 0, nz = 0, nruns = 0, modes = 0;
     std::vector<int> run_ids;
     int line = DEFAULT_LINE, target = DEFAULT_TARGET;
+    int dft = 0;  // 1: the workgroup kernel's transforms as matrix-core products (ql_dft.hpp)
     int MB = 1, W = 0, part_cap = 0, wg_cap = 0;
     hipStream_t s = nullptr;  // own stream (sets with several groups)
     hipEvent_t join = nullptr;
