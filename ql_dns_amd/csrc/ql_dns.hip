@@ -40,9 +40,10 @@ int fail(int code, const char *fmt, ...)
 
 void drop_graphs(Batch &b)
 {
-    if (b.g1) (void)hipGraphExecDestroy(b.g1);
-    if (b.gK) (void)hipGraphExecDestroy(b.gK);
-    b.g1 = b.gK = nullptr;
+    for (hipGraphExec_t &g : b.graphs) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 void release(Batch &b)
@@ -830,9 +831,9 @@ int capture(Batch &b, int nsteps, hipGraphExec_t *out)
 }
 
 // nsteps of the driver loop with remap as each run asks; graph replay unless
-// per-kernel timing is on.  Both replay graphs are captured by the first call
-// that steps, whatever its step count, so that later calls (a timed loop after
-// a warm-up) only launch.
+// per-kernel timing is on.  The replay graphs (16, 8, 4, 2 and 1 steps) are
+// all captured by the first call that steps, whatever its step count, so that
+// later calls (a timed loop after a warm-up) only launch.
 int run_steps(Batch &b, int nsteps, double *t_inout)
 {
     double t = *t_inout;
@@ -841,11 +842,11 @@ int run_steps(Batch &b, int nsteps, double *t_inout)
     if (b.timing) {
         for (int n = 0; n < nsteps; ++n) QL_RC(enqueue_step(b, true));
     } else if (nsteps > 0) {
-        if (!b.gK) QL_RC(capture(b, GRAPH_STEPS, &b.gK));
-        if (!b.g1) QL_RC(capture(b, 1, &b.g1));
+        for (int k = 0; k < 5; ++k)
+            if (!b.graphs[k]) QL_RC(capture(b, GRAPH_LEVELS[k], &b.graphs[k]));
         int left = nsteps;
-        for (; left >= GRAPH_STEPS; left -= GRAPH_STEPS) QL_HIP(hipGraphLaunch(b.gK, b.s));
-        for (; left > 0; --left) QL_HIP(hipGraphLaunch(b.g1, b.s));
+        for (int k = 0; k < 5; ++k)
+            for (; left >= GRAPH_LEVELS[k]; left -= GRAPH_LEVELS[k]) QL_HIP(hipGraphLaunch(b.graphs[k], b.s));
     }
     QL_HIP(hipStreamSynchronize(b.s));
     if (b.vdt) {

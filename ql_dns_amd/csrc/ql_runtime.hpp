@@ -40,6 +40,9 @@ constexpr double QL_PI = 3.14159265358979323846;
 
 // steps captured in the long replay graph
 constexpr int GRAPH_STEPS = 16;
+// step counts of the replay graphs: any step count is at most 4 launches past
+// its multiples of GRAPH_STEPS
+constexpr int GRAPH_LEVELS[5] = {GRAPH_STEPS, 8, 4, 2, 1};
 // launch-shape candidates of the tuner: z points per workgroup pass
 // (modes per workgroup MB = line / NZ) and the target workgroup count; line
 // LINE_WAVE selects the kernel with one wavefront per mode (MB = 1,
@@ -148,8 +151,8 @@ struct Batch {
     int old_init = 1;
     // diagnostics scratch
     double *dpart = nullptr, *dout = nullptr, *mfr_tmp = nullptr;
-    // replay graphs of 1 and GRAPH_STEPS steps (both captured by the first stepping call)
-    hipGraphExec_t g1 = nullptr, gK = nullptr;
+    // replay graphs of GRAPH_LEVELS[k] steps (all captured by the first stepping call)
+    hipGraphExec_t graphs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     // per-kernel timing (ql_profile_kernels)
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
