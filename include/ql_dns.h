@@ -229,8 +229,14 @@ int ql_batch_set_noise(ql_batch *b, int run, const ql_noise *cfg);
 int ql_batch_driving_noise(ql_batch *b, double t, double dt);
 /* integrator priming at t0 of every run (RK3CN.cpp:57, RK3CN_Elin.cpp:257) */
 int ql_batch_linear_ops_init(ql_batch *b, double t0);
-/* nsteps of main.cpp:108-123 for every run (as ql_steps), one clock */
+/* nsteps of main.cpp:108-123 for every run (as ql_steps), one clock; a set
+ * of several runs with variable steps (dt < 0) takes ql_batch_steps_runs */
 int ql_batch_steps(ql_batch *b, int nsteps, double *t_inout);
+/* nsteps for every run with each run's own time, t_runs[nruns] read and
+ * updated: with variable steps each run chooses its steps from its own CFL
+ * condition (RK3CN.cpp:79-83) on its own device clock; with fixed steps the
+ * times must be equal (one shared clock) */
+int ql_batch_steps_runs(ql_batch *b, int nsteps, double *t_runs);
 /* Launch-shape tuner: for each NZ group, times every candidate shape (z
  * points per workgroup pass 128 or 256, or one wavefront per mode (64), and
  * a target workgroup count) over

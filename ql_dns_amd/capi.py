@@ -26,7 +26,7 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward", "ql_calculate_cfl",
            "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution", "ql_set_noise",
            "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise", "ql_drive_real",
-           "ql_rhs_noalloc", "ql_batch_mixed_split", "ql_batch_set_transform")
+           "ql_rhs_noalloc", "ql_batch_mixed_split", "ql_batch_set_transform", "ql_batch_steps_runs")
 
 QL_ERR_STATE = -4
 
@@ -101,6 +101,7 @@ def load(path: str = LIB_PATH):
         "ql_batch_get_drive": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_batch_linear_ops_init": ([P, ctypes.c_double], ctypes.c_int),
         "ql_batch_steps": ([P, ctypes.c_int, D], ctypes.c_int),
+        "ql_batch_steps_runs": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_batch_tune": ([P, ctypes.c_int, D], ctypes.c_int),
         "ql_batch_get_config": ([P, ctypes.c_int, I, I], ctypes.c_int),
         "ql_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -393,6 +394,13 @@ class QLBatch:
 
     def linear_ops_init(self, t0: float = 0.0):
         self._check(self._L.ql_batch_linear_ops_init(self._h, t0))
+
+    def steps_runs(self, n: int, t_runs) -> np.ndarray:
+        """n steps with each run's own time (ql_batch_steps_runs); returns the
+        runs' times after them."""
+        tt = np.ascontiguousarray(t_runs, dtype=np.float64).copy()
+        self._check(self._L.ql_batch_steps_runs(self._h, n, _dp(tt)))
+        return tt
 
     def steps(self, n: int, t: float = 0.0) -> float:
         tt = ctypes.c_double(t)

@@ -313,6 +313,8 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
     b.vdt = p0.dt < 0 ? 1 : 0;
     b.dt = b.vdt ? -p0.dt : p0.dt;
     b.dt_last = b.dt;
+    b.rdt_last.assign(nruns, b.dt);
+    b.rt_prev.assign(nruns, 0.0);
     b.nMF = (p0.model == QL_MODEL_MHD_BQLIN) ? 2 : 4;
     b.runs.resize(nruns);
     std::vector<int> nzs;
@@ -368,7 +370,7 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_RC(dalloc(&g.pm, (size_t)g.nruns * qlk::PM_STRIDE));
         QL_RC(dalloc(&g.mode_run, g.modes));
         QL_RC(dalloc(&g.np, g.nruns));
-        QL_RC(dalloc(&g.clock, 1));
+        QL_RC(dalloc(&g.clock, g.nruns));
         g.split = b.precision == QL_PREC_MIXED;
         if (g.split) {
             QL_RC(dalloc(&g.amp, g.modes));
@@ -789,6 +791,7 @@ int enqueue_step(Batch &b, bool remap)
     return QL_OK;
 }
 
+// every run at time t, the step history shared (fixed steps, or one run)
 int set_clock(Batch &b, double t)
 {
     qlk::Clock c{};
@@ -799,16 +802,60 @@ int set_clock(Batch &b, double t)
     c.nstep = b.nstep;
     c.dt = b.dt;
     c.dt_last = b.dt_last;
-    for (Group &g : b.groups) qlk::k_clock_set<<<1, 64, 0, b.s>>>(g.clock, c, g.pm, g.rp, b.model, b.vdt);
+    for (Group &g : b.groups)
+        qlk::k_clock_fill<<<1, 256, 0, b.s>>>(g.clock, g.nruns, c, g.pm, g.rp, b.model, b.vdt);
     QL_HIP(hipGetLastError());
     return QL_OK;
 }
 
-// the host's view of the clock after variable steps
+// each run at its own time with its own step history (variable steps in a set
+// of several runs); the device then sets each run's next dt
+int set_clock_runs(Batch &b, const double *t_runs)
+{
+    for (Group &g : b.groups) {
+        std::vector<qlk::Clock> &h = g.h_clock;
+        h.resize(g.nruns);
+        for (int k = 0; k < g.nruns; ++k) {
+            const int r = g.run_ids[k];
+            qlk::Clock &c = h[k];
+            c = qlk::Clock{};
+            c.t = t_runs[r];
+            c.t_prev = b.rt_prev[r];
+            c.t_init = b.t_init;
+            c.old_init = b.old_init;
+            c.nstep = b.nstep;
+            c.dt = b.dt;
+            c.dt_last = b.rdt_last[r];
+        }
+        QL_HIP(hipMemcpyAsync(g.clock, h.data(), sizeof(qlk::Clock) * g.nruns, hipMemcpyHostToDevice, b.s));
+        qlk::k_clock_dt<<<1, 256, 0, b.s>>>(g.clock, g.nruns, g.pm, g.rp, b.model);
+        QL_HIP(hipGetLastError());
+    }
+    // the copies read g.h_clock: let them finish before it can change
+    QL_HIP(hipStreamSynchronize(b.s));
+    return QL_OK;
+}
+
+// the device clock of every run (run order)
+int read_clocks(Batch &b, std::vector<qlk::Clock> &out)
+{
+    out.resize(b.runs.size());
+    for (Group &g : b.groups) {
+        g.h_clock.resize(g.nruns);
+        QL_HIP(hipMemcpyAsync(g.h_clock.data(), g.clock, sizeof(qlk::Clock) * g.nruns, hipMemcpyDeviceToHost, b.s));
+    }
+    QL_HIP(hipStreamSynchronize(b.s));
+    for (Group &g : b.groups)
+        for (int k = 0; k < g.nruns; ++k) out[g.run_ids[k]] = g.h_clock[k];
+    return QL_OK;
+}
+
+// the host's view of run 0's clock after variable steps
 int read_clock(Batch &b, qlk::Clock *c)
 {
-    QL_HIP(hipMemcpyAsync(c, b.groups[0].clock, sizeof(qlk::Clock), hipMemcpyDeviceToHost, b.s));
-    QL_HIP(hipStreamSynchronize(b.s));
+    std::vector<qlk::Clock> all;
+    QL_RC(read_clocks(b, all));
+    *c = all[0];
     return QL_OK;
 }
 
@@ -834,10 +881,12 @@ int capture(Batch &b, int nsteps, hipGraphExec_t *out)
 // per-kernel timing is on.  The replay graphs (16, 8, 4, 2 and 1 steps) are
 // all captured by the first call that steps, whatever its step count, so that
 // later calls (a timed loop after a warm-up) only launch.
-int run_steps(Batch &b, int nsteps, double *t_inout)
+int run_steps(Batch &b, int nsteps, double *t_runs)
 {
-    double t = *t_inout;
-    QL_RC(set_clock(b, t));
+    const bool per_run = b.vdt && b.runs.size() > 1;
+    double t = t_runs[0];
+    if (per_run) QL_RC(set_clock_runs(b, t_runs));
+    else QL_RC(set_clock(b, t));
     QL_RC(prime_split_all(b));
     if (b.timing) {
         for (int n = 0; n < nsteps; ++n) QL_RC(enqueue_step(b, true));
@@ -851,11 +900,15 @@ int run_steps(Batch &b, int nsteps, double *t_inout)
     QL_HIP(hipStreamSynchronize(b.s));
     if (b.vdt) {
         if (nsteps > 0) {
-            qlk::Clock c{};
-            QL_RC(read_clock(b, &c));
-            t = c.t;
-            b.t_prev = c.t_prev;
-            b.dt_last = c.dt_last;
+            std::vector<qlk::Clock> cs;
+            QL_RC(read_clocks(b, cs));
+            for (size_t r = 0; r < cs.size(); ++r) {
+                t_runs[r] = cs[r].t;
+                b.rt_prev[r] = cs[r].t_prev;
+                b.rdt_last[r] = cs[r].dt_last;
+            }
+            b.t_prev = cs[0].t_prev;
+            b.dt_last = cs[0].dt_last;
         }
     } else {
         // the same accumulation as the device clock (main.cpp:116)
@@ -863,10 +916,10 @@ int run_steps(Batch &b, int nsteps, double *t_inout)
             b.t_prev = t;
             t = t + b.dt;
         }
+        for (size_t r = 0; r < b.runs.size(); ++r) t_runs[r] = t;
     }
     if (nsteps > 0) b.old_init = 0;
     b.nstep += nsteps;
-    *t_inout = t;
     return QL_OK;
 }
 
@@ -1210,8 +1263,6 @@ int init_batch(Batch &b, const ql_params *ps, int nruns, const char *who)
         if (c.model != a.model || c.integrator != a.integrator || c.precision != a.precision || c.device != a.device)
             return fail(QL_ERR_ARG, "%s: run %d: runs of a set share model, integrator, precision and device", who, r);
         if (c.dt != a.dt) return fail(QL_ERR_ARG, "%s: run %d: runs of a set share dt", who, r);
-        if (c.dt < 0 && nruns > 1)
-            return fail(QL_ERR_UNSUPPORTED, "%s: variable steps (dt < 0) need a set of one run", who);
         for (int q = 0; q < r; ++q)
             if (ps[q].NZ == c.NZ && ps[q].L[2] != c.L[2])
                 return fail(QL_ERR_ARG, "%s: runs %d and %d: runs of one NZ share Lz", who, q, r);
@@ -1583,6 +1634,8 @@ int ql_batch_linear_ops_init(ql_batch *bt, double t0)
     bt->b.nstep = 0;
     bt->b.t_init = t0;
     bt->b.t_prev = t0;
+    bt->b.rt_prev.assign(bt->b.runs.size(), t0);
+    bt->b.rdt_last.assign(bt->b.runs.size(), bt->b.dt_last);
     return QL_OK;
 }
 
@@ -1590,7 +1643,26 @@ int ql_batch_steps(ql_batch *bt, int nsteps, double *t_inout)
 {
     if (!bt || !t_inout) return fail(QL_ERR_ARG, "ql_batch_steps: null argument");
     if (nsteps < 0) return fail(QL_ERR_ARG, "ql_batch_steps: nsteps < 0");
-    return run_steps(bt->b, nsteps, t_inout);
+    Batch &b = bt->b;
+    if (b.vdt && b.runs.size() > 1)
+        return fail(QL_ERR_ARG, "ql_batch_steps: the runs of a variable-step set keep their own times "
+                                "(ql_batch_steps_runs)");
+    std::vector<double> t(b.runs.size(), *t_inout);
+    QL_RC(run_steps(b, nsteps, t.data()));
+    *t_inout = t[0];
+    return QL_OK;
+}
+
+int ql_batch_steps_runs(ql_batch *bt, int nsteps, double *t_runs)
+{
+    if (!bt || !t_runs) return fail(QL_ERR_ARG, "ql_batch_steps_runs: null argument");
+    if (nsteps < 0) return fail(QL_ERR_ARG, "ql_batch_steps_runs: nsteps < 0");
+    Batch &b = bt->b;
+    if (!b.vdt)
+        for (size_t r = 1; r < b.runs.size(); ++r)
+            if (t_runs[r] != t_runs[0])
+                return fail(QL_ERR_ARG, "ql_batch_steps_runs: the runs of a fixed-step set share one time");
+    return run_steps(b, nsteps, t_runs);
 }
 
 int ql_batch_diagnostics(ql_batch *bt, int run, double t, double out[18])

@@ -151,22 +151,23 @@ struct SubTimes {
     bool oinit;
 };
 template <bool CN>
-__device__ __forceinline__ SubTimes sub_times(const ModeArgs &a)
+__device__ __forceinline__ SubTimes sub_times(const ModeArgs &a, int run)
 {
     // scalars first, the struct once at the end (partial stores of a local
     // struct in separate branches can leave it in scratch memory)
     if (a.s == 0) return SubTimes{a.ts_x, a.dtl_x, 0.0, 0.0, 0.0, a.linC, a.nlc1, a.nlc2, false};
-    const double t = a.clock->t;
+    const Clock *ck = a.clock + run;
+    const double t = ck->t;
     double dt = a.dt, dtP1 = a.dtP1, dtP2 = a.dtP2, dtl2 = a.dtl2, dtl3 = a.dtl3;
     double dtP2o = a.dtP2, dtl3o = a.dtl3;  // the previous step's (old operator of substep 1)
     StepCoefs c{a.linC, a.nlc1, a.nlc2};
     if (a.vdt) {
-        dt = a.clock->dt;
+        dt = ck->dt;
         dtP1 = rn_mul(dt, P1);
         dtP2 = rn_mul(dt, P2);
         dtl2 = rn_mul(dt, P2 - P1);
         dtl3 = rn_mul(dt, 1.0 - P2);
-        const double dl = a.clock->dt_last;
+        const double dl = ck->dt_last;
         dtP2o = rn_mul(dl, P2);
         dtl3o = rn_mul(dl, 1.0 - P2);
         c = step_coefs(dt, a.s, CN);
@@ -174,10 +175,10 @@ __device__ __forceinline__ SubTimes sub_times(const ModeArgs &a)
     const double dtl1 = dtP1;  // dt * p1 (RK3CN_Elin.cpp:90)
     const double tnew = rn_add(t, dt);
     const bool s1 = a.s == 1, s2 = a.s == 2;
-    const bool oinit = s1 && a.clock->old_init != 0;
+    const bool oinit = s1 && ck->old_init != 0;
     const double ts = s1 ? t : rn_add(t, s2 ? dtP1 : dtP2);
     const double dtl = s1 ? dtl1 : (s2 ? dtl2 : dtl3);
-    const double tso = s1 ? (oinit ? a.clock->t_init : rn_add(a.clock->t_prev, dtP2o))
+    const double tso = s1 ? (oinit ? ck->t_init : rn_add(ck->t_prev, dtP2o))
                           : (s2 ? t : rn_add(t, dtP1));
     const double dtlo = s1 ? (oinit ? 0.0 : dtl3o) : (s2 ? dtl1 : dtl2);
     return SubTimes{ts, dtl, tso, dtlo, tnew, c.linC, c.nlc1, c.nlc2, oinit};
@@ -236,15 +237,24 @@ __device__ __forceinline__ bool remap_hit(double qt, double thr, double kx0, dou
     return rn_add(kx0, rn_mul(qt, ky)) > thr;
 }
 
-// the host's view of the clock, passed by value (no host buffer involved);
-// with variable steps dt is the next step's, from run 0's profile maxima
-[[maybe_unused]] static __global__ void k_clock_set(Clock *c, Clock v, const double *pm, const RunParams *rp,
-                                                    int model, int vdt)
+// every run's clock from the host's view v (passed by value, no host buffer
+// involved); with variable steps dt is each run's next step, from its
+// profile maxima
+[[maybe_unused]] static __global__ void k_clock_fill(Clock *c, int nruns, Clock v, const double *pm,
+                                                     const RunParams *rp, int model, int vdt)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (vdt) v.dt = cfl_dt(pm, rp[0], model);
-        *c = v;
+    for (int r = threadIdx.x; r < nruns; r += blockDim.x) {
+        Clock w = v;
+        if (vdt) w.dt = cfl_dt(pm + (size_t)r * PM_STRIDE, rp[r], model);
+        c[r] = w;
     }
+}
+
+// each run's next variable step dt after its clock was uploaded
+[[maybe_unused]] static __global__ void k_clock_dt(Clock *c, int nruns, const double *pm, const RunParams *rp,
+                                                   int model)
+{
+    for (int r = threadIdx.x; r < nruns; r += blockDim.x) c[r].dt = cfl_dt(pm + (size_t)r * PM_STRIDE, rp[r], model);
 }
 
 // t_prev <- t, t <- t + dt (main.cpp:116), once per step
@@ -461,7 +471,7 @@ __device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShare
     const int tid = threadIdx.x;
     double *pm = a.pm + (size_t)run * PM_STRIDE;
     StepCoefs cf{a.linC, a.nlc1, a.nlc2};
-    if (REC && a.vdt && MODE != MODE_PROFILES && a.s > 0) cf = step_coefs(a.clock->dt, a.s, MODE == MODE_CN);
+    if (REC && a.vdt && MODE != MODE_PROFILES && a.s > 0) cf = step_coefs(a.clock[run].dt, a.s, MODE == MODE_CN);
     const double linC = cf.linC, nlc1 = cf.nlc1, nlc2 = cf.nlc2;
     // the right-hand side of this substep read the profiles of the current
     // state (Calculate_CFL reads those, MHD_FullQlin.cpp:772)
@@ -488,7 +498,7 @@ __device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShare
     // the substep's mode kernel, the only reader of the clock, has finished;
     // dt is read above, before this block can change it (the next step's dt is
     // written at the end)
-    if (a.advance && run == 0 && tid == 0) clock_advance(a.clock);
+    if (a.advance && tid == 0) clock_advance(a.clock + run);
     const Phys ph = a.runs[run].ph;
     const int nMF = NMF_OF(MODEL), nslot = nMF * NZD;
     const size_t mo = (size_t)run * nslot;
@@ -633,10 +643,10 @@ __device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShare
             pm[PM_UX] = su / NZD;
         }
     }
-    if (REC && a.vdt && a.advance && run == 0) {
+    if (REC && a.vdt && a.advance) {
         __syncthreads();
-        // the next step's dt (RK3CN_Elin.cpp:76-80)
-        if (tid == 0) a.clock->dt = cfl_dt(pm, a.runs[0], MODEL);
+        // the run's next step dt (RK3CN_Elin.cpp:76-80)
+        if (tid == 0) a.clock[run].dt = cfl_dt(pm, a.runs[run], MODEL);
     }
 }
 

@@ -459,7 +459,7 @@ __device__ __forceinline__ void mode_body(const ModeArgs &a)
     const RunParams rp = a.runs[run];
     const Phys ph = rp.ph;
     // q t products: wave-uniform, kept in scalar registers
-    const SubTimes st = sub_times<MODE == MODE_CN>(a);
+    const SubTimes st = sub_times<MODE == MODE_CN>(a, run);
     const double linC = to_sgpr(st.linC), nlc1 = to_sgpr(st.nlc1), nlc2 = to_sgpr(st.nlc2);
     const double qt = to_sgpr(rn_mul(ph.q, st.ts)), qdtl = to_sgpr(rn_mul(ph.q, st.dtl));
     const double qt_old = to_sgpr(rn_mul(ph.q, st.tso)), qdtl_old = to_sgpr(rn_mul(ph.q, st.dtlo));

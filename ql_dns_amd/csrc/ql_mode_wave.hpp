@@ -148,7 +148,7 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     const int run = wgi.run, g0 = wgi.g0, g1 = wgi.g1, mode_off = wgi.mode_off;
     const RunParams rp = a.runs[run];
     const Phys ph = rp.ph;
-    const SubTimes st = sub_times<MODE == MODE_CN>(a);
+    const SubTimes st = sub_times<MODE == MODE_CN>(a, run);
     const double linC = to_sgpr(st.linC), nlc1 = to_sgpr(st.nlc1), nlc2 = to_sgpr(st.nlc2);
     const double qt = to_sgpr(rn_mul(ph.q, st.ts)), qdtl = to_sgpr(rn_mul(ph.q, st.dtl));
     const double qt_old = to_sgpr(rn_mul(ph.q, st.tso)), qdtl_old = to_sgpr(rn_mul(ph.q, st.dtlo));

@@ -89,7 +89,7 @@ struct NoiseArgs {
     const RunParams *runs;
     const NoiseParams *np;  // [runs]
     const int *mode_run;    // [modes] run of each group mode
-    const Clock *clock;     // step times after the step (t_prev = t, dt_last = dt)
+    const Clock *clock;     // [runs] step times after the step (t_prev = t, dt_last = dt)
     int nz;
     int use_clock;          // times from the clock, else t_x, dt_x
     int noise;              // apply the noise (else remap only)
@@ -105,8 +105,8 @@ struct NoiseArgs {
     const RunParams &rp = a.runs[run];
     const NoiseParams &np = a.np[run];
     const int nz = a.nz;
-    const double t = a.use_clock ? a.clock->t_prev : a.t_x;
-    const double dt = a.use_clock ? a.clock->dt_last : a.dt_x;
+    const double t = a.use_clock ? a.clock[run].t_prev : a.t_x;
+    const double dt = a.use_clock ? a.clock[run].dt_last : a.dt_x;
     const double kx0 = a.kx0[m], ky = a.ky[m];
     const int il = m - rp.mode_off;  // the mode's index in its run
     double2 *U = a.u + (size_t)m * 4 * nz;
@@ -148,7 +148,7 @@ struct NoiseArgs {
     if (a.do_remap && rp.remap) {
         // ShearingBox_Remap(q (t + dt)) (MHD_FullQlin.cpp:464-481); never a
         // ky = 0 mode, so no partner write above touches a remapped mode
-        const double tn = a.use_clock ? a.clock->t : rn_add(t, dt);
+        const double tn = a.use_clock ? a.clock[run].t : rn_add(t, dt);
         const bool hit = remap_hit(rn_mul(rp.ph.q, tn), rp.remap_thr, kx0, ky);
         __syncthreads();  // every thread has read kx0
         if (hit) {

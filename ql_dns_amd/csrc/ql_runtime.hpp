@@ -104,7 +104,8 @@ struct Group {
     bool noise = false;              // some run of the group is driven
     qlk::RunParams *rp = nullptr;
     qlk::WgInfo *wg = nullptr;
-    qlk::Clock *clock = nullptr;  // the group's step times
+    qlk::Clock *clock = nullptr;  // [runs] each run's step times
+    std::vector<qlk::Clock> h_clock;  // host staging of per-run clocks
     std::vector<qlk::RunParams> h_rp;
     // mixed precision: the FP64 companion launch of the modes that carry the
     // norm (ModeArgs::split)
@@ -148,6 +149,8 @@ struct Batch {
     hipEvent_t fork = nullptr;
     // host view of the clock: start of the previous step, linearOPs_Init time
     double t_prev = 0.0, t_init = 0.0;
+    // per run, for sets of several runs with variable steps (each its own clock)
+    std::vector<double> rt_prev, rdt_last;
     int old_init = 1;
     // diagnostics scratch
     double *dpart = nullptr, *dout = nullptr, *mfr_tmp = nullptr;
@@ -179,6 +182,7 @@ void dfree(T *&p)
 // shared by the translation units (defined in ql_dns.hip)
 int sync_all(Batch &b);
 int set_clock(Batch &b, double t);
+int set_clock_runs(Batch &b, const double *t_runs);
 int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, hipStream_t st);
 void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st);
 int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool remap, hipStream_t st);

@@ -82,13 +82,14 @@ def test_set_create_rejects_inconsistent_runs(second, code):
     assert len(L.ql_last_error()) > 0
 
 
-def test_variable_steps_need_a_set_of_one_run():
+def test_variable_step_sets_pass_validation():
+    """Variable steps in a set of several runs are accepted (each run keeps its
+    own clock); without a device creation stops at the device check."""
     from ql_dns_amd.params import QLParams
     p = QLParams(dt=-0.01)
     rc, h, L = _create_set([p, p.replace(NZ=64)])
-    assert rc == -3
-    assert h.value is None
-    assert b"one run" in L.ql_last_error()
+    assert rc == -2 and b"device" in L.ql_last_error()
+    assert L.ql_batch_steps_runs(None, 1, None) == -1
 
 
 def test_set_null_arguments_are_errors():

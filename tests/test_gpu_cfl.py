@@ -199,3 +199,40 @@ def test_fixed_step_cfl_is_the_current_state_value(model):
     g.rhs(to, 0.0)
     vo, vg = o.calculate_cfl(), g.calculate_cfl()
     assert abs(vg - vo) <= 1e-12 * vo
+
+
+@pytest.mark.parametrize("integrator", [1, 0])
+def test_variable_steps_in_a_run_set(integrator):
+    """A set of runs of different sizes and physics with variable steps: each
+    run keeps its own clock and chooses its own steps (ql_batch_steps_runs);
+    every run's times and state match the oracle stepping that run alone."""
+    from ql_dns_amd.capi import QLBatch, QLError
+    from ql_dns_amd.ic import initial_state
+    from oracle.oracle import Oracle
+    base = dict(integrator=integrator, dt=-0.05, CFL=0.5)
+    runs = [_params(NX=16, NY=16, NZ=32, q=1.5, **base),
+            _params(NX=8, NY=8, NZ=32, q=1.2, nu=1 / 600, eta=1 / 700, Ly=1.0, **base),
+            _params(NX=8, NY=6, NZ=64, q=1.7, B0z=0.05, Ly=1.0, **base),
+            _params(NX=4, NY=4, NZ=128, q=1.3, Ly=0.5, **base)]
+    states = [initial_state(p, seed=900 + r, mf_amp=0.3 + 0.4 * r) for r, p in enumerate(runs)]
+    b = QLBatch(runs)
+    for r, (lin, mf) in enumerate(states):
+        b.set_state(r, lin, mf)
+    b.linear_ops_init(0.0)
+    with pytest.raises(QLError):
+        b.steps(1, 0.0)  # one shared time does not fit runs with their own steps
+    tg = b.steps_runs(12, np.zeros(len(runs)))
+    tg = b.steps_runs(8, tg)
+    assert len(set(tg.tolist())) == len(runs)  # the runs took different steps
+    for r, (p, (lin, mf)) in enumerate(zip(runs, states)):
+        o = Oracle.from_params(p, nthreads=8)
+        o.set_state(lin, mf)
+        o.linear_ops_init(0.0)
+        to = o.steps(20, 0.0)
+        assert abs(tg[r] - to) <= 1e-12 * to, r
+        (lo, mo, ko), (lg, mg, kg) = o.get_state(), b.get_state(r)
+        assert np.array_equal(kg, ko)
+        assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL, r
+        assert abs(b.calculate_cfl(r) - o.calculate_cfl()) <= 1e-12 * o.calculate_cfl()
+        o.close()
+    b.close()
