@@ -45,6 +45,8 @@ def metric_name(p, workload: str, equations: str, integrator: str, precision: st
     if (workload, equations, integrator, precision) == ("box64x128x128", "MHD_FullQlin", "RK3CN_Elin", "fp64"):
         return METRIC
     return f"QL {integrator} steps/s at {p.NX}x{p.NY}x{p.NZ} ({equations}, {precision})"
+
+
 HBM_PEAK_GBS = 8000.0
 
 
@@ -281,6 +283,27 @@ def bench_adjoint(name: str = "box32x32x64", nsteps: int = 200, every: int = 20,
             "fd_directions": ndir, "fd_eps": eps, "max_rel_err_vs_fd": worst}
 
 
+def device_warm(p, lin, mf, shape, ms: float):
+    """Sustained device work right before the warm-up steps.  The GPU raises
+    its clocks over the first ~20 ms of sustained work after an idle spell of a
+    few ms (the setup's uploads, the tuner's restore and frees): measured at
+    64x128x128, 120 us per step falling to 114 us over ~100 steps, from any
+    state.  A second context of the same workload steps for `ms` of wall time
+    and is returned open (closing it frees memory, which idles the device); the
+    timed context's state is untouched."""
+    from ql_dns_amd.capi import QLContext
+    w = QLContext(p)
+    w.set_state(lin, mf)
+    w.set_config(shape["NZ"], shape["line"], shape["target"])
+    w.linear_ops_init(0.0)
+    t = w.steps(16, 0.0)  # captures its graphs
+    w.synchronize()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < ms * 1e-3:
+        t = w.steps(16, t)
+    return w
+
+
 def copy_calibration(nbytes: int = 1 << 30):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE / WRITE_SIZE) that rocprofv3 reports per dispatch."""
@@ -304,6 +327,8 @@ def main():
     ap.add_argument("--equations", choices=["MHD_FullQlin", "MHD_BQlin", "HD_fullU"], default="MHD_FullQlin",
                     help="model of the timed run (HD_fullU runs the velocity-only policy)")
     ap.add_argument("--profile-steps", type=int, default=50)
+    ap.add_argument("--device-warm-ms", type=float, default=40.0,
+                    help="sustained device work (a second context) right before the warm-up steps; 0: none")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 256-run ensemble section")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
@@ -345,6 +370,7 @@ def main():
     elif not args.no_tune:
         ctx.tune(3)  # launch shape; restores the state
     ctx.linear_ops_init(0.0)
+    warm = device_warm(p, lin, mf, ctx.config()[0], args.device_warm_ms) if args.device_warm_ms > 0 else None
     t = ctx.steps(args.warmup, 0.0)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -352,6 +378,8 @@ def main():
     ctx.synchronize()
     el = time.perf_counter() - t0
     steps_per_s = args.steps / el
+    if warm is not None:
+        warm.close()
 
     # per-kernel launch durations (HIP events on the library stream)
     _, avg_ms, counts = ctx.profile_kernels(args.profile_steps, t)

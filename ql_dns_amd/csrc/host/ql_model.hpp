@@ -84,7 +84,7 @@ class solution;
 class TimeVariables;
 class Model;
 
-// Auxiliary/Kdata.h:8-27: the wavenumber tables, k stored as i*k as the
+// Auxiliary/Kdata.h:17-34: the wavenumber tables, k stored as i*k as the
 // reference stores them, for callers that read Model::K (FullSave_Load.cpp:110,
 // 134, 196, 233, 291, 344).  One device holds every mode, so the local and
 // full index tables are the same.  kx is the current (remapped) kx0 of each

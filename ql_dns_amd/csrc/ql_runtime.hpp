@@ -129,6 +129,8 @@ constexpr double MIXED_AMP_TAU = 0.03;
 constexpr int SPLIT_EVERY = 16;
 // companion launch: about this many workgroups per run
 constexpr int COMP_TARGET_WG = 128;
+// device time the tuner runs the current shapes before timing candidates
+constexpr double TUNE_WARM_MS = 25.0;
 
 struct AdjState;  // adjoint checkpoints and buffers (ql_adjoint.hip)
 
