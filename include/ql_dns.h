@@ -270,11 +270,14 @@ int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
 /* ------------------------------------------------------------------------
  * Adjoint (reverse-mode gradient) of nsteps of ql_steps, for a single-run
- * MHD_FullQlin context in FP64 (either integrator, remap on or off).  The
- * reference has no adjoint; these entry points differentiate the reference's
- * step (Models/MHD_FullQlin.cpp:171-427, Integrators/RK3CN.cpp:77-201,
- * Integrators/RK3CN_Elin.cpp:74-188, main.cpp:108-123) as this library
- * computes it.  Remap decisions are held fixed (piecewise constant in q).
+ * MHD_FullQlin or MHD_BQlin context in FP64 with fixed steps and no driving
+ * noise (either integrator, remap on or off).  The reference has no adjoint;
+ * these entry points differentiate the reference's step
+ * (Models/MHD_FullQlin.cpp:171-427, Models/MHD_BQlin.cpp:117-306,
+ * Integrators/RK3CN.cpp:77-201, Integrators/RK3CN_Elin.cpp:74-188,
+ * main.cpp:108-123) as this library computes it.  Remap decisions are held
+ * fixed (piecewise constant in q).  MHD_BQlin rotates at 2 q / 3
+ * (MHD_BQlin.cpp:19), so its omega and B0z gradients are zero.
  *
  * ql_adjoint_forward: nsteps of ql_steps from the current state, storing a
  * checkpoint every `every` steps (device memory ~ (nsteps/every + 3 every + 8)

@@ -1,6 +1,6 @@
 // ql_adjoint.hip -- adjoint (reverse-mode gradient) of n steps of the QL main
 // loop (main.cpp:108-123: Step, t += dt, ShearingBox_Remap) for one
-// MHD_FullQlin run in FP64, with checkpointed recomputation.
+// MHD_FullQlin or MHD_BQlin run in FP64, with checkpointed recomputation.
 //
 // ql_adjoint_forward runs the steps exactly as ql_steps does (the same kernels
 // and launch shape, so the same bits) and stores the state every `every`
@@ -64,8 +64,8 @@ int adj_line_mb(int NZ) { return NZ >= 128 ? 1 : 128 / NZ; }
 int check_adjoint(const Batch &b, const char *who)
 {
     if (b.runs.size() != 1) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is for single-run contexts", who);
-    if (b.model != QL_MODEL_MHD_FULLQLIN || b.hd)
-        return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin only", who);
+    if ((b.model != QL_MODEL_MHD_FULLQLIN && b.model != QL_MODEL_MHD_BQLIN) || b.hd)
+        return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin and MHD_BQlin", who);
     if (b.precision != QL_PREC_FP64) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint runs in FP64 only", who);
     if (b.vdt) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs fixed steps (dt > 0)", who);
     if (b.groups[0].noise) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs the driving noise off", who);
@@ -133,31 +133,40 @@ HostTimes host_times(double t, double t_prev, int old_init, double t_init, doubl
     return r;
 }
 
-template <int NZ>
+template <int NZ, int MODEL>
 void launch_adj_nz(const qlk::AdjModeArgs &am, const qlk::AdjMFArgs &aa, const qlk::AdjMF2Args &ab, int W,
                    const double *gpart, const qlk::RunParams *rp1, double *grows, hipStream_t st, int stage)
 {
     constexpr int MB = NZ >= 128 ? 1 : 128 / NZ;
+    using P = typename std::conditional<MODEL == 0, qlk::ModelFull, qlk::ModelBQ>::type;
     if (stage == 0) {
-        qlk::k_adj_mf_a<NZ><<<1, qlk::NT, 0, st>>>(aa);
+        qlk::k_adj_mf_a<NZ, MODEL><<<1, qlk::NT, 0, st>>>(aa);
     } else if (stage == 1) {
-        qlk::k_adj_mode<NZ, MB><<<W, qlk::NT, 0, st>>>(am);
+        qlk::k_adj_mode<NZ, MB, P><<<W, qlk::NT, 0, st>>>(am);
         const int ncols = 8 * NZ + qlk::NGP;
         qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, 1), qlk::NT, 0, st>>>(gpart, rp1, ncols, grows);
     } else {
-        qlk::k_adj_mf_b<NZ><<<1, qlk::NT, 0, st>>>(ab);
+        qlk::k_adj_mf_b<NZ, MODEL><<<1, qlk::NT, 0, st>>>(ab);
     }
 }
 
-void launch_adj(int NZ, const qlk::AdjModeArgs &am, const qlk::AdjMFArgs &aa, const qlk::AdjMF2Args &ab, int W,
-                const double *gpart, const qlk::RunParams *rp1, double *grows, hipStream_t st, int stage)
+template <int MODEL>
+void launch_adj_m(int NZ, const qlk::AdjModeArgs &am, const qlk::AdjMFArgs &aa, const qlk::AdjMF2Args &ab, int W,
+                  const double *gpart, const qlk::RunParams *rp1, double *grows, hipStream_t st, int stage)
 {
     switch (NZ) {
-    case 32: launch_adj_nz<32>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
-    case 64: launch_adj_nz<64>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
-    case 128: launch_adj_nz<128>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
-    default: launch_adj_nz<256>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
+    case 32: launch_adj_nz<32, MODEL>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
+    case 64: launch_adj_nz<64, MODEL>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
+    case 128: launch_adj_nz<128, MODEL>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
+    default: launch_adj_nz<256, MODEL>(am, aa, ab, W, gpart, rp1, grows, st, stage); break;
     }
+}
+
+void launch_adj(int NZ, int model, const qlk::AdjModeArgs &am, const qlk::AdjMFArgs &aa, const qlk::AdjMF2Args &ab,
+                int W, const double *gpart, const qlk::RunParams *rp1, double *grows, hipStream_t st, int stage)
+{
+    if (model == QL_MODEL_MHD_BQLIN) launch_adj_m<1>(NZ, am, aa, ab, W, gpart, rp1, grows, st, stage);
+    else launch_adj_m<0>(NZ, am, aa, ab, W, gpart, rp1, grows, st, stage);
 }
 
 int adjoint_forward(Batch &b, int nsteps, int every, double *t_inout)
@@ -207,11 +216,14 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
         QL_RC(dalloc(&A->gM, nm));
         QL_RC(dalloc(&A->gnmp, nm));
         QL_RC(dalloc(&A->gsums, (size_t)4 * NZ));
+        QL_HIP(hipMemset(A->gsums, 0, sizeof(double) * 4 * NZ));  // MHD_BQlin fills 2 of the 4 lines
         QL_RC(dalloc(&A->gpart, (size_t)W * ncols));
         QL_RC(dalloc(&A->grows, (size_t)ncols));
         QL_RC(dalloc(&A->gpar_mf, (size_t)qlk::NGP));
         QL_RC(dalloc(&A->gpar, (size_t)qlk::NGP));
         QL_RC(dalloc(&A->mfz, (size_t)8 * NZ));
+        // MHD_BQlin writes 3 of the 8 profile lines; the others stay zero
+        QL_HIP(hipMemset(A->mfz, 0, sizeof(double) * 8 * NZ));
         QL_RC(dalloc(&A->rp1, 1));
         qlk::RunParams rp = g.h_rp[0];
         rp.wg_off = 0;
@@ -338,7 +350,7 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
                 ab.tw = g.tw;
                 ab.fac = 1.0 / NZ;
                 for (int stage = 0; stage < 3; ++stage) {
-                    launch_adj(NZ, am, aa, ab, A->W, A->gpart, A->rp1, A->grows, b.s, stage);
+                    launch_adj(NZ, b.model, am, aa, ab, A->W, A->gpart, A->rp1, A->grows, b.s, stage);
                     QL_HIP(hipGetLastError());
                 }
             }
@@ -351,6 +363,12 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
     double gp[qlk::NGP];
     QL_HIP(hipMemcpyAsync(gp, A->gpar, sizeof(double) * qlk::NGP, hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    if (b.model == QL_MODEL_MHD_BQLIN) {
+        // MHD_BQlin rotates at 2 q / 3 whatever omega is (MHD_BQlin.cpp:19): the
+        // kernels' rotation gradient belongs to q
+        gp[0] += 2.0 / 3.0 * gp[1];
+        gp[1] = 0.0;
+    }
     if (g_par)
         for (int c = 0; c < 5; ++c) g_par[c] = gp[c];
     return QL_OK;

@@ -45,13 +45,19 @@ def _run(p, lin, mf, n):
 PARAMS = ("q", "omega", "nu", "eta", "B0z")
 
 
-@pytest.mark.parametrize("integrator,NZ", [(1, 32), (0, 32), (1, 128), (0, 128)])
-def test_adjoint_matches_finite_differences(integrator, NZ):
+@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("integrator,NZ", [(1, 32), (0, 32), (1, 64), (1, 128), (0, 128), (0, 256)])
+def test_adjoint_matches_finite_differences(model, integrator, NZ):
+    """MHD_FullQlin (model 0) and MHD_BQlin (model 1); MHD_BQlin has no B0z
+    terms and rotates at 2 q / 3, so its B0z and omega derivatives are zero on
+    both sides and its q derivative carries the rotation's.  MHD_BQlin's state
+    directions include the kz = 0 slot of the ky = 0 modes' ux, which enters
+    its dz uz as a z-constant."""
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state, mode_tables
     # remaps happen within these steps (Ly = 1); B0z != 0 exercises its terms;
     # at NZ = 128 the forward runs the wavefront kernel (the default shape there)
-    p = _p(NX=8, NY=8, NZ=NZ, Ly=1.0, B0z=0.03, integrator=integrator, nu=1 / 1500, eta=1 / 1200)
+    p = _p(model=model, NX=8, NY=8, NZ=NZ, Ly=1.0, B0z=0.03, integrator=integrator, nu=1 / 1500, eta=1 / 1200)
     n, every = 14, 5
     lin0, mf0 = initial_state(p)
     alpha = 0.01 / float((np.abs(lin0) ** 2).sum())
@@ -70,7 +76,8 @@ def test_adjoint_matches_finite_differences(integrator, NZ):
     rms_m = np.sqrt(np.mean(np.abs(mf0) ** 2))
     for d in range(4):
         vl = (rng.standard_normal(lin0.shape) + 1j * rng.standard_normal(lin0.shape)) * rms_l
-        vl[::p.NY // 2, 0::2, 0] = 0  # directions within divergence-free states (ux, bx at ky = kz = 0)
+        if model == 0:
+            vl[::p.NY // 2, 0::2, 0] = 0  # directions within divergence-free states (ux, bx at ky = kz = 0)
         vm = (rng.standard_normal(mf0.shape) + 1j * rng.standard_normal(mf0.shape)) * rms_m
         ad = float(np.sum(gl0.real * vl.real + gl0.imag * vl.imag) + np.sum(gm0.real * vm.real + gm0.imag * vm.imag))
         Lp = _loss_parts(p, *_run(p, lin0 + EPS * vl, mf0 + EPS * vm, n), alpha)[0]
@@ -83,13 +90,18 @@ def test_adjoint_matches_finite_differences(integrator, NZ):
         Lp = _loss_parts(p, *_run(p.replace(**{name: base + h}), lin0, mf0, n), alpha)[0]
         Lm = _loss_parts(p, *_run(p.replace(**{name: base - h}), lin0, mf0, n), alpha)[0]
         fd = (Lp - Lm) / (2 * h)
+        if model == 1 and name in ("omega", "B0z"):
+            # MHD_BQlin has no B0z terms and rotates at 2 q / 3 whatever omega is
+            assert fd == 0.0 and gpar[k] == 0.0
+            continue
         assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
 
 
-def test_adjoint_independent_of_checkpoint_spacing():
+@pytest.mark.parametrize("model", [0, 1])
+def test_adjoint_independent_of_checkpoint_spacing(model):
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state
-    p = _p(NX=8, NY=6, NZ=64, Ly=1.0)
+    p = _p(model=model, NX=8, NY=6, NZ=64, Ly=1.0)
     n = 9
     lin0, mf0 = initial_state(p)
     res = []
@@ -112,7 +124,7 @@ def test_adjoint_independent_of_checkpoint_spacing():
 
 def test_adjoint_rejects_unsupported():
     from ql_dns_amd.capi import QLContext, QLError
-    for kw in (dict(model=1), dict(precision=1)):
+    for kw in (dict(model=2), dict(precision=1)):
         c = QLContext(_p(NX=4, NY=4, NZ=32, **kw))
         with pytest.raises(QLError):
             c.adjoint_forward(2, 1, 0.0)
