@@ -21,6 +21,8 @@
 #include <cstring>
 #include <mutex>
 
+#include <hip/hip_ext.h>
+
 #include "ql_mode.hpp"
 #include "ql_mode_wave.hpp"
 #include "ql_runtime.hpp"
@@ -411,47 +413,68 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
 // ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
+// A mode-kernel launch.  In the per-kernel timing pass the launch carries the
+// batch's start / stop events itself (hipExtLaunchKernel), so they hold the
+// dispatch's own begin and end times rather than those of separate event
+// packets around it; `first` / `last` say which of them this launch sets.
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+// set by enqueue_sequence for the launches of one timed sequence
+thread_local LaunchTiming tm_cur;
+
+template <class K>
+void launch_k(K kernel, int grid, int block, hipStream_t st, const LaunchTiming &tm, const qlk::ModeArgs &a)
+{
+    if (tm.start || tm.stop)
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, tm.start, tm.stop, 0, a);
+    else
+        kernel<<<grid, block, 0, st>>>(a);
+}
+
 template <int NZ, int MB, int MODE>
 void launch_mode_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
+    const LaunchTiming tm = tm_cur;
     const bool full = (b.model == QL_MODEL_MHD_FULLQLIN);
     constexpr int T = qlk::mode_threads(NZ, MB);
     // HD_fullU: the velocity-only policy (its magnetic fields are zero)
     if (b.precision == QL_PREC_MIXED) {
-        if (b.hd) qlk::k_mode32<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
-        else if (full) qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
-        else qlk::k_mode32<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
+        if (b.hd) launch_k(qlk::k_mode32<NZ, MB, qlk::ModelHD, MODE>, g.W, T, st, tm, a);
+        else if (full) launch_k(qlk::k_mode32<NZ, MB, qlk::ModelFull, MODE>, g.W, T, st, tm, a);
+        else launch_k(qlk::k_mode32<NZ, MB, qlk::ModelBQ, MODE>, g.W, T, st, tm, a);
     } else if (g.dft) {
         if constexpr (NZ <= 64) {  // dft_ok
-            if (b.hd) qlk::k_mode64d<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
-            else if (full) qlk::k_mode64d<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
-            else qlk::k_mode64d<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
+            if (b.hd) launch_k(qlk::k_mode64d<NZ, MB, qlk::ModelHD, MODE>, g.W, T, st, tm, a);
+            else if (full) launch_k(qlk::k_mode64d<NZ, MB, qlk::ModelFull, MODE>, g.W, T, st, tm, a);
+            else launch_k(qlk::k_mode64d<NZ, MB, qlk::ModelBQ, MODE>, g.W, T, st, tm, a);
         }
     } else {
-        if (b.hd) qlk::k_mode64<NZ, MB, qlk::ModelHD, MODE><<<g.W, T, 0, st>>>(a);
-        else if (full) qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE><<<g.W, T, 0, st>>>(a);
-        else qlk::k_mode64<NZ, MB, qlk::ModelBQ, MODE><<<g.W, T, 0, st>>>(a);
+        if (b.hd) launch_k(qlk::k_mode64<NZ, MB, qlk::ModelHD, MODE>, g.W, T, st, tm, a);
+        else if (full) launch_k(qlk::k_mode64<NZ, MB, qlk::ModelFull, MODE>, g.W, T, st, tm, a);
+        else launch_k(qlk::k_mode64<NZ, MB, qlk::ModelBQ, MODE>, g.W, T, st, tm, a);
     }
 }
 
 template <int NZ, int MODE>
 void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
+    const LaunchTiming tm = tm_cur;
     constexpr int TF = qlk::mode_waves<qlk::ModelFull>(NZ) * qlk::WAVE;
     constexpr int TH = qlk::mode_waves<qlk::ModelHD>(NZ) * qlk::WAVE;
     constexpr int TB = qlk::mode_waves<qlk::ModelBQ>(NZ) * qlk::WAVE;
     const bool bq = b.model == QL_MODEL_MHD_BQLIN;
     // HD_fullU: the velocity-only policy (its magnetic fields are zero)
     if (b.precision == QL_PREC_MIXED) {
-        if (b.hd) qlk::k_wave32<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
+        if (b.hd) launch_k(qlk::k_wave32<NZ, qlk::ModelHD, MODE>, g.W, TH, st, tm, a);
         else if constexpr (NZ <= 128) {  // wave_ok: no other FP32 wavefront kernel at NZ = 256
-            if (bq) qlk::k_wave32<NZ, qlk::ModelBQ, MODE><<<g.W, TB, 0, st>>>(a);
-            else qlk::k_wave32<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
+            if (bq) launch_k(qlk::k_wave32<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
+            else launch_k(qlk::k_wave32<NZ, qlk::ModelFull, MODE>, g.W, TF, st, tm, a);
         }
     } else {
-        if (b.hd) qlk::k_wave64<NZ, qlk::ModelHD, MODE><<<g.W, TH, 0, st>>>(a);
-        else if (bq) qlk::k_wave64<NZ, qlk::ModelBQ, MODE><<<g.W, TB, 0, st>>>(a);
-        else qlk::k_wave64<NZ, qlk::ModelFull, MODE><<<g.W, TF, 0, st>>>(a);
+        if (b.hd) launch_k(qlk::k_wave64<NZ, qlk::ModelHD, MODE>, g.W, TH, st, tm, a);
+        else if (bq) launch_k(qlk::k_wave64<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
+        else launch_k(qlk::k_wave64<NZ, qlk::ModelFull, MODE>, g.W, TF, st, tm, a);
     }
 }
 
@@ -468,11 +491,12 @@ void launch_wave_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
 template <int NZ, int MODE>
 void launch_comp_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
+    const LaunchTiming tm = tm_cur;
     constexpr int MB = NZ == 32 ? 4 : (NZ == 64 ? 2 : 1);
     constexpr int T = qlk::mode_threads(NZ, MB);
-    if (b.hd) qlk::k_mode64s<NZ, MB, qlk::ModelHD, MODE><<<g.W2, T, 0, st>>>(a);
-    else if (b.model == QL_MODEL_MHD_BQLIN) qlk::k_mode64s<NZ, MB, qlk::ModelBQ, MODE><<<g.W2, T, 0, st>>>(a);
-    else qlk::k_mode64s<NZ, MB, qlk::ModelFull, MODE><<<g.W2, T, 0, st>>>(a);
+    if (b.hd) launch_k(qlk::k_mode64s<NZ, MB, qlk::ModelHD, MODE>, g.W2, T, st, tm, a);
+    else if (b.model == QL_MODEL_MHD_BQLIN) launch_k(qlk::k_mode64s<NZ, MB, qlk::ModelBQ, MODE>, g.W2, T, st, tm, a);
+    else launch_k(qlk::k_mode64s<NZ, MB, qlk::ModelFull, MODE>, g.W2, T, st, tm, a);
 }
 
 template <int NZ>
@@ -663,7 +687,8 @@ int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool
 // buffers, or ql_rhs_noalloc's scratch)
 int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFArgs &m, hipStream_t st)
 {
-    if (b.timing) QL_HIP(hipEventRecord(b.ev[0], st));
+    // timing pass: ev[0] = the first mode launch's start, ev[1] = the last
+    // one's end (dispatch times), ev[2] = after the mean-field kernel
     const int ncols = b.nMF * g.NZ;
     if (a.split == qlk::SPLIT_LO) {
         // the FP64 companion first, then the FP32 launch (the FP32 launch
@@ -673,15 +698,18 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
         a2.split = qlk::SPLIT_HI;
         a2.wg = g.wg2;
         a2.partials = a.partials + (size_t)g.W * ncols;
+        tm_cur = LaunchTiming{b.timing ? b.ev[0] : nullptr, nullptr};
         launch_comp(b, g, a2, st);
         QL_HIP(hipGetLastError());
+        tm_cur = LaunchTiming{nullptr, b.timing ? b.ev[1] : nullptr};
         launch_mode(b, g, a, st);
         QL_HIP(hipGetLastError());
     } else {
+        tm_cur = LaunchTiming{b.timing ? b.ev[0] : nullptr, b.timing ? b.ev[1] : nullptr};
         launch_mode(b, g, a, st);
         QL_HIP(hipGetLastError());
     }
-    if (b.timing) QL_HIP(hipEventRecord(b.ev[1], st));
+    tm_cur = LaunchTiming{};
     qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(
         a.partials, g.rp, ncols, const_cast<double *>(m.sums));
     QL_HIP(hipGetLastError());
