@@ -66,6 +66,9 @@ void release(Batch &b)
     dfree(b.dpart), dfree(b.dout), dfree(b.mfr_tmp);
     for (auto &e : b.ev)
         if (e) (void)hipEventDestroy(e), e = nullptr;
+    for (hipEvent_t e : b.tpool) (void)hipEventDestroy(e);
+    b.tpool.clear();
+    b.tused = 0;
     if (b.fork) (void)hipEventDestroy(b.fork);
     if (b.s) (void)hipStreamDestroy(b.s);
     b.fork = nullptr;
@@ -687,8 +690,19 @@ int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool
 // buffers, or ql_rhs_noalloc's scratch)
 int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFArgs &m, hipStream_t st)
 {
-    // timing pass: ev[0] = the first mode launch's start, ev[1] = the last
-    // one's end (dispatch times), ev[2] = after the mean-field kernel
+    // timing pass: e[0] = the first mode launch's start, e[1] = the last
+    // one's end (dispatch times), e[2] = after the mean-field kernel; read by
+    // ql_profile_kernels when the pass has finished
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    if (b.timing) {
+        while (b.tpool.size() < b.tused + 3) {
+            hipEvent_t ev = nullptr;
+            QL_HIP(hipEventCreate(&ev));
+            b.tpool.push_back(ev);
+        }
+        for (int k = 0; k < 3; ++k) e[k] = b.tpool[b.tused + k];
+        b.tused += 3;
+    }
     const int ncols = b.nMF * g.NZ;
     if (a.split == qlk::SPLIT_LO) {
         // the FP64 companion first, then the FP32 launch (the FP32 launch
@@ -698,14 +712,14 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
         a2.split = qlk::SPLIT_HI;
         a2.wg = g.wg2;
         a2.partials = a.partials + (size_t)g.W * ncols;
-        tm_cur = LaunchTiming{b.timing ? b.ev[0] : nullptr, nullptr};
+        tm_cur = LaunchTiming{e[0], nullptr};
         launch_comp(b, g, a2, st);
         QL_HIP(hipGetLastError());
-        tm_cur = LaunchTiming{nullptr, b.timing ? b.ev[1] : nullptr};
+        tm_cur = LaunchTiming{nullptr, e[1]};
         launch_mode(b, g, a, st);
         QL_HIP(hipGetLastError());
     } else {
-        tm_cur = LaunchTiming{b.timing ? b.ev[0] : nullptr, b.timing ? b.ev[1] : nullptr};
+        tm_cur = LaunchTiming{e[0], e[1]};
         launch_mode(b, g, a, st);
         QL_HIP(hipGetLastError());
     }
@@ -715,17 +729,7 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
     QL_HIP(hipGetLastError());
     launch_mf(g, m, st);
     QL_HIP(hipGetLastError());
-    if (b.timing) {
-        QL_HIP(hipEventRecord(b.ev[2], st));
-        QL_HIP(hipEventSynchronize(b.ev[2]));
-        float ms0 = 0.f, ms1 = 0.f;
-        QL_HIP(hipEventElapsedTime(&ms0, b.ev[0], b.ev[1]));
-        QL_HIP(hipEventElapsedTime(&ms1, b.ev[1], b.ev[2]));
-        b.t_ms[0] += ms0;
-        b.t_ms[1] += ms1;
-        b.t_cnt[0] += 1;
-        b.t_cnt[1] += 1;
-    }
+    if (b.timing) QL_HIP(hipEventRecord(e[2], st));
     return QL_OK;
 }
 
@@ -1214,7 +1218,7 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
     // The GPU raises its clocks over the first ~20 ms of sustained work after
     // an idle spell of a few ms (measured at 64x128x128: 120 us per step
     // falling to 114 us over about 100 steps, from any state), so candidates
-    // timed from a cold start would be judged at different clocks: the
+    // timed from a cold start would be compared at different clocks: the
     // current shapes run first
     if (rc == QL_OK) rc = tune_warm(b);
     for (size_t gi = 0; gi < b.groups.size() && rc == QL_OK; ++gi) {
@@ -1619,9 +1623,21 @@ int ql_profile_kernels(ql_ctx *c, int nsteps, double *t_inout, double avg_ms[2],
         b.t_cnt[k] = 0;
     }
     b.timing = true;
+    b.tused = 0;
     int rc = run_steps(b, nsteps, t_inout);
     b.timing = false;
     if (rc) return rc;
+    QL_RC(sync_all(b));
+    for (size_t k = 0; k + 3 <= b.tused; k += 3) {
+        float ms0 = 0.f, ms1 = 0.f;
+        QL_HIP(hipEventElapsedTime(&ms0, b.tpool[k], b.tpool[k + 1]));
+        QL_HIP(hipEventElapsedTime(&ms1, b.tpool[k + 1], b.tpool[k + 2]));
+        b.t_ms[0] += ms0;
+        b.t_ms[1] += ms1;
+        b.t_cnt[0] += 1;
+        b.t_cnt[1] += 1;
+    }
+    b.tused = 0;
     for (int k = 0; k < 2; ++k) {
         counts[k] = b.t_cnt[k];
         avg_ms[k] = b.t_cnt[k] ? b.t_ms[k] / b.t_cnt[k] : 0.0;

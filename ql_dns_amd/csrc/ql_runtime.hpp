@@ -163,6 +163,11 @@ struct Batch {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     double t_ms[2] = {0, 0};
     int t_cnt[2] = {0, 0};
+    // timing pass: three events per substep sequence (mode start, mode end,
+    // after the mean fields), read once the pass has finished, so that the
+    // pass runs without host round trips between launches
+    std::vector<hipEvent_t> tpool;
+    size_t tused = 0;
     AdjState *adj = nullptr;
 };
 
