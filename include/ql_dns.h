@@ -270,8 +270,8 @@ int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
 /* ------------------------------------------------------------------------
  * Adjoint (reverse-mode gradient) of nsteps of ql_steps, for a single-run
- * MHD_FullQlin or MHD_BQlin context in FP64 with fixed steps and no driving
- * noise (either integrator, remap on or off).  The reference has no adjoint;
+ * MHD_FullQlin or MHD_BQlin context in FP64 with fixed steps, driven or not
+ * (either integrator, remap on or off).  The reference has no adjoint;
  * these entry points differentiate the reference's step
  * (Models/MHD_FullQlin.cpp:171-427, Models/MHD_BQlin.cpp:117-306,
  * Integrators/RK3CN.cpp:77-201, Integrators/RK3CN_Elin.cpp:74-188,

@@ -21,6 +21,7 @@ namespace qlh {
 
 struct AdjState {
     int nsteps = 0, every = 0, nseg = 0;
+    bool noise = false;  // the forward was driven
     double t_end = 0.0;
     // checkpoints at segment starts (index nseg: the final state)
     struct Ck {
@@ -31,8 +32,9 @@ struct AdjState {
     };
     std::vector<Ck> ck;
     double t_init = 0.0;
-    // substep states of one segment
-    double2 *st_u = nullptr, *st_mf = nullptr;
+    // substep states of one segment; with driving noise also each step's
+    // state after substep 3 before the noise and the remap
+    double2 *st_u = nullptr, *st_mf = nullptr, *st_pre = nullptr;
     double *st_kx0 = nullptr;
     // adjoint buffers
     double2 *gu = nullptr, *gnp = nullptr, *gM = nullptr, *gnmp = nullptr;
@@ -47,7 +49,7 @@ void adj_release(Batch &b)
     AdjState *A = b.adj;
     if (!A) return;
     for (auto &c : A->ck) dfree(c.u), dfree(c.mf), dfree(c.kx0);
-    dfree(A->st_u), dfree(A->st_mf), dfree(A->st_kx0);
+    dfree(A->st_u), dfree(A->st_mf), dfree(A->st_kx0), dfree(A->st_pre);
     dfree(A->gu), dfree(A->gnp), dfree(A->gM), dfree(A->gnmp);
     dfree(A->gsums), dfree(A->gpart), dfree(A->grows), dfree(A->gpar_mf), dfree(A->gpar), dfree(A->mfz);
     dfree(A->rp1);
@@ -68,7 +70,6 @@ int check_adjoint(const Batch &b, const char *who)
         return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin and MHD_BQlin", who);
     if (b.precision != QL_PREC_FP64) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint runs in FP64 only", who);
     if (b.vdt) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs fixed steps (dt > 0)", who);
-    if (b.groups[0].noise) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs the driving noise off", who);
     return QL_OK;
 }
 
@@ -179,6 +180,7 @@ int adjoint_forward(Batch &b, int nsteps, int every, double *t_inout)
     A->nsteps = nsteps;
     A->every = every;
     A->nseg = (nsteps + every - 1) / every;
+    A->noise = b.groups[0].noise;
     A->t_init = b.t_init;
     A->ck.resize(A->nseg + 1);
     double t = *t_inout;
@@ -200,6 +202,8 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
     QL_RC(check_adjoint(b, "ql_adjoint_backward"));
     AdjState *A = b.adj;
     if (!A) return fail(QL_ERR_STATE, "ql_adjoint_backward: no ql_adjoint_forward on this context");
+    if (A->noise != b.groups[0].noise)
+        return fail(QL_ERR_STATE, "ql_adjoint_backward: the driving noise changed since ql_adjoint_forward");
     Group &g = b.groups[0];
     const RunInfo &r = b.runs[0];
     const int NZ = g.NZ, nz = g.nz, nxy = r.nxy, K = A->every;
@@ -211,6 +215,7 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
         QL_RC(dalloc(&A->st_u, nl * (3 * K + 1)));
         QL_RC(dalloc(&A->st_mf, nm * (3 * K + 1)));
         QL_RC(dalloc(&A->st_kx0, (size_t)nxy * K));
+        if (g.noise) QL_RC(dalloc(&A->st_pre, nl * K));
         QL_RC(dalloc(&A->gu, nl));
         QL_RC(dalloc(&A->gnp, nl));
         QL_RC(dalloc(&A->gM, nm));
@@ -258,7 +263,13 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
                                   b.s));
             QL_RC(set_clock(b, t));
             for (int s = 1; s <= 3; ++s) {
-                QL_RC(enqueue_substep(b, g, s, s == 3, false, b.s));
+                // with noise the step's remap follows the noise (main.cpp:111-123)
+                QL_RC(enqueue_substep(b, g, s, s == 3 && !g.noise, false, b.s));
+                if (s == 3 && g.noise) {
+                    QL_HIP(hipMemcpyAsync(A->st_pre + (size_t)k * nl, g.u, sizeof(double2) * nl,
+                                          hipMemcpyDeviceToDevice, b.s));
+                    QL_RC(launch_noise(g, false, t, dt, true, r.p.remap != 0, b.s));
+                }
                 const size_t j = (size_t)3 * k + s;
                 QL_HIP(hipMemcpyAsync(A->st_u + j * nl, g.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s));
                 QL_HIP(hipMemcpyAsync(A->st_mf + j * nm, g.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s));
@@ -313,7 +324,8 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
 
                 qlk::AdjModeArgs am{};
                 am.u_in = A->st_u + jin * nl;
-                am.u_out = A->st_u + jout * nl;
+                // the substep's own output: before the step's noise at s = 3
+                am.u_out = (s == 3 && g.noise) ? A->st_pre + (size_t)k * nl : A->st_u + jout * nl;
                 am.gu = A->gu;
                 am.gnp = A->gnp;
                 am.kx0 = A->st_kx0 + (size_t)k * nxy;
@@ -340,6 +352,10 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
                 am.tso = ht.tso;
                 am.dtlo = ht.dtlo;
                 am.tnew = ht.tnew;
+                am.noise = g.noise ? 1 : 0;
+                if (g.noise) am.np = g.h_np[0];
+                am.t_step = st_t[k];
+                am.dt_step = dt;
 
                 qlk::AdjMF2Args ab{};
                 ab.gM = A->gM;

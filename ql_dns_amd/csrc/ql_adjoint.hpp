@@ -33,6 +33,7 @@
 #include <type_traits>
 
 #include "ql_mode.hpp"
+#include "ql_noise.hpp"
 
 namespace qlk {
 
@@ -64,6 +65,11 @@ struct AdjModeArgs {
     double rsf, remap_thr, fac;
     double linC, nlc1, nlc2;
     double ts, dtl, tso, dtlo, tnew;  // substep times (sub_times of the forward)
+    // driving noise added after substep 3 (before the remap): its multipliers
+    // depend on q through kx(t) at the step start t_step
+    NoiseParams np;
+    int noise;
+    double t_step, dt_step;
 };
 
 // first-pass source of the reverse transforms: compact kz-slot lines
@@ -203,6 +209,27 @@ __global__ __launch_bounds__(NT) void k_adj_mode(AdjModeArgs a)
             double2 gout[4], GN[4], gdir[4];
             const bool hit = remap_now && remap_hit(remap_qt, a.remap_thr, skx0[mb], ky);
             for (int f = 0; f < 4; ++f) gout[f] = hit ? make_double2(0.0, 0.0) : Gu[f * NZD];
+            // the step's driving noise (ql_noise.hpp) is state independent; its
+            // ux and bx multipliers sqrt(nf lap2 / lapF) depend on q through
+            // lapF = -kx(t)^2 + lap2, d/dq = mult kx t ky / lapF (ky = 0 modes:
+            // none); the band mask is held fixed like the remap decision
+            if (a.noise && a.s == 3 && !hit && a.np.sigma != 0.0 && ky != 0.0) {
+                const NoiseParams &np = a.np;
+                const double tn = a.t_step;
+                const double kxn = rn_add(skx0[mb], rn_mul(rn_mul(q, tn), ky));
+                const double lap2n = rn_add(rn_mul(-ky, ky), kz2);
+                const double lapF = rn_add(rn_mul(-kxn, kxn), lap2n);
+                if (lapF > -np.hi2 && lapF < -np.lo2) {
+                    const double nf = rn_mul(a.dt_step, np.nfac);
+                    const double mu = sqrt(rn_mul(nf, lap2n) / lapF);
+                    double2 z[4];
+                    noise_normals(np.seed, tn, i, m, z);
+                    double gz = 0.0;
+                    if (np.drive_u) gz += redot(z[0], gout[0]);
+                    if (np.drive_b) gz += redot(z[2], gout[2]);
+                    gq += mu * kxn * (tn * ky) / lapF * np.sigma * gz;
+                }
+            }
             const size_t off = (size_t)i * 4 * NZD + m;
             for (int pr = 0; pr < 2; ++pr) {
                 const double coef = pr == 0 ? ph.nu : ph.eta;

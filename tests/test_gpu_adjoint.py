@@ -31,9 +31,11 @@ def _loss_parts(p, lin, mf, alpha):
     return L, 2.0 * alpha * lin, g_mf
 
 
-def _run(p, lin, mf, n):
+def _run(p, lin, mf, n, noise=None):
     from ql_dns_amd.capi import QLContext
     c = QLContext(p)
+    if noise:
+        c.set_noise(**noise)
     c.set_state(lin, mf)
     c.linear_ops_init(0.0)
     c.steps(n, 0.0)
@@ -134,6 +136,12 @@ def test_adjoint_rejects_unsupported():
         c.adjoint_backward(None, None)  # no forward yet
     with pytest.raises(QLError):
         c.adjoint_forward(2, 0, 0.0)
+    # the backward recomputes the forward: the noise may not change in between
+    c.set_noise(5.0)
+    c.adjoint_forward(2, 1, 0.0)
+    c.set_noise(0.0)
+    with pytest.raises(QLError):
+        c.adjoint_backward(None, None)
     c.close()
 
 
@@ -185,3 +193,58 @@ def test_sensitivity_box_harness_matches_finite_differences():
                   float(Rm) - EPS * vR)
         fd = (Lp - Lm) / (2 * EPS)
         assert abs(ad - fd) <= TOL * abs(fd), f"direction {d}: adjoint {ad!r} fd {fd!r}"
+
+
+@pytest.mark.parametrize("model,integrator", [(0, 1), (0, 0), (1, 1)])
+def test_adjoint_with_driving_noise_matches_finite_differences(model, integrator):
+    """Driven runs (DrivingNoise after every step, then the remap): the noise
+    is a function of t, the mode and the slot only, so the same inputs redraw
+    the same noise and finite differences apply.  Its ux and bx multipliers
+    depend on q through kx(t); the noise is large here (f_noise 40) so that a
+    missing q term would show well above the 1e-6 bar, and the test also
+    checks that the noise changes dL/dq."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state, mode_tables
+    p = _p(model=model, NX=8, NY=8, NZ=32, Ly=1.0, B0z=0.03, integrator=integrator, nu=1 / 1500, eta=1 / 1200)
+    noise = dict(f_noise=40.0, seed=0x5EED0011)
+    n, every = 14, 5
+    lin0, mf0 = initial_state(p)
+    alpha = 0.01 / float((np.abs(lin0) ** 2).sum())
+    grads = {}
+    for nz in (None, noise):
+        c = QLContext(p)
+        if nz:
+            c.set_noise(**nz)
+        c.set_state(lin0, mf0)
+        c.linear_ops_init(0.0)
+        c.adjoint_forward(n, every, 0.0)
+        linT, mfT, kxT = c.get_state()
+        assert not np.array_equal(kxT, mode_tables(p)[0]), "no remap happened"
+        _, g_lin, g_mf = _loss_parts(p, linT, mfT, alpha)
+        grads[bool(nz)] = c.adjoint_backward(g_lin, g_mf)
+        c.close()
+    gl0, gm0, gpar = grads[True]
+    assert abs(gpar[0] - grads[False][2][0]) > 1e-3 * abs(gpar[0])  # the noise matters to dL/dq
+    rng = np.random.default_rng(11)
+    rms_l = np.sqrt(np.mean(np.abs(lin0) ** 2))
+    rms_m = np.sqrt(np.mean(np.abs(mf0) ** 2))
+    for d in range(2):
+        vl = (rng.standard_normal(lin0.shape) + 1j * rng.standard_normal(lin0.shape)) * rms_l
+        if model == 0:
+            vl[::p.NY // 2, 0::2, 0] = 0
+        vm = (rng.standard_normal(mf0.shape) + 1j * rng.standard_normal(mf0.shape)) * rms_m
+        ad = float(np.sum(gl0.real * vl.real + gl0.imag * vl.imag) + np.sum(gm0.real * vm.real + gm0.imag * vm.imag))
+        Lp = _loss_parts(p, *_run(p, lin0 + EPS * vl, mf0 + EPS * vm, n, noise), alpha)[0]
+        Lm = _loss_parts(p, *_run(p, lin0 - EPS * vl, mf0 - EPS * vm, n, noise), alpha)[0]
+        fd = (Lp - Lm) / (2 * EPS)
+        assert abs(ad - fd) <= TOL * abs(fd), f"state direction {d}: adjoint {ad!r} fd {fd!r}"
+    for k, name in enumerate(PARAMS[:4]):
+        if model == 1 and name == "omega":
+            continue
+        base = getattr(p, name)
+        h = EPS * abs(base)
+        Lp = _loss_parts(p, *_run(p.replace(**{name: base + h}), lin0, mf0, n, noise), alpha)[0]
+        Lm = _loss_parts(p, *_run(p.replace(**{name: base - h}), lin0, mf0, n, noise), alpha)[0]
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
+

@@ -192,11 +192,3 @@ def test_noise_in_run_sets():
         assert np.array_equal(kg, kb)
         assert rel_err(lb, lg) < 1e-10 and rel_err(mb, mg) < 1e-10
 
-
-def test_adjoint_needs_noise_off():
-    from ql_dns_amd.capi import QLContext, QLError
-    g = QLContext(_params())
-    g.set_noise(0.1)
-    with pytest.raises(QLError) as e:
-        g.adjoint_forward(4, 2, 0.0)
-    assert e.value.code == -3
