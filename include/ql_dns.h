@@ -270,18 +270,20 @@ int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
 /* ------------------------------------------------------------------------
  * Adjoint (reverse-mode gradient) of nsteps of ql_steps, for a single-run
- * MHD_FullQlin or MHD_BQlin context in FP64 with fixed steps, driven or not
- * (either integrator, remap on or off).  The reference has no adjoint;
+ * MHD_FullQlin, MHD_BQlin or HD_fullU context in FP64 with fixed steps, driven
+ * or not (either integrator, remap on or off).  The reference has no adjoint;
  * these entry points differentiate the reference's step
  * (Models/MHD_FullQlin.cpp:171-427, Models/MHD_BQlin.cpp:117-306,
  * Integrators/RK3CN.cpp:77-201, Integrators/RK3CN_Elin.cpp:74-188,
  * main.cpp:108-123) as this library computes it.  Remap decisions are held
  * fixed (piecewise constant in q).  MHD_BQlin rotates at 2 q / 3
- * (MHD_BQlin.cpp:19), so its omega and B0z gradients are zero.
+ * (MHD_BQlin.cpp:19), so its omega and B0z gradients are zero; HD_fullU has
+ * no eta and no B0z (zero gradients) and its layouts are its own (nLin = 2,
+ * nMF = 2, as ql_dims reports).
  *
  * ql_adjoint_forward: nsteps of ql_steps from the current state, storing a
  * checkpoint every `every` steps (device memory ~ (nsteps/every + 3 every + 8)
- * states).  ql_adjoint_backward: given dL/d(final state) (g_lin [nxy][4][nz]
+ * states).  ql_adjoint_backward: given dL/d(final state) (g_lin [nxy][nLin][nz]
  * complex, g_mf [nMF][nz] complex, either may be NULL for zero; complex
  * gradients as dL/dRe + i dL/dIm), returns dL/d(state before the forward) in
  * g_lin0 / g_mf0 (NULL to skip) and g_par[0..4] = dL/d(q, omega, nu, eta,

@@ -307,9 +307,9 @@ class QLContext:
     def adjoint_backward(self, g_lin=None, g_mf=None):
         """dL/d(initial lin, mf) and dL/d(q, omega, nu, eta, B0z) for the given
         dL/d(final lin, mf) (complex arrays; None = zero)."""
-        g_lin = _checked(g_lin, np.complex128, self.nxy * 4 * self.nz)
+        g_lin = _checked(g_lin, np.complex128, self.nxy * self.nLin * self.nz)
         g_mf = _checked(g_mf, np.complex128, self.nMF * self.nz)
-        gl0 = np.zeros((self.nxy, 4, self.nz), dtype=np.complex128)
+        gl0 = np.zeros((self.nxy, self.nLin, self.nz), dtype=np.complex128)
         gm0 = np.zeros((self.nMF, self.nz), dtype=np.complex128)
         gp = np.zeros(5)
         self._check(self._L.ql_adjoint_backward(self._h, _dp(g_lin), _dp(g_mf), _dp(gl0), _dp(gm0), _dp(gp)))

@@ -66,8 +66,8 @@ int adj_line_mb(int NZ) { return NZ >= 128 ? 1 : 128 / NZ; }
 int check_adjoint(const Batch &b, const char *who)
 {
     if (b.runs.size() != 1) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is for single-run contexts", who);
-    if ((b.model != QL_MODEL_MHD_FULLQLIN && b.model != QL_MODEL_MHD_BQLIN) || b.hd)
-        return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin and MHD_BQlin", who);
+    if (b.model != QL_MODEL_MHD_FULLQLIN && b.model != QL_MODEL_MHD_BQLIN && !b.hd)
+        return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin, MHD_BQlin and HD_fullU", who);
     if (b.precision != QL_PREC_FP64) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint runs in FP64 only", who);
     if (b.vdt) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs fixed steps (dt > 0)", who);
     return QL_OK;
@@ -236,10 +236,25 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
         QL_HIP(hipMemcpy(A->rp1, &rp, sizeof(rp), hipMemcpyHostToDevice));
         A->W = W;
     }
-    if (g_lin) QL_HIP(hipMemcpyAsync(A->gu, g_lin, sizeof(double2) * nl, hipMemcpyHostToDevice, b.s));
-    else QL_HIP(hipMemsetAsync(A->gu, 0, sizeof(double2) * nl, b.s));
-    if (g_mf) QL_HIP(hipMemcpyAsync(A->gM, g_mf, sizeof(double2) * nm, hipMemcpyHostToDevice, b.s));
-    else QL_HIP(hipMemsetAsync(A->gM, 0, sizeof(double2) * nm, b.s));
+    // HD_fullU runs the MHD_FullQlin kernels with b = B = 0 on the internal
+    // four-field layout: its caller's (ux, zeta) and (Ux, Uy) gradients sit at
+    // fields 0, 1 and mean-field slots 2, 3, the others are zero
+    std::vector<double2> hl, hm;
+    if (b.hd) {
+        hl.assign(nl, make_double2(0.0, 0.0));
+        hm.assign(nm, make_double2(0.0, 0.0));
+        if (g_lin)
+            for (int i = 0; i < nxy; ++i)
+                std::memcpy(&hl[(size_t)i * 4 * nz], g_lin + (size_t)i * 2 * nz * 2, sizeof(double2) * 2 * nz);
+        if (g_mf) std::memcpy(&hm[(size_t)2 * nz], g_mf, sizeof(double2) * 2 * nz);
+        QL_HIP(hipMemcpy(A->gu, hl.data(), sizeof(double2) * nl, hipMemcpyHostToDevice));
+        QL_HIP(hipMemcpy(A->gM, hm.data(), sizeof(double2) * nm, hipMemcpyHostToDevice));
+    } else {
+        if (g_lin) QL_HIP(hipMemcpyAsync(A->gu, g_lin, sizeof(double2) * nl, hipMemcpyHostToDevice, b.s));
+        else QL_HIP(hipMemsetAsync(A->gu, 0, sizeof(double2) * nl, b.s));
+        if (g_mf) QL_HIP(hipMemcpyAsync(A->gM, g_mf, sizeof(double2) * nm, hipMemcpyHostToDevice, b.s));
+        else QL_HIP(hipMemsetAsync(A->gM, 0, sizeof(double2) * nm, b.s));
+    }
     QL_HIP(hipMemsetAsync(A->gnp, 0, sizeof(double2) * nl, b.s));
     QL_HIP(hipMemsetAsync(A->gnmp, 0, sizeof(double2) * nm, b.s));
     QL_HIP(hipMemsetAsync(A->gpar, 0, sizeof(double) * qlk::NGP, b.s));
@@ -374,11 +389,22 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
     }
     // leave the context at the forward's final state
     QL_RC(load_ck(b, A->ck[A->nseg]));
-    if (g_lin0) QL_HIP(hipMemcpyAsync(g_lin0, A->gu, sizeof(double2) * nl, hipMemcpyDeviceToHost, b.s));
-    if (g_mf0) QL_HIP(hipMemcpyAsync(g_mf0, A->gM, sizeof(double2) * nm, hipMemcpyDeviceToHost, b.s));
+    if (b.hd) {
+        QL_HIP(hipStreamSynchronize(b.s));
+        QL_HIP(hipMemcpy(hl.data(), A->gu, sizeof(double2) * nl, hipMemcpyDeviceToHost));
+        QL_HIP(hipMemcpy(hm.data(), A->gM, sizeof(double2) * nm, hipMemcpyDeviceToHost));
+        if (g_lin0)
+            for (int i = 0; i < nxy; ++i)
+                std::memcpy(g_lin0 + (size_t)i * 2 * nz * 2, &hl[(size_t)i * 4 * nz], sizeof(double2) * 2 * nz);
+        if (g_mf0) std::memcpy(g_mf0, &hm[(size_t)2 * nz], sizeof(double2) * 2 * nz);
+    } else {
+        if (g_lin0) QL_HIP(hipMemcpyAsync(g_lin0, A->gu, sizeof(double2) * nl, hipMemcpyDeviceToHost, b.s));
+        if (g_mf0) QL_HIP(hipMemcpyAsync(g_mf0, A->gM, sizeof(double2) * nm, hipMemcpyDeviceToHost, b.s));
+    }
     double gp[qlk::NGP];
     QL_HIP(hipMemcpyAsync(gp, A->gpar, sizeof(double) * qlk::NGP, hipMemcpyDeviceToHost, b.s));
     QL_HIP(hipStreamSynchronize(b.s));
+    if (b.hd) gp[3] = gp[4] = 0.0;  // HD_fullU has no eta and no B0z
     if (b.model == QL_MODEL_MHD_BQLIN) {
         // MHD_BQlin rotates at 2 q / 3 whatever omega is (MHD_BQlin.cpp:19): the
         // kernels' rotation gradient belongs to q

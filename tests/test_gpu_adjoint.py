@@ -126,7 +126,7 @@ def test_adjoint_independent_of_checkpoint_spacing(model):
 
 def test_adjoint_rejects_unsupported():
     from ql_dns_amd.capi import QLContext, QLError
-    for kw in (dict(model=2), dict(precision=1)):
+    for kw in (dict(precision=1), dict(dt=-0.01)):
         c = QLContext(_p(NX=4, NY=4, NZ=32, **kw))
         with pytest.raises(QLError):
             c.adjoint_forward(2, 1, 0.0)
@@ -246,5 +246,49 @@ def test_adjoint_with_driving_noise_matches_finite_differences(model, integrator
         Lp = _loss_parts(p, *_run(p.replace(**{name: base + h}), lin0, mf0, n, noise), alpha)[0]
         Lm = _loss_parts(p, *_run(p.replace(**{name: base - h}), lin0, mf0, n, noise), alpha)[0]
         fd = (Lp - Lm) / (2 * h)
+        assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
+
+
+@pytest.mark.parametrize("integrator,NZ", [(1, 32), (0, 32), (1, 128)])
+def test_hd_adjoint_matches_finite_differences(integrator, NZ):
+    """HD_fullU (two fluctuation fields, mean flows Ux, Uy in its own layout):
+    the MHD_FullQlin reverse kernels with b = B = 0; dL/deta and dL/dB0z are
+    zero.  Loss: 0.5/NZ^2 sum |Ux|^2 + |Uy|^2 + alpha sum |u|^2."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state, mode_tables
+    p = _p(model=2, NX=8, NY=8, NZ=NZ, Ly=1.0, integrator=integrator, nu=1 / 1500)
+    n, every = 14, 5
+    lin0, mf0 = initial_state(p)
+    assert lin0.shape[1] == 2 and mf0.shape[0] == 2
+    alpha = 0.01 / float((np.abs(lin0) ** 2).sum())
+
+    def loss(lin, mf):
+        return 0.5 * float((np.abs(mf) ** 2).sum()) / p.NZ ** 2 + alpha * float((np.abs(lin) ** 2).sum())
+
+    c = QLContext(p)
+    c.set_state(lin0, mf0)
+    c.linear_ops_init(0.0)
+    c.adjoint_forward(n, every, 0.0)
+    linT, mfT, kxT = c.get_state()
+    assert not np.array_equal(kxT, mode_tables(p)[0]), "no remap happened"
+    gl0, gm0, gpar = c.adjoint_backward(2.0 * alpha * linT, mfT / p.NZ ** 2)
+    c.close()
+    assert gl0.shape == lin0.shape and gm0.shape == mf0.shape
+    assert gpar[3] == 0.0 and gpar[4] == 0.0
+    rng = np.random.default_rng(5)
+    rms_l = np.sqrt(np.mean(np.abs(lin0) ** 2))
+    rms_m = np.sqrt(np.mean(np.abs(mf0) ** 2))
+    for d in range(3):
+        vl = (rng.standard_normal(lin0.shape) + 1j * rng.standard_normal(lin0.shape)) * rms_l
+        vl[::p.NY // 2, 0, 0] = 0  # ux at ky = kz = 0: states stay divergence free
+        vm = (rng.standard_normal(mf0.shape) + 1j * rng.standard_normal(mf0.shape)) * rms_m
+        ad = float(np.sum(gl0.real * vl.real + gl0.imag * vl.imag) + np.sum(gm0.real * vm.real + gm0.imag * vm.imag))
+        fd = (loss(*_run(p, lin0 + EPS * vl, mf0 + EPS * vm, n)) - loss(*_run(p, lin0 - EPS * vl, mf0 - EPS * vm, n))) / (2 * EPS)
+        assert abs(ad - fd) <= TOL * abs(fd), f"state direction {d}: adjoint {ad!r} fd {fd!r}"
+    for k, name in enumerate(("q", "omega", "nu")):
+        base = getattr(p, name)
+        h = EPS * abs(base)
+        fd = (loss(*_run(p.replace(**{name: base + h}), lin0, mf0, n)) -
+              loss(*_run(p.replace(**{name: base - h}), lin0, mf0, n))) / (2 * h)
         assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
 
