@@ -416,10 +416,11 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
 // ---------------------------------------------------------------------------
 // launches
 // ---------------------------------------------------------------------------
-// A mode-kernel launch.  In the per-kernel timing pass the launch carries the
-// batch's start / stop events itself (hipExtLaunchKernel), so they hold the
+// A mode-kernel launch.  In the per-kernel timing pass the launch carries its
+// start and / or stop event itself (hipExtLaunchKernel), so they hold the
 // dispatch's own begin and end times rather than those of separate event
-// packets around it; `first` / `last` say which of them this launch sets.
+// packets around it; a null event is not set (the companion launch of the
+// mixed path sets the start, the FP32 launch after it the stop).
 struct LaunchTiming {
     hipEvent_t start = nullptr, stop = nullptr;
 };
