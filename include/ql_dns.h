@@ -256,8 +256,10 @@ int ql_batch_set_transform(ql_batch *b, int NZ, int transform);
  * times the largest of its run runs in FP64 (a companion launch of the FP64
  * kernel before the FP32 one), the others in FP32; the choice is made again
  * every 16 steps of the clock and after each state upload.  Sets tau when
- * tau > 0 (default 0.03; tau <= 0 leaves it) and writes to *nfp64 (may be
- * NULL) how many modes of the set reach it for the current state. */
+ * tau > 0 (default 0.03; tau <= 0 leaves it; tau > 1, which no mode reaches,
+ * runs every mode in FP32 with neither the flags nor the FP64 launch) and
+ * writes to *nfp64 (may be NULL) how many modes of the set reach it for the
+ * current state. */
 int ql_batch_mixed_split(ql_batch *b, double tau, int *nfp64);
 /* current shape per group: cfg[6k..6k+5] = NZ, line, target, modes per
  * workgroup pass, workgroups, transform (QL_TRANSFORM_*); *ngroups receives

@@ -611,9 +611,13 @@ qlk::ModeArgs mode_args(const Batch &b, const Group &g)
 
 // mixed-precision split of a mode launch: the FP32 launch skips the flagged
 // modes, which the FP64 companion takes
-void set_split(const Group &g, qlk::ModeArgs &a)
+// mixed precision with tau > 1: no mode can reach it, so every mode runs in
+// FP32 and neither the flags nor the companion are launched
+bool split_active(const Batch &b, const Group &g) { return g.split && b.amp_tau <= 1.0; }
+
+void set_split(const Batch &b, const Group &g, qlk::ModeArgs &a)
 {
-    if (!g.split) return;
+    if (!split_active(b, g)) return;
     a.split = qlk::SPLIT_LO;
     a.hi = g.hi;
 }
@@ -630,7 +634,7 @@ void launch_flags(const Batch &b, const Group &g, bool clocked, hipStream_t st)
 
 int prime_split(Batch &b, Group &g, hipStream_t st)
 {
-    if (!g.split || !g.amp_stale) return QL_OK;
+    if (!split_active(b, g) || !g.amp_stale) return QL_OK;
     launch_flags(b, g, false, st);
     QL_HIP(hipGetLastError());
     g.amp_stale = false;
@@ -771,7 +775,7 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
     qlk::ModeArgs a = mode_args(b, g);
     a.s = s;
     a.mode = cn ? qlk::MODE_CN : qlk::MODE_ELIN;
-    set_split(g, a);
+    set_split(b, g, a);
     a.linC = lc;
     a.nlc1 = nlc1;
     a.nlc2 = nlc2;
@@ -811,7 +815,7 @@ int enqueue_substep_all(Batch &b, int s, bool remap_after, bool advance)
 int enqueue_flags(Batch &b)
 {
     for (Group &g : b.groups)
-        if (g.split) launch_flags(b, g, true, b.s);
+        if (split_active(b, g)) launch_flags(b, g, true, b.s);
     QL_HIP(hipGetLastError());
     return QL_OK;
 }
@@ -1448,7 +1452,7 @@ int ql_rhs(ql_ctx *c, double t, double dt_lin, double *rhs_lin, double *rhs_mf, 
     qlk::ModeArgs a = mode_args(b, g);
     a.s = 0;
     a.mode = qlk::MODE_RHS;
-    set_split(g, a);
+    set_split(b, g, a);
     a.ts_x = t;
     a.dtl_x = dt_lin;
     a.n_out = g.N1;
@@ -1498,7 +1502,7 @@ int ql_rhs_noalloc(ql_ctx *c, double t, double dt_lin, double *d_rhs_lin, double
     qlk::ModeArgs a = mode_args(b, g);
     a.s = 0;
     a.mode = qlk::MODE_RHS;
-    set_split(g, a);
+    set_split(b, g, a);
     a.ts_x = t;
     a.dtl_x = dt_lin;
     a.n_out = reinterpret_cast<double2 *>(d_rhs_lin);
@@ -1872,7 +1876,7 @@ int ql_batch_mixed_split(ql_batch *bt, double tau, int *nfp64)
     if (!bt) return fail(QL_ERR_ARG, "ql_batch_mixed_split: null handle");
     Batch &b = bt->b;
     if (b.precision != QL_PREC_MIXED) return fail(QL_ERR_ARG, "ql_batch_mixed_split: not a mixed-precision set");
-    if (!(tau == tau) || tau > 1.0) return fail(QL_ERR_ARG, "ql_batch_mixed_split: tau %g outside (0, 1]", tau);
+    if (!(tau == tau) || std::isinf(tau)) return fail(QL_ERR_ARG, "ql_batch_mixed_split: tau %g not finite", tau);
     if (tau > 0.0) {
         b.amp_tau = tau;
         drop_graphs(b);  // the fraction is a launch argument of the captured steps
@@ -1882,6 +1886,7 @@ int ql_batch_mixed_split(ql_batch *bt, double tau, int *nfp64)
         QL_RC(prime_split_all(b));
         int n = 0;
         for (Group &g : b.groups) {
+            if (!split_active(b, g)) continue;  // tau > 1: every mode in FP32
             std::vector<unsigned char> hi(g.modes);
             QL_HIP(hipMemcpyAsync(hi.data(), g.hi, g.modes, hipMemcpyDeviceToHost, b.s));
             QL_HIP(hipStreamSynchronize(b.s));

@@ -136,10 +136,45 @@ def test_mixed_split_count_and_arguments():
     assert 1 < n_default < p.nxy - 1          # a strict subset
     assert c.mixed_split(1.0) == 1            # the largest mode only
     assert c.mixed_split(1e-300) == p.nxy - 1
+    assert c.mixed_split(1.5) == 0            # no mode reaches it: all FP32
     with pytest.raises(QLError):
-        c.mixed_split(1.5)
+        c.mixed_split(float("inf"))
     c.close()
     f = QLContext(p.replace(precision=0))
     with pytest.raises(QLError):
         f.mixed_split(0.1)
     f.close()
+
+
+def test_mixed_without_split_runs_all_fp32():
+    """tau > 1 switches the FP64 companion off: the steps then differ from
+    the split run's, stay close to FP64 over a short run, and repeat bitwise;
+    tau back at its default restores the split run's bits."""
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import PREC_MIXED, QLParams
+    p = QLParams(NX=16, NY=16, NZ=64, dt=0.005, precision=PREC_MIXED)
+    lin, mf = initial_state(p)
+
+    def run(tau):
+        c = _ctx(p, lin, mf)
+        if tau:
+            c.mixed_split(tau)
+        t = c.steps(40, 0.0)
+        out = c.get_state()
+        c.close()
+        return t, out
+
+    t_split, s_split = run(None)
+    t_off, s_off = run(2.0)
+    t_off2, s_off2 = run(2.0)
+    t_back, s_back = run(0.03)
+    f = _ctx(p.replace(precision=0), lin, mf)
+    f.steps(40, 0.0)
+    s64 = f.get_state()
+    f.close()
+    assert t_split == t_off == t_back
+    assert all(np.array_equal(x, y) for x, y in zip(s_off, s_off2))
+    assert all(np.array_equal(x, y) for x, y in zip(s_split, s_back))
+    assert not np.array_equal(s_off[0], s_split[0])
+    assert rel_err(s_off[1], s64[1]) < 1e-6 and rel_err(s_off[0], s64[0]) < 1e-6
+
