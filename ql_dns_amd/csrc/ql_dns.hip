@@ -245,8 +245,11 @@ int configure(Batch &b, Group &g, int line, int target)
     // mixed precision: the FP64 companion (one-wavefront-per-mode kernel);
     // workgroup j of a run's nw2 has g0 = j, g1 = nw2 and strides over the
     // run's modes
+    // (the wavefront kernel takes its flagged modes in FP64 itself,
+    // k_wave_mix, and has no companion)
     std::vector<qlk::WgInfo> wi2;
-    if (g.split) {
+    for (qlk::RunParams &rp : g.h_rp) rp.wg_off2 = rp.nwg2 = 0;
+    if (g.split && line != LINE_WAVE) {
         for (int k = 0; k < g.nruns; ++k) {
             qlk::RunParams &rp = g.h_rp[k];
             const int nw2 = comp_nwg(rp.nxy);
@@ -469,7 +472,14 @@ void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
     constexpr int TB = qlk::mode_waves<qlk::ModelBQ>(NZ) * qlk::WAVE;
     const bool bq = b.model == QL_MODEL_MHD_BQLIN;
     // HD_fullU: the velocity-only policy (its magnetic fields are zero)
-    if (b.precision == QL_PREC_MIXED) {
+    if (b.precision == QL_PREC_MIXED && a.split == qlk::SPLIT_LO) {
+        // flagged modes in FP64, the others in FP32, in one launch
+        if (b.hd) launch_k(qlk::k_wave_mix<NZ, qlk::ModelHD, MODE>, g.W, TH, st, tm, a);
+        else if constexpr (NZ <= 128) {  // wave_ok: no other FP32 wavefront kernel at NZ = 256
+            if (bq) launch_k(qlk::k_wave_mix<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
+            else launch_k(qlk::k_wave_mix<NZ, qlk::ModelFull, MODE>, g.W, TF, st, tm, a);
+        }
+    } else if (b.precision == QL_PREC_MIXED) {
         if (b.hd) launch_k(qlk::k_wave32<NZ, qlk::ModelHD, MODE>, g.W, TH, st, tm, a);
         else if constexpr (NZ <= 128) {  // wave_ok: no other FP32 wavefront kernel at NZ = 256
             if (bq) launch_k(qlk::k_wave32<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
@@ -709,7 +719,7 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
         b.tused += 3;
     }
     const int ncols = b.nMF * g.NZ;
-    if (a.split == qlk::SPLIT_LO) {
+    if (a.split == qlk::SPLIT_LO && g.line != LINE_WAVE) {
         // the FP64 companion first, then the FP32 launch (the FP32 launch
         // fills every CU in one round of workgroups, so running them side by
         // side would only delay the workgroups the companion displaces)

@@ -124,7 +124,9 @@ struct WaveCompactDst {
     }
 };
 
-template <int NZ, class P, class C, int MODE, int NW, bool SPL>
+// MIXW (C = float2): mixed precision in one launch, each mode in FP64 when
+// its flag is set and in FP32 otherwise (no companion launch)
+template <int NZ, class P, class C, int MODE, int NW, bool SPL, bool MIXW = false>
 __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
 {
     typedef RealOf<C> R;
@@ -139,7 +141,12 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     __shared__ R skzr[NZD];
     __shared__ double skz[NZD];  // FP64 wavenumbers of B and F
     __shared__ R smfz[NZ * P::NMFZ];
-    __shared__ C wbuf[NW][L::BUF];
+    // MIXW: the buffer holds an FP64 mode, and FP64 copies of the tables
+    constexpr int BUFC = MIXW ? ((int)(WaveLayout<NZ, P, double2>::BUF * sizeof(double2)) + (int)sizeof(C) - 1) / (int)sizeof(C)
+                              : L::BUF;
+    __shared__ C wbuf[NW][BUFC > L::BUF ? BUFC : L::BUF];
+    __shared__ double2 tw64[MIXW ? NZ : 1];
+    __shared__ double smfz64[MIXW ? NZ * P::NMFZ : 1];
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid / WAVE), lane = tid % WAVE;
     C *buf = wbuf[wave];
@@ -179,6 +186,14 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
 #pragma unroll
         for (int w = 0; w < MFN; ++w)
             if (tid + w * NTH < NZ * P::NMFZ) smfz[tid + w * NTH] = (R)mfv[w];
+        if constexpr (MIXW) {
+#pragma unroll
+            for (int w = 0; w < TWN; ++w)
+                if (tid + w * NTH < NZ) tw64[tid + w * NTH] = a.tw[tid + w * NTH];
+#pragma unroll
+            for (int w = 0; w < MFN; ++w)
+                if (tid + w * NTH < NZ * P::NMFZ) smfz64[tid + w * NTH] = mfv[w];
+        }
     }
     double acc[ZI][P::NACC];
 #pragma unroll
@@ -195,7 +210,7 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
     int cbase = g0 + wave;
     for (int l = g0 + wave; l < g1; l += NW) {
         const int i = mode_off + l;
-        if constexpr (SPL) {
+        if constexpr (SPL && !MIXW) {
             if (a.split == SPLIT_LO) {
                 // the flags of this wavefront's next 64 modes, one per lane in
                 // one round trip, then read from a ballot
@@ -207,162 +222,179 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
                 if ((skip >> ((l - cbase) / NW)) & 1ull) continue;
             }
         }
-        const double kx0 = to_sgpr(a.kx0[i]), ky = to_sgpr(a.ky[i]);
-        const double kx = to_sgpr(rn_add(kx0, rn_mul(qt, ky)));  // kx(t), MHD_FullQlin.cpp:727
-        const size_t row = (size_t)i * 4 * NZD;
+        // one mode in precision CC (the FP64 or the FP32 transforms), its
+        // line buffer and tables
+        auto pass = [&](auto ctag, auto *bufc, const auto *twc, const auto *skzrc, const auto *smfzc) {
+            using CC = decltype(ctag);
+            typedef RealOf<CC> RR;
+            using LL = WaveLayout<NZ, P, CC>;
+            const double kx0 = to_sgpr(a.kx0[i]), ky = to_sgpr(a.ky[i]);
+            const double kx = to_sgpr(rn_add(kx0, rn_mul(qt, ky)));  // kx(t), MHD_FullQlin.cpp:727
+            const size_t row = (size_t)i * 4 * NZD;
 
-        // B: 1/lap2, 1/lapF and the base spectral fields (MHD_FullQlin.cpp:220-221,
-        // 722-738), fac1D folded in
-#pragma unroll
-        for (int k = 0; k < KI; ++k) {
-            if (k > 0) __builtin_amdgcn_sched_barrier(0);  // one slot at a time (registers)
-            const int m = lane + k * WAVE;
-            if (KI * WAVE == NZD || m < NZD) {
-                const double2 u0 = a.u[row + m], u1 = a.u[row + NZD + m];
-                double2 u2 = make_double2(0.0, 0.0), u3 = u2;
-                if constexpr (P::NPAIR == 2) {
-                    u2 = a.u[row + 2 * NZD + m];
-                    u3 = a.u[row + 3 * NZD + m];
-                }
-                const double kzk = skz[m], kz2 = -(kzk * kzk);
-                const bool zero = (ky == 0.0 && m == 0);
-                const double lap2 = zero ? 0.0 : (-ky * ky + kz2);
-                const double ilap2 = zero ? 1.0 : 1.0 / lap2;
-                P::bases(u0, u1, u2, u3, kx, ky, kzk, ilap2, a.fac, buf + m, L::NZDP);
-            }
-        }
-        wave_sync();
-
-        // C: inverse transforms of the padded lines
-        {
-            const PadIO<NZ, C> pad{buf};
-            wave_pass<NZ, P::NLINE, true, PL::R1, 1>(lane, tw, WaveLineSrc<NZ, P, C>{buf, skzr}, pad);
-            wave_pass<NZ, P::NLINE, true, PL::R2, PL::R1>(lane, tw, pad, BufIO<L::LS, C>{buf});
-        }
-
-        // D: products on the z grid, in place, and the EMF / stress sums
-        {
-            const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
-            // ModelBQ's dz uz constant of a ky = 0 mode: i kx fac1D ux at kz = 0
-            const C dzc = from_d<C>(ky == 0.0 ? imul(kx, cscale(a.fac, a.u[row])) : make_double2(0.0, 0.0));
-#pragma unroll
-            for (int k = 0; k < ZI; ++k) {
-                if (k > 0) __builtin_amdgcn_sched_barrier(0);  // one z point at a time (registers)
-                const int zz = lane + k * WAVE;
-                if (ZI * WAVE == NZ || zz < NZ) {
-                    C *X = buf + zz;
-                    R mfl[P::NMFZ];
-#pragma unroll
-                    for (int c = 0; c < P::NMFZ; ++c) mfl[c] = smfz[c * NZ + zz];
-                    C prod[2][3];
-                    double racc[2][2];
-                    P::products(0, X, L::LS, mfl, (R)kx, (R)ky, dzc, prod[0], racc[0]);
-                    P::products(1, X, L::LS, mfl, (R)kx, (R)ky, dzc, prod[1], racc[1]);
-#pragma unroll
-                    for (int part = 0; part < 2; ++part) {
-                        acc[k][P::acc_slot(part, 0)] += ftfac * racc[part][0];
-                        if (P::NACC == 4) acc[k][P::acc_slot(part, 1)] += ftfac * racc[part][1];
+            // B: 1/lap2, 1/lapF and the base spectral fields (MHD_FullQlin.cpp:220-221,
+            // 722-738), fac1D folded in
+    #pragma unroll
+            for (int k = 0; k < KI; ++k) {
+                if (k > 0) __builtin_amdgcn_sched_barrier(0);  // one slot at a time (registers)
+                const int m = lane + k * WAVE;
+                if (KI * WAVE == NZD || m < NZD) {
+                    const double2 u0 = a.u[row + m], u1 = a.u[row + NZD + m];
+                    double2 u2 = make_double2(0.0, 0.0), u3 = u2;
+                    if constexpr (P::NPAIR == 2) {
+                        u2 = a.u[row + 2 * NZD + m];
+                        u3 = a.u[row + 3 * NZD + m];
                     }
-#pragma unroll
-                    for (int part = 0; part < 2; ++part)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            if (c < P::nprod(part)) X[P::out_line(part, c) * L::LS] = prod[part][c];
+                    const double kzk = skz[m], kz2 = -(kzk * kzk);
+                    const bool zero = (ky == 0.0 && m == 0);
+                    const double lap2 = zero ? 0.0 : (-ky * ky + kz2);
+                    const double ilap2 = zero ? 1.0 : 1.0 / lap2;
+                    P::bases(u0, u1, u2, u3, kx, ky, kzk, ilap2, a.fac, bufc + m, LL::NZDP);
                 }
             }
-        }
-        wave_sync();
+            wave_sync();
 
-        // E: forward transforms of the product lines, dealiased band kept
-        {
-            const PadIO<NZ, C> pad{buf};
-            wave_pass<NZ, P::NOUT, false, PL::R1, 1>(lane, tw, BufIO<L::LS, C>{buf}, pad);
-            wave_pass<NZ, P::NOUT, false, PL::R2, PL::R1>(lane, tw, pad, WaveCompactDst<NZ, C>{buf});
-        }
-
-        // F: RHS, dissipation operator at the substep end, update
-        bool hit = false;
-        if (do_remap) hit = remap_hit(remap_qt, rp.remap_thr, kx0, ky);
-#pragma unroll
-        for (int k = 0; k < KI; ++k) {
-            // one slot at a time: the scheduler would otherwise hoist the
-            // loads of every slot of the lane and run out of registers
-            if (k > 0) __builtin_amdgcn_sched_barrier(0);
-            const int m = lane + k * WAVE;
-            if (!(KI * WAVE == NZD || m < NZD)) continue;
-            const size_t off = row + m;
-            const double kzk = skz[m], kz2 = -(kzk * kzk);
-            // 1/lapF as in B (recomputed rather than held through the transforms)
-            const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
-            const double ilapF = (l == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
-            const double2 U0 = a.u[off], U1 = a.u[off + NZD];
-            double2 U2 = make_double2(0.0, 0.0), U3 = U2;
-            if constexpr (P::NPAIR == 2) {
-                U2 = a.u[off + 2 * NZD];
-                U3 = a.u[off + 3 * NZD];
+            // CC: inverse transforms of the padded lines
+            {
+                const PadIO<NZ, CC> pad{bufc};
+                wave_pass<NZ, P::NLINE, true, PL::R1, 1>(lane, twc, WaveLineSrc<NZ, P, CC>{bufc, skzrc}, pad);
+                wave_pass<NZ, P::NLINE, true, PL::R2, PL::R1>(lane, twc, pad, BufIO<LL::LS, CC>{bufc});
             }
-            double2 np[4];
-#pragma unroll
-            for (int f = 0; f < 2 * P::NPAIR; ++f) np[f] = a.n_prev ? a.n_prev[off + f * NZD] : make_double2(0.0, 0.0);
-            const C *T = buf + m;
-#pragma unroll
-            for (int pr = 0; pr < P::NPAIR; ++pr) {
-                double2 N0, N1;
-                P::rhs_pair(pr, T, NZD, U0, U1, U2, U3, kx, ky, kzk, ilapF, ph, N0, N1);
-                const double Ln = P::lin(pr, kx + qdtl * ky, ky, kz2, ph);
-                const size_t off0 = off + (2 * pr) * NZD, off1 = off0 + NZD;
-                if constexpr (MODE == MODE_RHS) {
+
+            // D: products on the z grid, in place, and the EMF / stress sums
+            {
+                const double ftfac = (ky == 0.0 ? 1.0 : 2.0) * rp.rsf;  // MHD_FullQlin.cpp:314-318
+                // ModelBQ's dz uz constant of a ky = 0 mode: i kx fac1D ux at kz = 0
+                const CC dzc = from_d<CC>(ky == 0.0 ? imul(kx, cscale(a.fac, a.u[row])) : make_double2(0.0, 0.0));
+    #pragma unroll
+                for (int k = 0; k < ZI; ++k) {
+                    if (k > 0) __builtin_amdgcn_sched_barrier(0);  // one z point at a time (registers)
+                    const int zz = lane + k * WAVE;
+                    if (ZI * WAVE == NZ || zz < NZ) {
+                        CC *X = bufc + zz;
+                        RR mfl[P::NMFZ];
+    #pragma unroll
+                        for (int c = 0; c < P::NMFZ; ++c) mfl[c] = smfzc[c * NZ + zz];
+                        CC prod[2][3];
+                        double racc[2][2];
+                        P::products(0, X, LL::LS, mfl, (RR)kx, (RR)ky, dzc, prod[0], racc[0]);
+                        P::products(1, X, LL::LS, mfl, (RR)kx, (RR)ky, dzc, prod[1], racc[1]);
+    #pragma unroll
+                        for (int part = 0; part < 2; ++part) {
+                            acc[k][P::acc_slot(part, 0)] += ftfac * racc[part][0];
+                            if (P::NACC == 4) acc[k][P::acc_slot(part, 1)] += ftfac * racc[part][1];
+                        }
+    #pragma unroll
+                        for (int part = 0; part < 2; ++part)
+    #pragma unroll
+                            for (int c = 0; c < 3; ++c)
+                                if (c < P::nprod(part)) X[P::out_line(part, c) * LL::LS] = prod[part][c];
+                    }
+                }
+            }
+            wave_sync();
+
+            // E: forward transforms of the product lines, dealiased band kept
+            {
+                const PadIO<NZ, CC> pad{bufc};
+                wave_pass<NZ, P::NOUT, false, PL::R1, 1>(lane, twc, BufIO<LL::LS, CC>{bufc}, pad);
+                wave_pass<NZ, P::NOUT, false, PL::R2, PL::R1>(lane, twc, pad, WaveCompactDst<NZ, CC>{bufc});
+            }
+
+            // F: RHS, dissipation operator at the substep end, update
+            bool hit = false;
+            if (do_remap) hit = remap_hit(remap_qt, rp.remap_thr, kx0, ky);
+    #pragma unroll
+            for (int k = 0; k < KI; ++k) {
+                // one slot at a time: the scheduler would otherwise hoist the
+                // loads of every slot of the lane and run out of registers
+                if (k > 0) __builtin_amdgcn_sched_barrier(0);
+                const int m = lane + k * WAVE;
+                if (!(KI * WAVE == NZD || m < NZD)) continue;
+                const size_t off = row + m;
+                const double kzk = skz[m], kz2 = -(kzk * kzk);
+                // 1/lapF as in B (recomputed rather than held through the transforms)
+                const double lap2 = (ky == 0.0 && m == 0) ? 0.0 : (-ky * ky + kz2);
+                const double ilapF = (l == 0 && m == 0) ? 1.0 : 1.0 / (-kx * kx + lap2);
+                const double2 U0 = a.u[off], U1 = a.u[off + NZD];
+                double2 U2 = make_double2(0.0, 0.0), U3 = U2;
+                if constexpr (P::NPAIR == 2) {
+                    U2 = a.u[off + 2 * NZD];
+                    U3 = a.u[off + 3 * NZD];
+                }
+                double2 np[4];
+    #pragma unroll
+                for (int f = 0; f < 2 * P::NPAIR; ++f) np[f] = a.n_prev ? a.n_prev[off + f * NZD] : make_double2(0.0, 0.0);
+                const CC *T = bufc + m;
+    #pragma unroll
+                for (int pr = 0; pr < P::NPAIR; ++pr) {
+                    double2 N0, N1;
+                    P::rhs_pair(pr, T, NZD, U0, U1, U2, U3, kx, ky, kzk, ilapF, ph, N0, N1);
+                    const double Ln = P::lin(pr, kx + qdtl * ky, ky, kz2, ph);
+                    const size_t off0 = off + (2 * pr) * NZD, off1 = off0 + NZD;
+                    if constexpr (MODE == MODE_RHS) {
+                        if (a.n_out) {
+                            a.n_out[off0] = N0;
+                            a.n_out[off1] = N1;
+                        }
+                        if (a.lin_out) a.lin_out[off0] = a.lin_out[off1] = Ln;
+                        continue;
+                    }
+                    const double2 u0 = pr ? U2 : U0, u1 = pr ? U3 : U1;
+                    const double2 p0 = np[2 * pr], p1 = np[2 * pr + 1];
+                    double2 un0, un1;
+                    if constexpr (MODE == MODE_ELIN) {
+                        // RK3CN_Elin.cpp:303-305, :335-337, :368-370
+                        const double e = exp(linC * Ln);
+                        double2 s0 = cscale(nlc1, N0), s1 = cscale(nlc1, N1);
+                        if (a.n_prev) {
+                            s0 = cadd(s0, cscale(nlc2, p0));
+                            s1 = cadd(s1, cscale(nlc2, p1));
+                        }
+                        un0 = cscale(e, cadd(s0, u0));
+                        un1 = cscale(e, cadd(s1, u1));
+                    } else {
+                        // RK3CN.cpp:105-107, :141-143, :177-179; the "old" operator is
+                        // the one of the previous substep (or linearOPs_Init at t0)
+                        double Lo;
+                        const double kxo = rn_add(kx0, rn_mul(qt_old, ky));
+                        if (st.oinit) {
+                            Lo = P::lin_init(pr, kxo, lap2, ph);
+                        } else {
+                            Lo = P::lin(pr, kxo + qdtl_old * ky, ky, kz2, ph);
+                        }
+                        const double denom = 1.0 / (1.0 - linC * Ln);
+                        const double c1 = nlc1 * denom, co = (1 + linC * Lo) * denom;
+                        double2 s0 = cscale(c1, N0), s1 = cscale(c1, N1);
+                        if (a.n_prev) {
+                            const double c2 = nlc2 * denom;
+                            s0 = cadd(s0, cscale(c2, p0));
+                            s1 = cadd(s1, cscale(c2, p1));
+                        }
+                        un0 = cadd(s0, cscale(co, u0));
+                        un1 = cadd(s1, cscale(co, u1));
+                    }
+                    if (hit) un0 = un1 = make_double2(0.0, 0.0);
+                    a.u[off0] = un0;
+                    a.u[off1] = un1;
                     if (a.n_out) {
                         a.n_out[off0] = N0;
                         a.n_out[off1] = N1;
                     }
-                    if (a.lin_out) a.lin_out[off0] = a.lin_out[off1] = Ln;
-                    continue;
-                }
-                const double2 u0 = pr ? U2 : U0, u1 = pr ? U3 : U1;
-                const double2 p0 = np[2 * pr], p1 = np[2 * pr + 1];
-                double2 un0, un1;
-                if constexpr (MODE == MODE_ELIN) {
-                    // RK3CN_Elin.cpp:303-305, :335-337, :368-370
-                    const double e = exp(linC * Ln);
-                    double2 s0 = cscale(nlc1, N0), s1 = cscale(nlc1, N1);
-                    if (a.n_prev) {
-                        s0 = cadd(s0, cscale(nlc2, p0));
-                        s1 = cadd(s1, cscale(nlc2, p1));
-                    }
-                    un0 = cscale(e, cadd(s0, u0));
-                    un1 = cscale(e, cadd(s1, u1));
-                } else {
-                    // RK3CN.cpp:105-107, :141-143, :177-179; the "old" operator is
-                    // the one of the previous substep (or linearOPs_Init at t0)
-                    double Lo;
-                    const double kxo = rn_add(kx0, rn_mul(qt_old, ky));
-                    if (st.oinit) {
-                        Lo = P::lin_init(pr, kxo, lap2, ph);
-                    } else {
-                        Lo = P::lin(pr, kxo + qdtl_old * ky, ky, kz2, ph);
-                    }
-                    const double denom = 1.0 / (1.0 - linC * Ln);
-                    const double c1 = nlc1 * denom, co = (1 + linC * Lo) * denom;
-                    double2 s0 = cscale(c1, N0), s1 = cscale(c1, N1);
-                    if (a.n_prev) {
-                        const double c2 = nlc2 * denom;
-                        s0 = cadd(s0, cscale(c2, p0));
-                        s1 = cadd(s1, cscale(c2, p1));
-                    }
-                    un0 = cadd(s0, cscale(co, u0));
-                    un1 = cadd(s1, cscale(co, u1));
-                }
-                if (hit) un0 = un1 = make_double2(0.0, 0.0);
-                a.u[off0] = un0;
-                a.u[off1] = un1;
-                if (a.n_out) {
-                    a.n_out[off0] = N0;
-                    a.n_out[off1] = N1;
                 }
             }
+            if (hit && lane == 0) a.kx0[i] = rn_sub(kx0, rp.remap_shift);
+        };
+        if constexpr (MIXW) {
+            // mixed precision per mode: the flagged modes in FP64 in this
+            // same launch, the others in FP32, both in the wavefront's buffer
+            if (a.split == SPLIT_LO && a.hi[i])
+                pass(double2{}, reinterpret_cast<double2 *>(buf), tw64, skz, smfz64);
+            else
+                pass(C{}, buf, tw, skzr, smfz);
+        } else {
+            pass(C{}, buf, tw, skzr, smfz);
         }
-        if (hit && lane == 0) a.kx0[i] = rn_sub(kx0, rp.remap_shift);
         // the next mode's B overwrites the transformed products F read
         wave_sync();
     }
@@ -410,6 +442,12 @@ __global__ __launch_bounds__(mode_waves<P>(NZ) * WAVE) __attribute__((amdgpu_wav
 void k_wave32(ModeArgs a)
 {
     mode_wave_body<NZ, P, float2, MODE, mode_waves<P>(NZ), true>(a);
+}
+template <int NZ, class P, int MODE>
+__global__ __launch_bounds__(mode_waves<P>(NZ) * WAVE) __attribute__((amdgpu_waves_per_eu(wave_occ<P>(NZ))))
+void k_wave_mix(ModeArgs a)
+{
+    mode_wave_body<NZ, P, float2, MODE, mode_waves<P>(NZ), true, true>(a);
 }
 
 }  // namespace qlk
