@@ -188,6 +188,10 @@ def bench_mixed(name: str = "box128x128x256", marks=(100, 500, 2000)):
     ctx = {}
     for prec in ("fp64", "mixed"):
         c = QLContext(p.replace(precision=PREC_MIXED) if prec == "mixed" else p)
+        # 16 untimed steps capture the replay graphs; then the same start state
+        c.set_state(lin, mf)
+        c.linear_ops_init(0.0)
+        c.steps(16, 0.0)
         c.set_state(lin, mf)
         c.linear_ops_init(0.0)
         ctx[prec] = c
@@ -396,7 +400,9 @@ def main():
     elin = 1 if args.integrator == "RK3CN_Elin" else 2
     bits = 64 if args.precision == "fp64" else 32
     pol = {"MHD_FullQlin": "ModelFull", "MHD_BQlin": "ModelBQ", "HD_fullU": "ModelHD"}[args.equations]
-    kname = (f"k_wave{bits}<{p.NZ},{pol},{elin}>" if shape.get("line") == 64
+    # (mixed precision: the wavefront kernel that also runs the FP64-flagged modes)
+    kname = ((f"k_wave_mix<{p.NZ},{pol},{elin}>" if bits == 32 else f"k_wave{bits}<{p.NZ},{pol},{elin}>")
+             if shape.get("line") == 64
              else f"k_mode{bits}{'d' if shape.get('transform') else ''}<{p.NZ},{shape.get('modes_per_pass')},{pol},{elin}>")
     out = {
         "metric": metric_name(p, args.workload, args.equations, args.integrator, args.precision),

@@ -215,8 +215,6 @@ int comp_nwg(int nxy)
 // near-equal contiguous ranges
 int configure(Batch &b, Group &g, int line, int target)
 {
-    if (line == LINE_WAVE && !wave_ok(g.NZ, b.precision, b.hd))
-        return fail(QL_ERR_ARG, "line %d (one wavefront per mode) at NZ = 256 needs FP64 (or HD_fullU)", LINE_WAVE);
     if (g.dft && !dft_ok(g.NZ, b.precision, line)) g.dft = 0;  // the shape has no matrix-core variant
     // the tables are rewritten below: nothing queued may still read them
     QL_RC(sync_all(b));
@@ -475,16 +473,12 @@ void launch_wave_m(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
     if (b.precision == QL_PREC_MIXED && a.split == qlk::SPLIT_LO) {
         // flagged modes in FP64, the others in FP32, in one launch
         if (b.hd) launch_k(qlk::k_wave_mix<NZ, qlk::ModelHD, MODE>, g.W, TH, st, tm, a);
-        else if constexpr (NZ <= 128) {  // wave_ok: no other FP32 wavefront kernel at NZ = 256
-            if (bq) launch_k(qlk::k_wave_mix<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
-            else launch_k(qlk::k_wave_mix<NZ, qlk::ModelFull, MODE>, g.W, TF, st, tm, a);
-        }
+        else if (bq) launch_k(qlk::k_wave_mix<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
+        else launch_k(qlk::k_wave_mix<NZ, qlk::ModelFull, MODE>, g.W, TF, st, tm, a);
     } else if (b.precision == QL_PREC_MIXED) {
         if (b.hd) launch_k(qlk::k_wave32<NZ, qlk::ModelHD, MODE>, g.W, TH, st, tm, a);
-        else if constexpr (NZ <= 128) {  // wave_ok: no other FP32 wavefront kernel at NZ = 256
-            if (bq) launch_k(qlk::k_wave32<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
-            else launch_k(qlk::k_wave32<NZ, qlk::ModelFull, MODE>, g.W, TF, st, tm, a);
-        }
+        else if (bq) launch_k(qlk::k_wave32<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
+        else launch_k(qlk::k_wave32<NZ, qlk::ModelFull, MODE>, g.W, TF, st, tm, a);
     } else {
         if (b.hd) launch_k(qlk::k_wave64<NZ, qlk::ModelHD, MODE>, g.W, TH, st, tm, a);
         else if (bq) launch_k(qlk::k_wave64<NZ, qlk::ModelBQ, MODE>, g.W, TB, st, tm, a);
@@ -542,7 +536,6 @@ void launch_mode_t(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipSt
 void launch_mode(const Batch &b, const Group &g, const qlk::ModeArgs &a, hipStream_t st)
 {
     if (g.line == LINE_WAVE) {
-        // configure admits it only where wave_ok
         switch (g.NZ) {
         case 32: launch_wave_t<32>(b, g, a, st); break;
         case 64: launch_wave_t<64>(b, g, a, st); break;
@@ -1243,7 +1236,6 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
         std::vector<std::pair<int, int>> seen;  // (kernel and MB, W) already timed
         for (int cand = 0; cand < 2 * (int)(sizeof(LINES) / sizeof(LINES[0])); ++cand) {
             const int line = LINES[cand / 2], dft = cand % 2;
-            if (line == LINE_WAVE && !wave_ok(g.NZ, b.precision, b.hd)) continue;
             if (dft && !dft_ok(g.NZ, b.precision, line)) continue;
             for (int target : TARGETS) {
                 g.dft = dft;

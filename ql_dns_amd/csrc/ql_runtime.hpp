@@ -46,7 +46,7 @@ constexpr int GRAPH_LEVELS[5] = {GRAPH_STEPS, 8, 4, 2, 1};
 // launch-shape candidates of the tuner: z points per workgroup pass
 // (modes per workgroup MB = line / NZ) and the target workgroup count; line
 // LINE_WAVE selects the kernel with one wavefront per mode (MB = 1,
-// ql_mode_wave.hpp; where wave_ok)
+// ql_mode_wave.hpp)
 constexpr int LINE_WAVE = 64;
 constexpr int LINES[] = {LINE_WAVE, 128, 256};
 constexpr int TARGETS[] = {256, 512, 1024, 2048, 4096, 1 << 30};
@@ -54,18 +54,12 @@ constexpr int DEFAULT_LINE = 256, DEFAULT_TARGET = 512;
 
 inline bool valid_NZ(int NZ) { return NZ == 32 || NZ == 64 || NZ == 128 || NZ == 256; }
 inline int mb_for(int NZ, int line) { return (line == LINE_WAVE || NZ >= line) ? 1 : line / NZ; }
-// where the wavefront kernel exists and is used: every model, at NZ = 256 in
-// FP32 only for HD_fullU's velocity-only policy (the MHD_FullQlin and
-// MHD_BQlin kernels run one wavefront per SIMD there in FP64; the FP32
-// MHD_FullQlin version measured slower than the workgroup kernel: 109 vs
-// 101 us at 128x128x256)
-inline bool wave_ok(int NZ, int precision, bool hd) { return NZ <= 128 || precision == 0 || hd; }
-// shape before tuning: the wavefront kernel where it measured fastest (NZ = 128,
-// and NZ = 256 where it exists), with about one workgroup per CU
-inline int default_line(int NZ, int precision, bool hd)
-{
-    return (NZ >= 128 && wave_ok(NZ, precision, hd)) ? LINE_WAVE : DEFAULT_LINE;
-}
+// shape before tuning: the wavefront kernel, which measured fastest at NZ =
+// 128 and 256 for every model and precision (in mixed precision at NZ = 256 it
+// runs the flagged modes in FP64 itself, which beats the workgroup kernel plus
+// its companion: 2672 against 2625 steps/s at 128x128x256), with about one
+// workgroup per CU
+inline int default_line(int NZ, int, bool) { return NZ >= 128 ? LINE_WAVE : DEFAULT_LINE; }
 inline int default_target(int line) { return line == LINE_WAVE ? 256 : DEFAULT_TARGET; }
 // the matrix-core transform (QL_TRANSFORM_DFT_MFMA): FP64 workgroup kernel, NZ 32 and 64
 inline bool dft_ok(int NZ, int precision, int line)

@@ -6,8 +6,8 @@ relative of the oracle (kx0 after remaps and fixed-step times exactly equal);
 the same run under line 64 and under (256, 512), which sum the stress
 partials in different orders, within 1e-12; one shape bitwise reproducible;
 the mixed path within 1e-6 of FP64.  HD_fullU runs its velocity-only policy
-(ModelHD) there, MHD_BQlin its own (ModelBQ).  MHD_FullQlin's FP32 path has
-no wavefront kernel at NZ = 256 and refuses the shape.
+(ModelHD) there, MHD_BQlin its own (ModelBQ).  In mixed precision the
+wavefront kernel runs the flagged modes in FP64 itself (k_wave_mix).
 """
 import numpy as np
 import pytest
@@ -145,15 +145,37 @@ def test_run_set_matches_oracle_per_run():
 
 
 @pytest.mark.parametrize("model", [0, 1])
-def test_fp32_at_256_refuses_the_wave_shape(model):
-    from ql_dns_amd.capi import QLContext, QLError
+def test_mixed_at_256_takes_the_wave_shape(model):
+    """Mixed precision at NZ = 256 defaults to the wavefront kernel (flagged
+    modes in FP64 in the same launch, k_wave_mix); it agrees with the
+    workgroup shape (companion + FP32 launch) and with FP64 to the FP32
+    bar, and the workgroup shape stays available."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
     from ql_dns_amd.params import PREC_MIXED
-    c = QLContext(_params(model=model, NX=4, NY=4, NZ=256, precision=PREC_MIXED))
-    assert c.config()[0]["line"] != WAVE
-    with pytest.raises(QLError) as e:
-        c.set_config(256, WAVE, 512)
-    assert e.value.code == -1
-    c.set_config(256, 256, 512)
+    p = _params(model=model, NX=8, NY=8, NZ=256, precision=PREC_MIXED)
+    lin, mf = initial_state(p)
+    out = []
+    for line in (None, 256):
+        c = QLContext(p)
+        if line is None:
+            assert c.config()[0]["line"] == WAVE
+        else:
+            c.set_config(256, line, 512)
+        c.set_state(lin, mf)
+        c.linear_ops_init(0.0)
+        c.steps(20, 0.0)
+        out.append(c.get_state())
+        c.close()
+    f = QLContext(p.replace(precision=0))
+    f.set_state(lin, mf)
+    f.linear_ops_init(0.0)
+    f.steps(20, 0.0)
+    ref = f.get_state()
+    f.close()
+    for o in out:
+        assert np.array_equal(o[2], ref[2])
+        assert rel_err(o[1], ref[1]) < 1e-6 and rel_err(o[0], ref[0]) < 1e-6
 
 
 @pytest.mark.parametrize("model", [0, 1, 2])
