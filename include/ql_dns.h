@@ -272,8 +272,12 @@ int ql_batch_set_config(ql_batch *b, int NZ, int line, int target);
 
 /* ------------------------------------------------------------------------
  * Adjoint (reverse-mode gradient) of nsteps of ql_steps, for a single-run
- * MHD_FullQlin, MHD_BQlin or HD_fullU context in FP64 with fixed steps, driven
- * or not (either integrator, remap on or off).  The reference has no adjoint;
+ * MHD_FullQlin, MHD_BQlin or HD_fullU context in FP64, driven or not (either
+ * integrator, remap on or off), with fixed or variable steps.  With variable
+ * steps (dt < 0) the step sizes are those the forward chose and are held
+ * fixed: the gradient is that of the forward with its step sequence frozen
+ * (the CFL condition's own dependence on the state and the parameters is not
+ * differentiated).  The reference has no adjoint;
  * these entry points differentiate the reference's step
  * (Models/MHD_FullQlin.cpp:171-427, Models/MHD_BQlin.cpp:117-306,
  * Integrators/RK3CN.cpp:77-201, Integrators/RK3CN_Elin.cpp:74-188,

@@ -23,11 +23,13 @@ struct AdjState {
     int nsteps = 0, every = 0, nseg = 0;
     bool noise = false;  // the forward was driven
     double t_end = 0.0;
-    // checkpoints at segment starts (index nseg: the final state)
+    // checkpoints at segment starts (index nseg: the final state); with
+    // variable steps also the step taken last and the profile maxima the next
+    // step's dt comes from
     struct Ck {
         double2 *u = nullptr, *mf = nullptr;
-        double *kx0 = nullptr;
-        double t = 0.0, t_prev = 0.0;
+        double *kx0 = nullptr, *pm = nullptr;
+        double t = 0.0, t_prev = 0.0, dt_last = 0.0;
         int old_init = 0;
     };
     std::vector<Ck> ck;
@@ -48,7 +50,7 @@ void adj_release(Batch &b)
 {
     AdjState *A = b.adj;
     if (!A) return;
-    for (auto &c : A->ck) dfree(c.u), dfree(c.mf), dfree(c.kx0);
+    for (auto &c : A->ck) dfree(c.u), dfree(c.mf), dfree(c.kx0), dfree(c.pm);
     dfree(A->st_u), dfree(A->st_mf), dfree(A->st_kx0), dfree(A->st_pre);
     dfree(A->gu), dfree(A->gnp), dfree(A->gM), dfree(A->gnmp);
     dfree(A->gsums), dfree(A->gpart), dfree(A->grows), dfree(A->gpar_mf), dfree(A->gpar), dfree(A->mfz);
@@ -69,7 +71,6 @@ int check_adjoint(const Batch &b, const char *who)
     if (b.model != QL_MODEL_MHD_FULLQLIN && b.model != QL_MODEL_MHD_BQLIN && !b.hd)
         return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint is implemented for MHD_FullQlin, MHD_BQlin and HD_fullU", who);
     if (b.precision != QL_PREC_FP64) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint runs in FP64 only", who);
-    if (b.vdt) return fail(QL_ERR_UNSUPPORTED, "%s: the adjoint needs fixed steps (dt > 0)", who);
     return QL_OK;
 }
 
@@ -81,12 +82,15 @@ int save_ck(Batch &b, AdjState::Ck &c, double t)
         QL_RC(dalloc(&c.u, nl));
         QL_RC(dalloc(&c.mf, nm));
         QL_RC(dalloc(&c.kx0, g.modes));
+        QL_RC(dalloc(&c.pm, (size_t)qlk::PM_STRIDE));
     }
     QL_HIP(hipMemcpyAsync(c.u, g.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s));
     QL_HIP(hipMemcpyAsync(c.mf, g.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s));
     QL_HIP(hipMemcpyAsync(c.kx0, g.kx0, sizeof(double) * g.modes, hipMemcpyDeviceToDevice, b.s));
+    QL_HIP(hipMemcpyAsync(c.pm, g.pm, sizeof(double) * qlk::PM_STRIDE, hipMemcpyDeviceToDevice, b.s));
     c.t = t;
     c.t_prev = b.t_prev;
+    c.dt_last = b.dt_last;
     c.old_init = b.old_init;
     return QL_OK;
 }
@@ -98,28 +102,33 @@ int load_ck(Batch &b, const AdjState::Ck &c)
     QL_HIP(hipMemcpyAsync(g.u, c.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s));
     QL_HIP(hipMemcpyAsync(g.mf, c.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s));
     QL_HIP(hipMemcpyAsync(g.kx0, c.kx0, sizeof(double) * g.modes, hipMemcpyDeviceToDevice, b.s));
+    QL_HIP(hipMemcpyAsync(g.pm, c.pm, sizeof(double) * qlk::PM_STRIDE, hipMemcpyDeviceToDevice, b.s));
     b.t_prev = c.t_prev;
+    b.dt_last = c.dt_last;
     b.old_init = c.old_init;
     return refresh_profiles(b, g, b.s);
 }
 
-// substep times of the forward (sub_times in ql_kernels.hpp, same roundings)
+// substep times of the forward (sub_times in ql_kernels.hpp, same roundings);
+// dt_last, the step before (equal to dt with fixed steps), places the RK3CN
+// "old" operator of substep 1
 struct HostTimes {
     double ts, dtl, tso, dtlo, tnew;
     int oinit;
 };
-HostTimes host_times(double t, double t_prev, int old_init, double t_init, double dt, int s)
+HostTimes host_times(double t, double t_prev, int old_init, double t_init, double dt, double dt_last, int s)
 {
     HostTimes r{};
     const double dtP1 = dt * P1, dtP2 = dt * P2;
     const double dtl1 = dt * P1, dtl2 = dt * (P2 - P1), dtl3 = dt * (1.0 - P2);
+    const double dtP2o = dt_last * P2, dtl3o = dt_last * (1.0 - P2);
     r.tnew = t + dt;
     if (s == 1) {
         r.ts = t;
         r.dtl = dtl1;
         r.oinit = old_init;
-        r.tso = old_init ? t_init : t_prev + dtP2;
-        r.dtlo = old_init ? 0.0 : dtl3;
+        r.tso = old_init ? t_init : t_prev + dtP2o;
+        r.dtlo = old_init ? 0.0 : dtl3o;
     } else if (s == 2) {
         r.ts = t + dtP1;
         r.dtl = dtl2;
@@ -260,13 +269,12 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
     QL_HIP(hipMemsetAsync(A->gpar, 0, sizeof(double) * qlk::NGP, b.s));
 
     const bool cn = (b.integrator == QL_INT_RK3CN);
-    const double dt = b.dt;
     for (int sg = A->nseg - 1; sg >= 0; --sg) {
         const int n = std::min(K, A->nsteps - sg * K);
         // recompute the segment, keeping every substep state
         QL_RC(load_ck(b, A->ck[sg]));
         double t = A->ck[sg].t;
-        std::vector<double> st_t(n), st_tp(n);
+        std::vector<double> st_t(n), st_tp(n), st_dt(n), st_dtl(n);
         std::vector<int> st_oi(n);
         QL_HIP(hipMemcpyAsync(A->st_u, g.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s));
         QL_HIP(hipMemcpyAsync(A->st_mf, g.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s));
@@ -277,6 +285,16 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
             QL_HIP(hipMemcpyAsync(A->st_kx0 + (size_t)k * nxy, g.kx0, sizeof(double) * nxy, hipMemcpyDeviceToDevice,
                                   b.s));
             QL_RC(set_clock(b, t));
+            // with variable steps the device chose the step from the profile
+            // maxima, as in the forward (ql_step); the reverse takes it as given
+            double dt = b.dt;
+            if (b.vdt) {
+                qlk::Clock ck{};
+                QL_RC(read_clock(b, &ck));
+                dt = ck.dt;
+            }
+            st_dt[k] = dt;
+            st_dtl[k] = b.vdt ? b.dt_last : dt;
             for (int s = 1; s <= 3; ++s) {
                 // with noise the step's remap follows the noise (main.cpp:111-123)
                 QL_RC(enqueue_substep(b, g, s, s == 3 && !g.noise, false, b.s));
@@ -290,6 +308,7 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
                 QL_HIP(hipMemcpyAsync(A->st_mf + j * nm, g.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s));
             }
             b.t_prev = t;
+            b.dt_last = dt;
             b.old_init = 0;
             t = t + dt;
         }
@@ -297,7 +316,8 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
         for (int k = n - 1; k >= 0; --k) {
             for (int s = 3; s >= 1; --s) {
                 const size_t jin = (size_t)3 * k + s - 1, jout = jin + 1;
-                const HostTimes ht = host_times(st_t[k], st_tp[k], st_oi[k], A->t_init, dt, s);
+                const double dt = st_dt[k];
+                const HostTimes ht = host_times(st_t[k], st_tp[k], st_oi[k], A->t_init, dt, st_dtl[k], s);
                 double lc, nlc1, nlc2 = 0.0;
                 if (s == 1) {
                     lc = cn ? dt / 2 * (A0) : dt * (A0);
@@ -318,6 +338,7 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
                 pm.mfz = A->mfz;
                 pm.mfn_out = g.mfN1;
                 pm.advance = 0;
+                pm.vdt = pm.rec = 0;  // profiles only: no CFL records, no clock
                 launch_mf(g, pm, b.s);
                 QL_HIP(hipGetLastError());
 

@@ -184,6 +184,7 @@ void dfree(T *&p)
 int sync_all(Batch &b);
 int set_clock(Batch &b, double t);
 int set_clock_runs(Batch &b, const double *t_runs);
+int read_clock(Batch &b, qlk::Clock *c);
 int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, hipStream_t st);
 void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st);
 int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool remap, hipStream_t st);

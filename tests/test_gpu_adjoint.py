@@ -11,6 +11,8 @@ relative 1e-6.
 import numpy as np
 import pytest
 
+from conftest import rel_err
+
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
@@ -126,7 +128,7 @@ def test_adjoint_independent_of_checkpoint_spacing(model):
 
 def test_adjoint_rejects_unsupported():
     from ql_dns_amd.capi import QLContext, QLError
-    for kw in (dict(precision=1), dict(dt=-0.01)):
+    for kw in (dict(precision=1),):
         c = QLContext(_p(NX=4, NY=4, NZ=32, **kw))
         with pytest.raises(QLError):
             c.adjoint_forward(2, 1, 0.0)
@@ -292,3 +294,108 @@ def test_hd_adjoint_matches_finite_differences(integrator, NZ):
               loss(*_run(p.replace(**{name: base - h}), lin0, mf0, n))) / (2 * h)
         assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
 
+
+
+@pytest.mark.parametrize("model,integrator", [(0, 1), (0, 0), (1, 1), (2, 1)])
+def test_variable_steps_at_the_cap_equal_fixed_steps(model, integrator):
+    """Variable steps (dt < 0) whose CFL step always exceeds the cap |dt| take
+    the cap every step: forward and adjoint then equal those of fixed steps
+    of that size (the step sizes come from the device clock, the RK3CN old
+    operator from the step before, the checkpoints restore the profile
+    maxima the next step's size comes from)."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    p = _p(model=model, NX=8, NY=8, NZ=32, Ly=1.0, integrator=integrator, nu=1 / 1500, eta=1 / 1200, dt=0.004)
+    n, every = 14, 5
+    lin0, mf0 = initial_state(p)
+    out = []
+    for q in (p, p.replace(dt=-0.004, CFL=1e3)):
+        c = QLContext(q)
+        c.set_state(lin0, mf0)
+        c.linear_ops_init(0.0)
+        t = c.adjoint_forward(n, every, 0.0)
+        linT, mfT, kxT = c.get_state()
+        g_lin = 2e-3 * linT
+        g_mf = mfT / p.NZ ** 2
+        out.append((t, linT, mfT, kxT, c.adjoint_backward(g_lin, g_mf)))
+        c.close()
+    (ta, la, ma, ka, ga), (tb, lb, mb, kb, gb) = out
+    assert ta == tb and np.array_equal(ka, kb)
+    assert rel_err(lb, la) < 1e-13 and rel_err(mb, ma) < 1e-13
+    for x, y in zip(gb, ga):
+        assert rel_err(np.asarray(x), np.asarray(y)) < 1e-12
+
+
+def _frozen_run(p, lin, mf, dts):
+    """The forward with its step sizes held fixed: step k is one fixed step of
+    size dts[k] (ql_step), then the remap at q (t + dt), as main.cpp:111-123."""
+    from ql_dns_amd.capi import QLContext
+    t, kx0 = 0.0, None
+    for dt in dts:
+        c = QLContext(p.replace(dt=float(dt)))
+        c.set_state(lin, mf, kx0)
+        c.linear_ops_init(t)
+        assert c.step(t) == dt
+        t = t + dt
+        c.remap(p.q * t)
+        lin, mf, kx0 = c.get_state()
+        c.close()
+    return lin, mf
+
+
+def test_variable_steps_adjoint_matches_frozen_step_differences():
+    """Varying CFL steps (RK3CN_Elin): the adjoint differentiates the forward
+    with its step sizes frozen at the ones the forward chose (the step size's
+    own dependence on the state through the CFL condition is not
+    differentiated); central differences of that frozen forward agree within
+    the 1e-6 bar, for state and parameter directions."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    p = _p(NX=8, NY=8, NZ=32, Ly=1.0, B0z=0.03, nu=1 / 1500, eta=1 / 1200, dt=-0.05, CFL=1.0)
+    n, every = 14, 5
+    lin0, mf0 = initial_state(p)
+    alpha = 0.01 / float((np.abs(lin0) ** 2).sum())
+    # the step sizes the device chooses (ql_step equals ql_steps step by step)
+    c = QLContext(p)
+    c.set_state(lin0, mf0)
+    c.linear_ops_init(0.0)
+    t, dts = 0.0, []
+    for _ in range(n):
+        dt = c.step(t)
+        dts.append(dt)
+        t = t + dt
+        c.remap(p.q * t)
+    lin_s, mf_s, _ = c.get_state()
+    c.close()
+    assert len(set(dts)) > 5, "the steps do not vary"
+    c = QLContext(p)
+    c.set_state(lin0, mf0)
+    c.linear_ops_init(0.0)
+    assert c.adjoint_forward(n, every, 0.0) == t
+    linT, mfT, _ = c.get_state()
+    assert np.array_equal(linT, lin_s) and np.array_equal(mfT, mf_s)
+    _, g_lin, g_mf = _loss_parts(p, linT, mfT, alpha)
+    gl0, gm0, gpar = c.adjoint_backward(g_lin, g_mf)
+    c.close()
+    lf, mff = _frozen_run(p, lin0, mf0, dts)
+    assert rel_err(lf, linT) < 1e-13 and rel_err(mff, mfT) < 1e-13
+
+    rng = np.random.default_rng(23)
+    rms_l = np.sqrt(np.mean(np.abs(lin0) ** 2))
+    rms_m = np.sqrt(np.mean(np.abs(mf0) ** 2))
+    for d in range(2):
+        vl = (rng.standard_normal(lin0.shape) + 1j * rng.standard_normal(lin0.shape)) * rms_l
+        vl[::p.NY // 2, 0::2, 0] = 0
+        vm = (rng.standard_normal(mf0.shape) + 1j * rng.standard_normal(mf0.shape)) * rms_m
+        ad = float(np.sum(gl0.real * vl.real + gl0.imag * vl.imag) + np.sum(gm0.real * vm.real + gm0.imag * vm.imag))
+        Lp = _loss_parts(p, *_frozen_run(p, lin0 + EPS * vl, mf0 + EPS * vm, dts), alpha)[0]
+        Lm = _loss_parts(p, *_frozen_run(p, lin0 - EPS * vl, mf0 - EPS * vm, dts), alpha)[0]
+        fd = (Lp - Lm) / (2 * EPS)
+        assert abs(ad - fd) <= TOL * abs(fd), f"state direction {d}: adjoint {ad!r} fd {fd!r}"
+    for k, name in enumerate(PARAMS):
+        base = getattr(p, name)
+        h = EPS * abs(base)
+        Lp = _loss_parts(p, *_frozen_run(p.replace(**{name: base + h}), lin0, mf0, dts), alpha)[0]
+        Lm = _loss_parts(p, *_frozen_run(p.replace(**{name: base - h}), lin0, mf0, dts), alpha)[0]
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"

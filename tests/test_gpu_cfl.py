@@ -130,14 +130,6 @@ def test_tuner_keeps_the_variable_step_state():
     assert rel_err(res[1][1], res[0][1]) < TOL and rel_err(res[1][2], res[0][2]) < TOL
 
 
-def test_adjoint_needs_fixed_steps():
-    from ql_dns_amd.capi import QLContext, QLError
-    g = QLContext(_params(dt=-0.01))
-    with pytest.raises(QLError) as e:
-        g.adjoint_forward(4, 2, 0.0)
-    assert e.value.code == -3
-
-
 # ----------------------------------------------------------------------------
 # Check_Solution
 # ----------------------------------------------------------------------------
