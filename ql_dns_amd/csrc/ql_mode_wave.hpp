@@ -422,8 +422,8 @@ __device__ __forceinline__ void mode_wave_body(const ModeArgs &a)
 
 // MHD_FullQlin (ModelFull), MHD_BQlin (ModelBQ) and HD_fullU (ModelHD); workgroups of
 // mode_waves(NZ) wavefronts: 8 at NZ <= 128 (one workgroup per CU at NZ = 128
-// in FP64: 115 KB of LDS; measured at 64x128x128: 8 wavefronts and 252
-// workgroups 31.1 us, 4 and 504 31.9 us, 2 and 1008 31.6 us and twice the
+// in FP64: 116 KB of LDS; measured at 64x128x128 in an earlier build: 8
+// wavefronts and 252 workgroups 31.1 us, 4 and 504 31.9 us, 2 and 1008 31.6 us and twice the
 // partial rows for the column reduction), 4 at NZ = 256 (26 KB per wavefront
 // in FP64, one wavefront per SIMD with the whole register file)
 // (the velocity-only policy, with 3 lines per transform, keeps 8 at NZ = 256)
