@@ -252,6 +252,15 @@ int ql_batch_tune(ql_batch *b, int trial_steps, double *step_ms);
  * exist.  A later ql_batch_set_config keeps it where the new shape has it. */
 enum { QL_TRANSFORM_FFT = 0, QL_TRANSFORM_DFT_MFMA = 1 };
 int ql_batch_set_transform(ql_batch *b, int NZ, int transform);
+/* Substep tail of the NZ group: the column reduction and the mean-field
+ * kernel as two launches (QL_TAIL_TWO_KERNELS, the default) or one launch with
+ * a workgroup per mean field and run (QL_TAIL_FUSED: fixed steps, NZ <= 128,
+ * at most 65536 / NZ partial rows per run, else QL_ERR_ARG).  Both sum the
+ * stress / EMF partial rows in the same order: the results are bitwise equal.
+ * No counterpart in the reference (its MPI Allreduce, MHD_FullQlin.cpp:386,
+ * followed by the mean-field update, :386-423). */
+enum { QL_TAIL_TWO_KERNELS = 0, QL_TAIL_FUSED = 1 };
+int ql_batch_set_tail(ql_batch *b, int NZ, int tail);
 /* Mixed precision (QL_PREC_MIXED): a mode whose largest |value| reaches tau
  * times the largest of its run runs in FP64 (a companion launch of the FP64
  * kernel before the FP32 one), the others in FP32; the choice is made again
@@ -261,9 +270,10 @@ int ql_batch_set_transform(ql_batch *b, int NZ, int transform);
  * writes to *nfp64 (may be NULL) how many modes of the set reach it for the
  * current state. */
 int ql_batch_mixed_split(ql_batch *b, double tau, int *nfp64);
-/* current shape per group: cfg[6k..6k+5] = NZ, line, target, modes per
- * workgroup pass, workgroups, transform (QL_TRANSFORM_*); *ngroups receives
- * the group count */
+/* current shape per group: cfg[7k..7k+6] = NZ, line, target, modes per
+ * workgroup pass, workgroups, transform (QL_TRANSFORM_*), substep tail in use
+ * (QL_TAIL_*); *ngroups receives the group count */
+enum { QL_CONFIG_INTS = 7 };
 int ql_batch_get_config(const ql_batch *b, int max_groups, int *ngroups, int *cfg);
 /* sets the shape of the NZ group: line in {64, 128, 256}, target >= 1.
  * line 64 selects the kernel with one wavefront per mode (modes per pass 1;

@@ -26,7 +26,8 @@ EXPORTS = ("ql_create", "ql_destroy", "ql_dims", "ql_set_state", "ql_get_state",
            "ql_batch_set_config", "ql_adjoint_forward", "ql_adjoint_backward", "ql_calculate_cfl",
            "ql_check_solution", "ql_batch_calculate_cfl", "ql_batch_check_solution", "ql_set_noise",
            "ql_driving_noise", "ql_batch_set_noise", "ql_batch_driving_noise", "ql_drive_real",
-           "ql_rhs_noalloc", "ql_batch_mixed_split", "ql_batch_set_transform", "ql_batch_steps_runs")
+           "ql_rhs_noalloc", "ql_batch_mixed_split", "ql_batch_set_transform", "ql_batch_steps_runs",
+           "ql_batch_set_tail")
 
 QL_ERR_STATE = -4
 
@@ -107,6 +108,7 @@ def load(path: str = LIB_PATH):
         "ql_batch_set_config": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "ql_batch_mixed_split": ([P, ctypes.c_double, I], ctypes.c_int),
         "ql_batch_set_transform": ([P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "ql_batch_set_tail": ([P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "ql_adjoint_forward": ([P, ctypes.c_int, ctypes.c_int, D], ctypes.c_int),
         "ql_adjoint_backward": ([P, D, D, D, D, D], ctypes.c_int),
         "ql_calculate_cfl": ([P, D], ctypes.c_int),
@@ -159,10 +161,10 @@ def _check_solution(L, fn, args, strict: bool):
 def _get_config(L, h):
     n = ctypes.c_int(0)
     _raise(L, L.ql_batch_get_config(h, 0, ctypes.byref(n), None))
-    cfg = (ctypes.c_int * (6 * max(1, n.value)))()
+    keys = ("NZ", "line", "target", "modes_per_pass", "workgroups", "transform", "tail")
+    cfg = (ctypes.c_int * (len(keys) * max(1, n.value)))()
     _raise(L, L.ql_batch_get_config(h, n.value, ctypes.byref(n), cfg))
-    keys = ("NZ", "line", "target", "modes_per_pass", "workgroups", "transform")
-    return [dict(zip(keys, cfg[6 * k:6 * k + 6])) for k in range(n.value)]
+    return [dict(zip(keys, cfg[len(keys) * k:len(keys) * (k + 1)])) for k in range(n.value)]
 
 
 def _tune(L, h, trial_steps: int) -> float:
@@ -329,6 +331,10 @@ class QLContext:
         """QL_TRANSFORM_FFT (0) or QL_TRANSFORM_DFT_MFMA (1) for the NZ group."""
         self._check(self._L.ql_batch_set_transform(self._L.ql_batch_of(self._h), NZ, transform))
 
+    def set_tail(self, NZ: int, tail: int):
+        """QL_TAIL_TWO_KERNELS (0) or QL_TAIL_FUSED (1) for the NZ group."""
+        self._check(self._L.ql_batch_set_tail(self._L.ql_batch_of(self._h), NZ, tail))
+
     def mixed_split(self, tau: float = 0.0) -> int:
         """Mixed precision: sets the FP64 amplitude fraction when tau > 0 and
         returns how many modes currently reach it (ql_batch_mixed_split)."""
@@ -444,3 +450,6 @@ class QLBatch:
 
     def set_transform(self, NZ: int, transform: int):
         self._check(self._L.ql_batch_set_transform(self._h, NZ, transform))
+
+    def set_tail(self, NZ: int, tail: int):
+        self._check(self._L.ql_batch_set_tail(self._h, NZ, tail))

@@ -305,7 +305,8 @@ int adjoint_backward(Batch &b, const double *g_lin, const double *g_mf, double *
                 }
                 const size_t j = (size_t)3 * k + s;
                 QL_HIP(hipMemcpyAsync(A->st_u + j * nl, g.u, sizeof(double2) * nl, hipMemcpyDeviceToDevice, b.s));
-                QL_HIP(hipMemcpyAsync(A->st_mf + j * nm, g.mf, sizeof(double2) * nm, hipMemcpyDeviceToDevice, b.s));
+                QL_HIP(hipMemcpyAsync(A->st_mf + j * nm, mf_after_substep(b, g, s), sizeof(double2) * nm,
+                                      hipMemcpyDeviceToDevice, b.s));
             }
             b.t_prev = t;
             b.dt_last = dt;

@@ -54,6 +54,7 @@ void release(Batch &b)
     drop_graphs(b);
     for (Group &g : b.groups) {
         dfree(g.u), dfree(g.N1), dfree(g.N2), dfree(g.mf), dfree(g.mfN1), dfree(g.mfN2), dfree(g.drive);
+        dfree(g.mfc1), dfree(g.mfc2);
         dfree(g.tw), dfree(g.tw32), dfree(g.kx0), dfree(g.ky), dfree(g.kzv), dfree(g.mfz), dfree(g.part);
         dfree(g.sums), dfree(g.lin_tmp), dfree(g.rp), dfree(g.wg);
         dfree(g.clock), dfree(g.pm), dfree(g.mode_run), dfree(g.np);
@@ -362,6 +363,8 @@ int create_batch(Batch &b, const ql_params *ps, int nruns)
         QL_RC(dalloc(&g.N1, nl));
         QL_RC(dalloc(&g.N2, nl));
         QL_RC(dalloc(&g.mf, nm));
+        QL_RC(dalloc(&g.mfc1, nm));
+        QL_RC(dalloc(&g.mfc2, nm));
         QL_RC(dalloc(&g.mfN1, nm));
         QL_RC(dalloc(&g.mfN2, nm));
         QL_RC(dalloc(&g.drive, nm));
@@ -693,6 +696,48 @@ int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool
     return QL_OK;
 }
 
+// The substep tail as one launch (k_mf_fused) applies to fixed steps (no CFL
+// records), NZ <= 128 and runs of at most TAIL_MAX_ROWS_NZ / NZ partial rows
+// (each of its workgroups reads one field's columns of every row)
+constexpr long TAIL_MAX_ROWS_NZ = 65536;
+bool fused_tail_fits(const Batch &b, const Group &g)
+{
+    if (b.vdt || g.NZ > 128) return false;
+    for (const qlk::RunParams &rp : g.h_rp)
+        if ((long)(rp.nwg + rp.nwg2) * g.NZ > TAIL_MAX_ROWS_NZ) return false;
+    return true;
+}
+bool fused_tail(const Batch &b, const Group &g) { return g.tail && fused_tail_fits(b, g); }
+
+// the buffer holding the group's mean fields after substep s of a step
+const double2 *mf_after_substep(const Batch &b, const Group &g, int s)
+{
+    if (!fused_tail(b, g) || s == 3) return g.mf;
+    return s == 1 ? g.mfc1 : g.mfc2;
+}
+
+template <int NZ>
+void launch_fused_tail_t(const Group &g, const qlk::MFArgs &m, const double *part, hipStream_t st)
+{
+    const dim3 grid(m.nMF, g.nruns);
+    if (m.model == QL_MODEL_MHD_BQLIN) {
+        if (m.mode == qlk::MODE_CN) qlk::k_mf_fused<NZ, qlk::MODE_CN, 1><<<grid, qlk::MFF_T, 0, st>>>(m, part);
+        else qlk::k_mf_fused<NZ, qlk::MODE_ELIN, 1><<<grid, qlk::MFF_T, 0, st>>>(m, part);
+    } else {
+        if (m.mode == qlk::MODE_CN) qlk::k_mf_fused<NZ, qlk::MODE_CN, 0><<<grid, qlk::MFF_T, 0, st>>>(m, part);
+        else qlk::k_mf_fused<NZ, qlk::MODE_ELIN, 0><<<grid, qlk::MFF_T, 0, st>>>(m, part);
+    }
+}
+
+void launch_fused_tail(const Group &g, const qlk::MFArgs &m, const double *part, hipStream_t st)
+{
+    switch (g.NZ) {
+    case 32: launch_fused_tail_t<32>(g, m, part, st); break;
+    case 64: launch_fused_tail_t<64>(g, m, part, st); break;
+    default: launch_fused_tail_t<128>(g, m, part, st); break;
+    }
+}
+
 // mode kernel -> column reduction of its partials -> mean-field kernel; the
 // partial rows and column sums are a.partials and m.sums (the group's own
 // buffers, or ql_rhs_noalloc's scratch)
@@ -732,11 +777,16 @@ int enqueue_sequence(Batch &b, Group &g, const qlk::ModeArgs &a, const qlk::MFAr
         QL_HIP(hipGetLastError());
     }
     tm_cur = LaunchTiming{};
-    qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(
-        a.partials, g.rp, ncols, const_cast<double *>(m.sums));
-    QL_HIP(hipGetLastError());
-    launch_mf(g, m, st);
-    QL_HIP(hipGetLastError());
+    if (fused_tail(b, g) && (m.mode == qlk::MODE_ELIN || m.mode == qlk::MODE_CN)) {
+        launch_fused_tail(g, m, a.partials, st);
+        QL_HIP(hipGetLastError());
+    } else {
+        qlk::k_reduce_cols<<<dim3((ncols + qlk::RED_CW - 1) / qlk::RED_CW, g.nruns), qlk::NT, 0, st>>>(
+            a.partials, g.rp, ncols, const_cast<double *>(m.sums));
+        QL_HIP(hipGetLastError());
+        launch_mf(g, m, st);
+        QL_HIP(hipGetLastError());
+    }
     if (b.timing) QL_HIP(hipEventRecord(e[2], st));
     return QL_OK;
 }
@@ -795,6 +845,10 @@ int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, h
     m.mfn_out = (s == 2) ? g.mfN2 : g.mfN1;
     m.advance = (advance && s == 3) ? 1 : 0;
     m.s = s;
+    if (fused_tail(b, g)) {
+        m.mf_in = mf_after_substep(b, g, s == 1 ? 3 : s - 1);
+        m.mf = const_cast<double2 *>(mf_after_substep(b, g, s));
+    }
     QL_RC(enqueue_sequence(b, g, a, m, st));
     if (noisy) QL_RC(launch_noise(g, true, 0.0, 0.0, true, remap_after, st));
     return QL_OK;
@@ -1232,7 +1286,7 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
     for (size_t gi = 0; gi < b.groups.size() && rc == QL_OK; ++gi) {
         Group &g = b.groups[gi];
         double best = 1e300;
-        int best_line = g.line, best_target = g.target, best_dft = g.dft;
+        int best_line = g.line, best_target = g.target, best_dft = g.dft, best_tail = g.tail;
         std::vector<std::pair<int, int>> seen;  // (kernel and MB, W) already timed
         for (int cand = 0; cand < 2 * (int)(sizeof(LINES) / sizeof(LINES[0])); ++cand) {
             const int line = LINES[cand / 2], dft = cand % 2;
@@ -1245,32 +1299,40 @@ int tune(Batch &b, int trial_steps, double *best_ms_out)
                 for (auto &sw : seen) dup = dup || (sw.first == kind && sw.second == g.W);
                 if (dup) continue;
                 seen.emplace_back(kind, g.W);
-                for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, false, b.s);
-                if (rc) break;
-                if (hipEventRecord(b.ev[0], b.s) != hipSuccess) {
-                    rc = fail(QL_ERR_HIP, "tune: event record failed");
-                    break;
-                }
-                for (int n = 0; n < trial_steps && rc == QL_OK; ++n)
+                // both substep tails where the one-launch tail applies
+                for (int tail = 0; tail <= 1 && rc == QL_OK; ++tail) {
+                    if (tail && !fused_tail_fits(b, g)) continue;
+                    g.tail = tail;
                     for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, false, b.s);
+                    if (rc) break;
+                    if (hipEventRecord(b.ev[0], b.s) != hipSuccess) {
+                        rc = fail(QL_ERR_HIP, "tune: event record failed");
+                        break;
+                    }
+                    for (int n = 0; n < trial_steps && rc == QL_OK; ++n)
+                        for (int s = 1; s <= 3 && rc == QL_OK; ++s) rc = enqueue_substep(b, g, s, false, false, b.s);
+                    if (rc) break;
+                    float ms = 0.f;
+                    if (hipEventRecord(b.ev[1], b.s) != hipSuccess || hipEventSynchronize(b.ev[1]) != hipSuccess ||
+                        hipEventElapsedTime(&ms, b.ev[0], b.ev[1]) != hipSuccess) {
+                        rc = fail(QL_ERR_HIP, "tune: event timing failed");
+                        break;
+                    }
+                    const double per = ms / trial_steps;
+                    if (per < best) {
+                        best = per;
+                        best_line = line;
+                        best_target = target;
+                        best_dft = dft;
+                        best_tail = tail;
+                    }
+                }
                 if (rc) break;
-                float ms = 0.f;
-                if (hipEventRecord(b.ev[1], b.s) != hipSuccess || hipEventSynchronize(b.ev[1]) != hipSuccess ||
-                    hipEventElapsedTime(&ms, b.ev[0], b.ev[1]) != hipSuccess) {
-                    rc = fail(QL_ERR_HIP, "tune: event timing failed");
-                    break;
-                }
-                const double per = ms / trial_steps;
-                if (per < best) {
-                    best = per;
-                    best_line = line;
-                    best_target = target;
-                    best_dft = dft;
-                }
             }
             if (rc) break;
         }
         g.dft = best_dft;
+        g.tail = best_tail;
         if (rc == QL_OK) rc = configure(b, g, best_line, best_target);
         total_best += best;
     }
@@ -1873,6 +1935,24 @@ int ql_batch_set_transform(ql_batch *bt, int NZ, int transform)
     return fail(QL_ERR_ARG, "ql_batch_set_transform: no group with NZ = %d", NZ);
 }
 
+int ql_batch_set_tail(ql_batch *bt, int NZ, int tail)
+{
+    if (!bt) return fail(QL_ERR_ARG, "ql_batch_set_tail: null handle");
+    Batch &b = bt->b;
+    for (Group &g : b.groups) {
+        if (g.NZ != NZ) continue;
+        if (tail != QL_TAIL_TWO_KERNELS && tail != QL_TAIL_FUSED)
+            return fail(QL_ERR_ARG, "ql_batch_set_tail: unknown tail %d", tail);
+        if (tail == QL_TAIL_FUSED && !fused_tail_fits(b, g))
+            return fail(QL_ERR_ARG,
+                        "ql_batch_set_tail: the one-launch tail needs fixed steps, NZ <= 128 and at most %ld "
+                        "partial rows per run (NZ %d)", TAIL_MAX_ROWS_NZ / g.NZ, g.NZ);
+        g.tail = tail;
+        return configure(b, g, g.line, g.target);
+    }
+    return fail(QL_ERR_ARG, "ql_batch_set_tail: no group with NZ = %d", NZ);
+}
+
 int ql_batch_mixed_split(ql_batch *bt, double tau, int *nfp64)
 {
     if (!bt) return fail(QL_ERR_ARG, "ql_batch_mixed_split: null handle");
@@ -1913,12 +1993,14 @@ int ql_batch_get_config(const ql_batch *bt, int max_groups, int *ngroups, int *c
     if (cfg)
         for (int k = 0; k < max_groups && k < (int)b.groups.size(); ++k) {
             const Group &g = b.groups[k];
-            cfg[6 * k + 0] = g.NZ;
-            cfg[6 * k + 1] = g.line;
-            cfg[6 * k + 2] = g.target;
-            cfg[6 * k + 3] = g.MB;
-            cfg[6 * k + 4] = g.W;
-            cfg[6 * k + 5] = g.dft;
+            int *c = cfg + (size_t)QL_CONFIG_INTS * k;
+            c[0] = g.NZ;
+            c[1] = g.line;
+            c[2] = g.target;
+            c[3] = g.MB;
+            c[4] = g.W;
+            c[5] = g.dft;
+            c[6] = fused_tail(b, g) ? QL_TAIL_FUSED : QL_TAIL_TWO_KERNELS;
         }
     return QL_OK;
 }

@@ -376,7 +376,8 @@ __device__ __forceinline__ double block_sum(double v, double *red)
 // ---------------------------------------------------------------------------
 struct MFArgs {
     const double *sums;       // reduced stress / EMF z-profiles [runs][nMF][NZ]
-    double2 *mf;              // [runs][nMF][nz]
+    double2 *mf;              // [runs][nMF][nz]: the updated mean fields
+    const double2 *mf_in;     // the mean fields at the substep start (null: mf, updated in place)
     double2 *mfn_out;         // mean-field RHS out
     const double2 *mfn_prev;  // previous substep mean-field RHS (nullable)
     double2 *drive;           // [runs][nMF][nz]
@@ -424,6 +425,7 @@ __device__ __forceinline__ void meanfield_load(const MFArgs &a, int run, MFShare
     const int nslot = NMF_OF(MODEL) * NZD, nsum = NMF_OF(MODEL) * NZ;
     const size_t mo = (size_t)run * nslot;
     const bool upd = MODE != MODE_PROFILES;
+    const double2 *mf_in = a.mf_in ? a.mf_in : a.mf;
     double2 t[TW], m[MW], p[MW];
     double su[SW];
 #pragma unroll
@@ -432,7 +434,7 @@ __device__ __forceinline__ void meanfield_load(const MFArgs &a, int run, MFShare
 #pragma unroll
     for (int w = 0; w < MW; ++w) {
         const int k = tid + w * NT;
-        m[w] = (k < nslot) ? a.mf[mo + k] : make_double2(0.0, 0.0);
+        m[w] = (k < nslot) ? mf_in[mo + k] : make_double2(0.0, 0.0);
         p[w] = (upd && k < nslot && a.mfn_prev) ? a.mfn_prev[mo + k] : make_double2(0.0, 0.0);
     }
 #pragma unroll
@@ -455,6 +457,58 @@ __device__ __forceinline__ void meanfield_load(const MFArgs &a, int run, MFShare
 #pragma unroll
     for (int w = 0; w < SW; ++w)
         if (tid + w * NT < nsum) sh.ssum[tid + w * NT] = su[w];
+}
+
+// Slot m of mean field f: its drive, d/dz = i kz of the field's transformed
+// stress / EMF line S (the EMF, MHD_FullQlin.cpp:326-336, and the stresses,
+// accumulated before their z-derivative by the mode kernel; the non-negative
+// half is kept and the negative half filled by conjugate symmetry, :386-399),
+// the mean-field RHS (:411-423, MHD_BQlin.cpp:274-280) from the substep's mean
+// fields M [field][NZD], and for MODE_ELIN / MODE_CN the update
+// (RK3CN_Elin.cpp:109-112 / RK3CN.cpp:112-115) with P, the slot's previous
+// substep RHS (used when has_prev)
+template <int NZD, int HALF, int MODE, int MODEL>
+__device__ __forceinline__ void mf_slot(int f, int m, const double2 *S, const double *skz, const double2 *M,
+                                        const Phys &ph, bool has_prev, double2 P, double linC, double nlc1,
+                                        double nlc2, double2 &Dv, double2 &N, double2 &Mn)
+{
+    // unfused roundings: the one-launch tail and k_meanfield inline this into
+    // different code, where contracted multiply-adds could be chosen differently
+#pragma clang fp contract(off)
+    const bool cj = m >= HALF;
+    const int src = cj ? NZD - m : m;
+    const double2 Dv0 = imul(skz[src], S[src]);
+    Dv = cj ? make_double2(Dv0.x, -Dv0.y) : Dv0;
+    const double kz = skz[m], kz2 = -(kz * kz);
+    N = make_double2(0.0, 0.0);
+    double Lmf = 0.0;
+    if (ph.quasilinear) {
+        if (MODEL == 0) {
+            const double2 M0 = M[m], M1 = M[NZD + m], M2 = M[2 * NZD + m], M3 = M[3 * NZD + m];
+            if (f == 0) N = cadd(imul(ph.B0z * kz, M2), Dv);
+            else if (f == 1) N = cadd(cadd(cscale(-ph.q, M0), imul(ph.B0z * kz, M3)), Dv);
+            else if (f == 2) N = cadd(cadd(cscale(2 * ph.omega, M3), imul(ph.B0z * kz, M0)), Dv);
+            else N = cadd(cadd(cscale(ph.q - 2.0 * ph.omega, M2), imul(ph.B0z * kz, M1)), Dv);
+            Lmf = (f < 2 ? ph.eta : ph.nu) * kz2;
+        } else {
+            if (f == 0) N = Dv;
+            else N = cadd(cscale(-ph.q, M[m]), Dv);
+            Lmf = ph.eta * kz2;
+        }
+    }
+    Mn = make_double2(0.0, 0.0);
+    if (MODE == MODE_RHS) return;
+    const double2 Mo = M[f * NZD + m];
+    if (MODE == MODE_ELIN) {
+        double2 sacc = cscale(nlc1, N);
+        if (has_prev) sacc = cadd(sacc, cscale(nlc2, P));
+        Mn = cscale(exp(linC * Lmf), cadd(sacc, Mo));
+    } else {
+        const double denom = 1.0 / (1.0 - linC * Lmf);
+        double2 sacc = cscale(nlc1 * denom, N);
+        if (has_prev) sacc = cadd(sacc, cscale(nlc2 * denom, P));
+        Mn = cadd(sacc, cscale((1 + linC * Lmf) * denom, Mo));
+    }
 }
 
 // Both transforms are two passes (radix 16 x 8 at NZ = 128) between separate
@@ -522,49 +576,13 @@ __device__ __forceinline__ void meanfield_body(const MFArgs &a, int run, MFShare
             const int k = tid + w * NT;
             if (k >= nslot) continue;
             const int f = k / NZD, m = k % NZD;
-            // d/dz = i kz of every line (the EMF, MHD_FullQlin.cpp:326-336, and
-            // the stresses, accumulated before their z-derivative by the mode
-            // kernel); the non-negative half is kept and the negative half
-            // filled by conjugate symmetry (:386-399)
-            const bool cj = m >= HALF;
-            const int src = cj ? NZD - m : m;
-            const double2 Dv0 = imul(sh.skz[src], sh.line[f * NZ + src]);
-            const double2 Dv = cj ? make_double2(Dv0.x, -Dv0.y) : Dv0;
+            double2 Dv, N, Mn;
+            mf_slot<NZD, HALF, MODE, MODEL>(f, m, sh.line + f * NZ, sh.skz, sh.smf, ph, a.mfn_prev != nullptr,
+                                            sh.sprev[k], linC, nlc1, nlc2, Dv, N, Mn);
             a.drive[mo + k] = Dv;
             if (hd) sh.sdrive[k] = Dv;
-            // mean-field RHS (MHD_FullQlin.cpp:411-423, MHD_BQlin.cpp:274-280)
-            // and update (RK3CN_Elin.cpp:109-112 / RK3CN.cpp:112-115)
-            const double kz = sh.skz[m], kz2 = -(kz * kz);
-            double2 N = make_double2(0.0, 0.0);
-            double Lmf = 0.0;
-            if (ph.quasilinear) {
-                if (MODEL == 0) {
-                    const double2 M0 = sh.smf[m], M1 = sh.smf[NZD + m], M2 = sh.smf[2 * NZD + m], M3 = sh.smf[3 * NZD + m];
-                    if (f == 0) N = cadd(imul(ph.B0z * kz, M2), Dv);
-                    else if (f == 1) N = cadd(cadd(cscale(-ph.q, M0), imul(ph.B0z * kz, M3)), Dv);
-                    else if (f == 2) N = cadd(cadd(cscale(2 * ph.omega, M3), imul(ph.B0z * kz, M0)), Dv);
-                    else N = cadd(cadd(cscale(ph.q - 2.0 * ph.omega, M2), imul(ph.B0z * kz, M1)), Dv);
-                    Lmf = (f < 2 ? ph.eta : ph.nu) * kz2;
-                } else {
-                    if (f == 0) N = Dv;
-                    else N = cadd(cscale(-ph.q, sh.smf[m]), Dv);
-                    Lmf = ph.eta * kz2;
-                }
-            }
             a.mfn_out[mo + k] = N;
             if (MODE == MODE_RHS) continue;
-            const double2 M = sh.smf[k];
-            double2 Mn;
-            if (MODE == MODE_ELIN) {
-                double2 sacc = cscale(nlc1, N);
-                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2, sh.sprev[k]));
-                Mn = cscale(exp(linC * Lmf), cadd(sacc, M));
-            } else {
-                const double denom = 1.0 / (1.0 - linC * Lmf);
-                double2 sacc = cscale(nlc1 * denom, N);
-                if (a.mfn_prev) sacc = cadd(sacc, cscale(nlc2 * denom, sh.sprev[k]));
-                Mn = cadd(sacc, cscale((1 + linC * Lmf) * denom, M));
-            }
             a.mf[mo + k] = Mn;
             sh.smfn[k] = Mn;
         }
@@ -661,6 +679,162 @@ __global__ __launch_bounds__(NT) void k_meanfield(MFArgs a)
     __shared__ MFShared<NZ> sh;
     meanfield_load<NZ, MODE, MODEL>(a, blockIdx.x, sh);
     meanfield_body<NZ, MODE, MODEL, REC>(a, blockIdx.x, sh);
+}
+
+// ---------------------------------------------------------------------------
+// The substep tail in one launch: one workgroup per (mean field f, run), grid
+// (nMF, runs).  It sums its field's NZ columns of the run's partial rows in
+// the order of k_reduce_cols (RED_CH row chunks per column, each summed with
+// Neumaier compensation, the chunks then combined in order: the same bits),
+// transforms the line, forms the field's drive, RHS and update (mf_slot, as
+// k_meanfield) and the field's real-space profile lines.  The couplings read
+// the other fields at the substep start, which other workgroups of the launch
+// update, so the update is written to another buffer than the one read
+// (a.mf_in -> a.mf).  Substep updates with fixed steps and no CFL records only
+// (the profile maxima and HD_fullU's records combine the fields).
+// ---------------------------------------------------------------------------
+constexpr int MFF_T = 1024;          // threads per workgroup
+constexpr int RED_CH = NT / RED_CW;  // row chunks per column, as k_reduce_cols
+
+template <int NZ>
+struct MFFShared {
+    static constexpr int NZD = Dealias<NZ>::nz;
+    double red[RED_CH][NZ], red2[RED_CH][NZ];  // chunk sums and their compensations
+    double ssum[NZ];                           // the field's summed z-profile
+    double2 pad[3 * PadIO<NZ, double2>::LP];   // between the two passes of a transform
+    double2 line[NZ];                          // the field's stress / EMF spectrum
+    double2 tw[NZ];
+    double skz[NZD];
+    double2 smf[4 * NZD];  // every mean field at the substep start
+    double2 sprev[NZD];    // the field's previous substep RHS
+    double2 smfn[NZD];     // the field's updated mean field
+};
+
+template <int NZ, int MODE, int MODEL>
+__global__ __launch_bounds__(MFF_T) void k_mf_fused(MFArgs a, const double *__restrict__ part)
+{
+    static_assert(MODE == MODE_ELIN || MODE == MODE_CN, "substep updates");
+    __shared__ MFFShared<NZ> sh;
+    using D = Dealias<NZ>;
+    using PL = Plan2<NZ>;
+    constexpr int NZD = D::nz, HALF = D::half;
+    constexpr int NPAIR = NZ * RED_CH;                    // (column, chunk) pairs of the field
+    constexpr int NP = (NPAIR + MFF_T - 1) / MFF_T;       // pairs per thread
+    constexpr int B = 16;                                 // rows of a pair in flight
+    static_assert(4 * NZD <= MFF_T && NZ <= MFF_T, "one table entry per thread");
+    const int f = blockIdx.x, run = blockIdx.y, tid = threadIdx.x;
+    const RunParams &rp = a.runs[run];
+    const Phys ph = rp.ph;
+    const int nMF = NMF_OF(MODEL), ncols = nMF * NZ, nslot = nMF * NZD;
+    const size_t mo = (size_t)run * nslot;
+    const double2 *mf_in = a.mf_in ? a.mf_in : a.mf;
+    // tables first, so that their loads are in flight with the rows'
+    const double2 twv = tid < NZ ? a.tw[tid] : make_double2(0.0, 0.0);
+    const double kzv = tid < NZD ? a.kzv[tid] : 0.0;
+    const double2 mv = tid < nslot ? mf_in[mo + tid] : make_double2(0.0, 0.0);
+    const double2 pv = (tid < NZD && a.mfn_prev) ? a.mfn_prev[mo + (size_t)f * NZD + tid] : make_double2(0.0, 0.0);
+    // the field's columns, every pair of the thread advancing together
+    const int nrows = rp.nwg + rp.nwg2, per = (nrows + RED_CH - 1) / RED_CH;
+    double s[NP], e[NP];
+    int r0[NP], r1[NP], col[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int pq = tid + q * MFF_T;
+        const bool act = NPAIR % MFF_T == 0 || pq < NPAIR;
+        const int z = pq % NZ, chunk = pq / NZ;
+        col[q] = f * NZ + z;
+        r0[q] = chunk * per;
+        r1[q] = act ? min(nrows, r0[q] + per) : r0[q];
+        s[q] = e[q] = 0.0;
+    }
+    for (int j = 0; j < per; j += B) {
+        double v[NP][B];
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const int r = r0[q] + j + k;
+                v[q][k] = r < r1[q] ? part[(size_t)run_row(rp, r) * ncols + col[q]] : 0.0;
+            }
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const double t = s[q] + v[q][k];
+                e[q] += (fabs(s[q]) >= fabs(v[q][k])) ? ((s[q] - t) + v[q][k]) : ((v[q][k] - t) + s[q]);
+                s[q] = t;
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int pq = tid + q * MFF_T;
+        if (NPAIR % MFF_T == 0 || pq < NPAIR) {
+            sh.red[pq / NZ][pq % NZ] = s[q];
+            sh.red2[pq / NZ][pq % NZ] = e[q];
+        }
+    }
+    if (tid < NZ) sh.tw[tid] = twv;
+    if (tid < NZD) {
+        sh.skz[tid] = kzv;
+        sh.sprev[tid] = pv;
+    }
+    if (tid < nslot) sh.smf[tid] = mv;
+    if (a.advance && f == 0 && tid == 0) clock_advance(a.clock + run);
+    __syncthreads();
+    if (tid < NZ) {
+        double t = 0.0, te = 0.0;
+        for (int k = 0; k < RED_CH; ++k) {
+            const double v = sh.red[k][tid];
+            const double u = t + v;
+            te += (fabs(t) >= fabs(v)) ? ((t - u) + v) : ((v - u) + t);
+            te += sh.red2[k][tid];
+            t = u;
+        }
+        sh.ssum[tid] = t + te;
+    }
+    __syncthreads();
+    const PadIO<NZ, double2> pad{sh.pad};
+    {
+        const double *S = sh.ssum;
+        const auto ld = [S](int, int j) { return make_double2(S[j], 0.0); };
+        fft_pass_io<NZ, 1, false, PL::R1, 1, false, MFF_T>(sh.tw, ld, pad);
+        fft_pass_io<NZ, 1, false, PL::R2, PL::R1, false, MFF_T>(sh.tw, pad, BufIO<NZ, double2>{sh.line});
+    }
+    if (tid < NZD) {
+        double2 Dv, N, Mn;
+        mf_slot<NZD, HALF, MODE, MODEL>(f, tid, sh.line, sh.skz, sh.smf, ph, a.mfn_prev != nullptr, sh.sprev[tid],
+                                        a.linC, a.nlc1, a.nlc2, Dv, N, Mn);
+        const size_t k = mo + (size_t)f * NZD + tid;
+        a.drive[k] = Dv;
+        a.mfn_out[k] = N;
+        a.mf[k] = Mn;
+        sh.smfn[tid] = Mn;
+    }
+    __syncthreads();
+    // the field's real-space profile lines (k_meanfield's lines 2f, 2f + 1;
+    // MHD_BQlin: By, dz By, dzz By from field 1, and field 0's Bx line, which
+    // only the CFL records read, is not formed)
+    if (MODEL == 0 || f == 1) {
+        constexpr int NL = MODEL == 0 ? 2 : 3;
+        const double fac = a.fac;
+        const auto padded = [&](int l, int j) {
+            double2 v = make_double2(0.0, 0.0);
+            if (D::kept(j)) {
+                const int m = D::slot(j);
+                const double kz = sh.skz[m];
+                const double2 M = sh.smfn[m];
+                if (MODEL == 0) v = (l & 1) ? imul(fac * kz, M) : cscale(fac, M);
+                else if (l == 0) v = cscale(fac, M);
+                else if (l == 1) v = imul(fac * kz, M);
+                else v = cscale(fac * (-(kz * kz)), M);
+            }
+            return v;
+        };
+        double *Z = a.mfz + (size_t)run * 8 * NZ + (size_t)(MODEL == 0 ? 2 * f : 0) * NZ;
+        const auto out = [Z](int l, int i, double2 v) { Z[l * NZ + i] = v.x; };
+        fft_pass_io<NZ, NL, true, PL::R1, 1, false, MFF_T>(sh.tw, padded, pad);
+        fft_pass_io<NZ, NL, true, PL::R2, PL::R1, false, MFF_T>(sh.tw, pad, out);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -88,6 +88,11 @@ struct Group {
     hipEvent_t join = nullptr;
     double2 *u = nullptr, *N1 = nullptr, *N2 = nullptr;
     double2 *mf = nullptr, *mfN1 = nullptr, *mfN2 = nullptr, *drive = nullptr, *tw = nullptr;
+    // the substep tail as one launch (k_mf_fused) where it applies; it writes
+    // the mean fields of substeps 1 and 2 to mfc1, mfc2 and those of substep 3
+    // to mf, so mf holds them at the end of every step
+    int tail = 0;
+    double2 *mfc1 = nullptr, *mfc2 = nullptr;
     float2 *tw32 = nullptr;
     double *kx0 = nullptr, *ky = nullptr, *kzv = nullptr, *mfz = nullptr, *part = nullptr, *sums = nullptr;
     double *lin_tmp = nullptr;
@@ -186,6 +191,8 @@ int set_clock(Batch &b, double t);
 int set_clock_runs(Batch &b, const double *t_runs);
 int read_clock(Batch &b, qlk::Clock *c);
 int enqueue_substep(Batch &b, Group &g, int s, bool remap_after, bool advance, hipStream_t st);
+bool fused_tail(const Batch &b, const Group &g);
+const double2 *mf_after_substep(const Batch &b, const Group &g, int s);
 void launch_mf(const Group &g, const qlk::MFArgs &m, hipStream_t st);
 int launch_noise(Group &g, bool use_clock, double t, double dt, bool noise, bool remap, hipStream_t st);
 qlk::MFArgs mf_args(const Batch &b, const Group &g);
