@@ -425,7 +425,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": avg_ms[0]},
-        "kernels_ms": {kname: avg_ms[0], "k_reduce_cols+k_meanfield": avg_ms[1],
+        "kernels_ms": {kname: avg_ms[0], ("k_mf_fused" if shape.get("tail") else "k_reduce_cols+k_meanfield"): avg_ms[1],
                        "launches_profiled": counts},
         "state_finite": finite,
         "launch_shape": shape,
