@@ -472,9 +472,6 @@ __device__ __forceinline__ void mf_slot(int f, int m, const double2 *S, const do
                                         const Phys &ph, bool has_prev, double2 P, double linC, double nlc1,
                                         double nlc2, double2 &Dv, double2 &N, double2 &Mn)
 {
-    // unfused roundings: the one-launch tail and k_meanfield inline this into
-    // different code, where contracted multiply-adds could be chosen differently
-#pragma clang fp contract(off)
     const bool cj = m >= HALF;
     const int src = cj ? NZD - m : m;
     const double2 Dv0 = imul(skz[src], S[src]);
