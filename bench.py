@@ -299,6 +299,7 @@ def device_warm(p, lin, mf, shape, ms: float):
     w = QLContext(p)
     w.set_state(lin, mf)
     w.set_config(shape["NZ"], shape["line"], shape["target"])
+    w.set_tail(shape["NZ"], shape["tail"])
     w.linear_ops_init(0.0)
     t = w.steps(16, 0.0)  # captures its graphs
     w.synchronize()
@@ -344,6 +345,9 @@ def main():
     ap.add_argument("--transform", choices=["fft", "dft"], default=None,
                     help="z transforms of the workgroup kernel: LDS radix passes or matrix-core products "
                          "(NZ 32 and 64, FP64; implies --no-tune and line 128)")
+    ap.add_argument("--tail", choices=["auto", "two", "one"], default="auto",
+                    help="substep tail: as the tuner (or the default) leaves it, the column reduction and "
+                         "mean-field kernel as two launches, or one launch (k_mf_fused)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -373,6 +377,8 @@ def main():
         ctx.set_transform(p.NZ, 1 if args.transform == "dft" else 0)
     elif not args.no_tune:
         ctx.tune(3)  # launch shape; restores the state
+    if args.tail != "auto":
+        ctx.set_tail(p.NZ, 1 if args.tail == "one" else 0)
     ctx.linear_ops_init(0.0)
     warm = device_warm(p, lin, mf, ctx.config()[0], args.device_warm_ms) if args.device_warm_ms > 0 else None
     t = ctx.steps(args.warmup, 0.0)
