@@ -197,17 +197,19 @@ def test_sensitivity_box_harness_matches_finite_differences():
         assert abs(ad - fd) <= TOL * abs(fd), f"direction {d}: adjoint {ad!r} fd {fd!r}"
 
 
-@pytest.mark.parametrize("model,integrator", [(0, 1), (0, 0), (1, 1)])
+@pytest.mark.parametrize("model,integrator", [(0, 1), (0, 0), (1, 1), (2, 1)])
 def test_adjoint_with_driving_noise_matches_finite_differences(model, integrator):
     """Driven runs (DrivingNoise after every step, then the remap): the noise
     is a function of t, the mode and the slot only, so the same inputs redraw
     the same noise and finite differences apply.  Its ux and bx multipliers
     depend on q through kx(t); the noise is large here (f_noise 40) so that a
     missing q term would show well above the 1e-6 bar, and the test also
-    checks that the noise changes dL/dq."""
+    checks that the noise changes dL/dq.  HD_fullU (model 2): the loss reads
+    its mean flows (Ux, Uy), and dL/deta is zero on both sides."""
     from ql_dns_amd.capi import QLContext
     from ql_dns_amd.ic import initial_state, mode_tables
-    p = _p(model=model, NX=8, NY=8, NZ=32, Ly=1.0, B0z=0.03, integrator=integrator, nu=1 / 1500, eta=1 / 1200)
+    p = _p(model=model, NX=8, NY=8, NZ=32, Ly=1.0, B0z=0.0 if model == 2 else 0.03, integrator=integrator,
+           nu=1 / 1500, eta=1 / 1200)
     noise = dict(f_noise=40.0, seed=0x5EED0011)
     n, every = 14, 5
     lin0, mf0 = initial_state(p)
@@ -232,7 +234,7 @@ def test_adjoint_with_driving_noise_matches_finite_differences(model, integrator
     rms_m = np.sqrt(np.mean(np.abs(mf0) ** 2))
     for d in range(2):
         vl = (rng.standard_normal(lin0.shape) + 1j * rng.standard_normal(lin0.shape)) * rms_l
-        if model == 0:
+        if model in (0, 2):
             vl[::p.NY // 2, 0::2, 0] = 0
         vm = (rng.standard_normal(mf0.shape) + 1j * rng.standard_normal(mf0.shape)) * rms_m
         ad = float(np.sum(gl0.real * vl.real + gl0.imag * vl.imag) + np.sum(gm0.real * vm.real + gm0.imag * vm.imag))
@@ -248,6 +250,9 @@ def test_adjoint_with_driving_noise_matches_finite_differences(model, integrator
         Lp = _loss_parts(p, *_run(p.replace(**{name: base + h}), lin0, mf0, n, noise), alpha)[0]
         Lm = _loss_parts(p, *_run(p.replace(**{name: base - h}), lin0, mf0, n, noise), alpha)[0]
         fd = (Lp - Lm) / (2 * h)
+        if model == 2 and name == "eta":
+            assert fd == 0.0 and gpar[k] == 0.0
+            continue
         assert abs(gpar[k] - fd) <= TOL * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
 
 
