@@ -1,0 +1,89 @@
+"""A seeded sweep of configurations against the oracle: model, integrator,
+box, physics, fixed or variable steps, driving noise and the library's own
+choices (launch shape, z transform, substep tail) drawn together, so that
+combinations no targeted test names are checked too.
+
+Bar as for the other parity tests: states, mean fields, drives and
+diagnostics within 1e-10 relative of the oracle after the steps, kx0 after
+remaps exactly equal, the step times exactly equal with fixed steps and to
+1e-12 with variable (CFL) steps.  A choice the library refuses for a
+configuration (QLError from set_config / set_transform / set_tail) is
+skipped, the rest of the case still runs.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _draw(seed):
+    rng = np.random.default_rng(1000 + seed)
+    model = int(rng.integers(0, 3))
+    NZ = int(rng.choice([32, 64, 128, 256]))
+    NX = int(rng.choice([4, 6, 8, 10, 16]))
+    NY = int(rng.choice([4, 6, 8, 16]))
+    kw = dict(model=model, integrator=int(rng.integers(0, 2)), NX=NX, NY=NY, NZ=NZ,
+              Lx=float(rng.choice([1.0, 2.0])), Ly=float(rng.choice([1.0, 4.0])),
+              q=float(rng.uniform(1.0, 1.8)), viscosity_order=int(rng.choice([2, 2, 4])),
+              nu=float(1.0 / rng.uniform(500, 3000)), eta=float(1.0 / rng.uniform(500, 3000)))
+    if model == 0:
+        kw["B0z"] = float(rng.choice([0.0, 0.03]))
+    vdt = rng.random() < 0.3
+    if vdt:
+        kw.update(dt=-0.02, CFL=0.5)
+    noise = float(rng.choice([0.0, 0.0, 2.0]))
+    if vdt:
+        # a variable step agrees with the oracle's to rounding only, and the
+        # noise is keyed on the bits of the step's start time: driven variable
+        # steps are compared with the oracle replaying the steps instead
+        # (test_gpu_noise.py::test_variable_steps_with_noise_match_oracle)
+        noise = 0.0
+    line = int(rng.choice([64, 128, 256]))
+    target = int(rng.choice([256, 512, 1024]))
+    transform = int(rng.integers(0, 2))
+    tail = int(rng.integers(0, 2))
+    return kw, noise, (line, target), transform, tail
+
+
+@pytest.mark.parametrize("seed", range(120))
+def test_sweep_matches_oracle(seed):
+    from ql_dns_amd.capi import QLContext, QLError
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import QLParams
+    from oracle.oracle import Oracle
+    kw, noise, shape, transform, tail = _draw(seed)
+    p = QLParams(**kw)
+    print(f"seed {seed}: {kw} noise {noise} shape {shape} transform {transform} tail {tail}", flush=True)
+    lin, mf = initial_state(p, seed=seed)
+    o = Oracle.from_params(p, nthreads=8)
+    g = QLContext(p)
+    for call in (lambda: g.set_config(p.NZ, *shape), lambda: g.set_transform(p.NZ, transform),
+                 lambda: g.set_tail(p.NZ, tail)):
+        try:
+            call()
+        except QLError:
+            pass
+    for x in (o, g):
+        if noise:
+            x.set_noise(noise, seed=77 + seed)
+        x.set_state(lin, mf)
+        x.linear_ops_init(0.0)
+    to, tg = o.steps(8, 0.0), g.steps(8, 0.0)
+    if p.dt > 0:
+        assert tg == to
+    else:
+        assert abs(tg - to) <= 1e-12 * to  # CFL steps agree to rounding
+    lo, mo, ko = o.get_state()
+    lg, mg, kg = g.get_state()
+    assert np.array_equal(kg, ko)
+    assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL
+    assert rel_err(g.drive(), o.drive()) < TOL
+    do, dg = o.diagnostics(to), g.diagnostics(tg)
+    fin = np.isfinite(do)
+    assert np.array_equal(fin, np.isfinite(dg))
+    assert np.all(np.abs(dg[fin] - do[fin]) <= TOL * np.maximum(np.abs(do[fin]), 1e-300) + 1e-300)
+    g.close()
