@@ -87,3 +87,52 @@ def test_sweep_matches_oracle(seed):
     assert np.array_equal(fin, np.isfinite(dg))
     assert np.all(np.abs(dg[fin] - do[fin]) <= TOL * np.maximum(np.abs(do[fin]), 1e-300) + 1e-300)
     g.close()
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_run_set_sweep_matches_oracle(seed):
+    """Seeded run sets: 2 to 7 runs of one model, integrator and step, each with
+    its own box, NZ and physics, the library's choices drawn per NZ group;
+    every run against its own oracle."""
+    from ql_dns_amd.capi import QLBatch, QLError
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import QLParams
+    from oracle.oracle import Oracle
+    rng = np.random.default_rng(5000 + seed)
+    model, integrator = int(rng.integers(0, 3)), int(rng.integers(0, 2))
+    runs = []
+    for _ in range(int(rng.integers(2, 8))):
+        kw = dict(model=model, integrator=integrator, NX=int(rng.choice([4, 6, 8, 16])),
+                  NY=int(rng.choice([4, 6, 8, 16])), NZ=int(rng.choice([32, 64, 128])),
+                  Lx=float(rng.choice([1.0, 2.0])), Ly=float(rng.choice([1.0, 4.0])),
+                  q=float(rng.uniform(1.0, 1.8)), viscosity_order=int(rng.choice([2, 2, 4])),
+                  nu=float(1.0 / rng.uniform(500, 3000)), eta=float(1.0 / rng.uniform(500, 3000)))
+        if model == 0:
+            kw["B0z"] = float(rng.choice([0.0, 0.03]))
+        runs.append(QLParams(**kw))
+    print(f"seed {seed}: model {model} integrator {integrator} runs "
+          f"{[(p.NX, p.NY, p.NZ) for p in runs]}", flush=True)
+    st = [initial_state(p, seed=100 * seed + r) for r, p in enumerate(runs)]
+    b = QLBatch(runs)
+    for NZ in sorted({p.NZ for p in runs}):
+        for call in (lambda: b.set_config(NZ, int(rng.choice([64, 128, 256])), int(rng.choice([256, 512, 1024]))),
+                     lambda: b.set_transform(NZ, int(rng.integers(0, 2))),
+                     lambda: b.set_tail(NZ, int(rng.integers(0, 2)))):
+            try:
+                call()
+            except QLError:
+                pass
+    for r, (lin, mf) in enumerate(st):
+        b.set_state(r, lin, mf)
+    b.linear_ops_init(0.0)
+    tb = b.steps(7, 0.0)
+    for r, (p, (lin, mf)) in enumerate(zip(runs, st)):
+        o = Oracle.from_params(p, nthreads=8)
+        o.set_state(lin, mf)
+        o.linear_ops_init(0.0)
+        assert o.steps(7, 0.0) == tb
+        (lo, mo, ko), (lg, mg, kg) = o.get_state(), b.get_state(r)
+        assert np.array_equal(kg, ko), r
+        assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL, r
+        assert rel_err(b.drive(r), o.drive()) < TOL, r
+    b.close()
