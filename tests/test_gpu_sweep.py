@@ -136,3 +136,44 @@ def test_run_set_sweep_matches_oracle(seed):
         assert rel_err(mg, mo) < TOL and rel_err(lg, lo) < TOL, r
         assert rel_err(b.drive(r), o.drive()) < TOL, r
     b.close()
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_mixed_sweep_close_to_fp64(seed):
+    """Seeded mixed-precision draws (model, integrator, box, NZ, shape, the
+    FP64 split on or off) against the FP64 run of the same configuration
+    after 20 steps: mean fields, drives and states within 1e-6, kx0 and times
+    equal."""
+    from ql_dns_amd.capi import QLContext, QLError
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import PREC_MIXED, QLParams
+    rng = np.random.default_rng(9000 + seed)
+    model = int(rng.integers(0, 3))
+    kw = dict(model=model, integrator=int(rng.integers(0, 2)), NX=int(rng.choice([4, 8, 16])),
+              NY=int(rng.choice([4, 8, 16])), NZ=int(rng.choice([32, 64, 128, 256])),
+              Ly=float(rng.choice([1.0, 4.0])), q=float(rng.uniform(1.0, 1.8)), dt=0.005,
+              nu=1 / 4000, eta=1 / 4000)
+    if model == 0:
+        kw["B0z"] = float(rng.choice([0.0, 0.03]))
+    p = QLParams(**kw)
+    shape = (int(rng.choice([64, 128, 256])), int(rng.choice([256, 512, 1024])))
+    tau = float(rng.choice([0.03, 0.1, 2.0]))
+    print(f"seed {seed}: {kw} shape {shape} tau {tau}", flush=True)
+    lin, mf = initial_state(p, seed=seed)
+    out = []
+    for q in (p, p.replace(precision=PREC_MIXED)):
+        c = QLContext(q)
+        if q.precision == PREC_MIXED:
+            try:
+                c.set_config(p.NZ, *shape)
+            except QLError:
+                pass
+            c.mixed_split(tau)
+        c.set_state(lin, mf)
+        c.linear_ops_init(0.0)
+        t = c.steps(20, 0.0)
+        out.append((t,) + tuple(c.get_state()) + (c.drive(),))
+        c.close()
+    (ta, la, ma, ka, da), (tb, lb, mb, kb, db) = out
+    assert ta == tb and np.array_equal(ka, kb)
+    assert rel_err(mb, ma) < 1e-6 and rel_err(db, da) < 1e-6 and rel_err(lb, la) < 1e-6
