@@ -177,3 +177,68 @@ def test_mixed_sweep_close_to_fp64(seed):
     (ta, la, ma, ka, da), (tb, lb, mb, kb, db) = out
     assert ta == tb and np.array_equal(ka, kb)
     assert rel_err(mb, ma) < 1e-6 and rel_err(db, da) < 1e-6 and rel_err(lb, la) < 1e-6
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_adjoint_sweep_matches_finite_differences(seed):
+    """Seeded adjoint draws (model, integrator, NZ, box, noise, checkpoint
+    spacing): directional derivatives of L = 0.5/NZ^2 |mf|^2 + alpha |u|^2
+    along two state directions and dL/dq, dL/dnu within 1e-6 of central
+    differences of the GPU forward."""
+    from ql_dns_amd.capi import QLContext
+    from ql_dns_amd.ic import initial_state
+    from ql_dns_amd.params import QLParams
+    rng = np.random.default_rng(7000 + seed)
+    model = int(rng.integers(0, 3))
+    kw = dict(model=model, integrator=int(rng.integers(0, 2)), NX=int(rng.choice([4, 6, 8])),
+              NY=int(rng.choice([4, 6, 8])), NZ=int(rng.choice([32, 64, 128])), Ly=float(rng.choice([1.0, 4.0])),
+              q=float(rng.uniform(1.0, 1.8)), nu=1 / 1500, eta=1 / 1200)
+    if model == 0:
+        kw["B0z"] = float(rng.choice([0.0, 0.03]))
+    p = QLParams(**kw)
+    noise = dict(f_noise=5.0, seed=300 + seed) if rng.random() < 0.4 else None
+    n, every = 10, int(rng.choice([1, 3, 5, 10]))
+    print(f"seed {seed}: {kw} noise {noise} every {every}", flush=True)
+    lin0, mf0 = initial_state(p, seed=seed)
+    alpha = 0.01 / float((np.abs(lin0) ** 2).sum())
+
+    def run(q, lin, mf):
+        c = QLContext(q)
+        if noise:
+            c.set_noise(**noise)
+        c.set_state(lin, mf)
+        c.linear_ops_init(0.0)
+        c.steps(n, 0.0)
+        lT, mT, _ = c.get_state()
+        c.close()
+        return lT, mT
+
+    def loss(lin, mf):
+        return 0.5 * float((np.abs(mf) ** 2).sum()) / p.NZ ** 2 + alpha * float((np.abs(lin) ** 2).sum())
+
+    c = QLContext(p)
+    if noise:
+        c.set_noise(**noise)
+    c.set_state(lin0, mf0)
+    c.linear_ops_init(0.0)
+    c.adjoint_forward(n, every, 0.0)
+    lT, mT, _ = c.get_state()
+    gl0, gm0, gpar = c.adjoint_backward(2.0 * alpha * lT, mT / p.NZ ** 2)
+    c.close()
+    eps = 1e-6
+    rms_l = np.sqrt(np.mean(np.abs(lin0) ** 2))
+    rms_m = np.sqrt(np.mean(np.abs(mf0) ** 2))
+    for d in range(2):
+        vl = (rng.standard_normal(lin0.shape) + 1j * rng.standard_normal(lin0.shape)) * rms_l
+        if model != 1:
+            vl[::p.NY // 2, 0::2, 0] = 0  # divergence-free directions (ux, bx at ky = kz = 0)
+        vm = (rng.standard_normal(mf0.shape) + 1j * rng.standard_normal(mf0.shape)) * rms_m
+        ad = float(np.sum(gl0.real * vl.real + gl0.imag * vl.imag) + np.sum(gm0.real * vm.real + gm0.imag * vm.imag))
+        fd = (loss(*run(p, lin0 + eps * vl, mf0 + eps * vm)) - loss(*run(p, lin0 - eps * vl, mf0 - eps * vm))) / (2 * eps)
+        assert abs(ad - fd) <= 1e-6 * abs(fd), f"state direction {d}: adjoint {ad!r} fd {fd!r}"
+    for k, name in ((0, "q"), (2, "nu")):
+        base = getattr(p, name)
+        h = eps * abs(base)
+        fd = (loss(*run(p.replace(**{name: base + h}), lin0, mf0)) -
+              loss(*run(p.replace(**{name: base - h}), lin0, mf0))) / (2 * h)
+        assert abs(gpar[k] - fd) <= 1e-6 * abs(fd), f"d/d{name}: adjoint {gpar[k]!r} fd {fd!r}"
