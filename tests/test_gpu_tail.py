@@ -70,6 +70,8 @@ def test_fused_tail_mixed_noise_and_eager():
     from ql_dns_amd.params import PREC_MIXED
     p = _p(NX=32, NY=32, NZ=128, Ly=1.0, precision=PREC_MIXED)
     _equal(_run(p, 1), _run(p, 0))
+    # the workgroup shape: the FP64 companion's partial rows follow the FP32 launch's
+    _equal(_run(p, 1, shape=(256, 512)), _run(p, 0, shape=(256, 512)))
     noise = dict(f_noise=2.0, seed=9)
     q = _p(NX=16, NY=16, NZ=64, Ly=1.0)
     _equal(_run(q, 1, noise=noise), _run(q, 0, noise=noise))
